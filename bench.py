@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the DVR ray march on one MI355X.
+
+Workload (default `--config c1`): 512^3 uint16 nested-shell volume, 6-key TF
+(4096 entries), Phong with central-difference gradients, 1024^2 image, step
+1/1024 (samplingRate 2), 36 orbit views.  One bench step = one orbit (36
+frames).  Volume, TF and all entry/exit images are resident in HBM before the
+timed region; the timed region issues the 36 march launches per step on the
+library's stream and synchronizes.
+
+Prints one JSON line: frames/s (value), Gsamples/s, the roofline of the march
+kernel (algorithmic bytes per launch / HIP-event launch time), and the CPU
+baseline (the FP32 CPURaycaster restatement, no shading, on the host's cores,
+on a bounded band of rows of view 0).
+
+    python bench.py [--steps K] [--warmup W] [--config c1|c0] [--kernel auto|reference|bricked]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked"])
+    ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="also run the plain copy kernel on this many MiB (counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
+    """Time the oracle's CPURaycaster restatement (no shading) on a band of
+    rows at the image centre; scale by the plain sample count of a frame."""
+    from oracle import oracle as O
+    cores = os.cpu_count() or 1
+    h = en.shape[0]
+    rows = 8
+    while True:
+        y0 = max(0, h // 2 - rows // 2)
+        band_e = np.ascontiguousarray(en[y0:y0 + rows])
+        band_x = np.ascontiguousarray(ex[y0:y0 + rows])
+        t0 = time.perf_counter()
+        _, n = O.render(vol, tf, (0.0, 1.0), band_e, band_x, step, nthreads=cores)
+        dt = time.perf_counter() - t0
+        if dt * 2.5 >= target_s or rows >= h:
+            break
+        rows = min(h, max(rows * 2, int(rows * target_s / max(dt, 1e-3) * 0.8)))
+    sps = n / dt
+    return {
+        "value": round(sps / samples_per_frame, 5),
+        "unit": "frames/s",
+        "cores": cores,
+        "kind": "port",
+        "gsamples_per_s": round(sps / 1e9, 5),
+        "sample": (f"rows {y0}..{y0 + rows - 1} of view 0 ({rows}x{en.shape[1]} rays, {n} samples, "
+                   f"{dt:.1f} s), no shading; frames/s = samples/s / plain samples per frame"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    from voreen_amd import hipdvr as H, workloads as W
+
+    cfg = W.CONFIGS[args.config]
+    dims = cfg["dims"]
+    wimg, himg = cfg["image"]
+    nviews = args.views or cfg["views"]
+    kernel = {"auto": H.KERNEL_AUTO, "reference": H.KERNEL_REFERENCE,
+              "bricked": H.KERNEL_BRICKED}[args.kernel]
+    t_setup = time.perf_counter()
+    vol = W.make_volume(args.config)
+    keys, tf = W.make_tf(args.config)
+    step = W.step_for(dims, cfg["rate"])
+    llf, urb = W.world_box(dims)
+    spacing = tuple(float((urb[i] - llf[i]) / dims[i]) for i in range(3))
+    offset = tuple(float(llf[i] + 0.5 * spacing[i]) for i in range(3))
+    vsz = 1 if vol.dtype == np.uint8 else 2
+
+    ctx = H.Context(0)
+    v = ctx.volume(vol, spacing=spacing, offset=offset)
+    t = ctx.transfunc(tf, (0.0, 1.0))
+    img_bytes = wimg * himg * 16
+    cams = [W.orbit_camera(i, nviews) for i in range(nviews)]
+    bufs = []
+    for cam in cams:
+        de, dx, do = ctx.alloc(img_bytes), ctx.alloc(img_bytes), ctx.alloc(img_bytes)
+        ctx.entry_exit_device(cam, llf, urb, wimg, himg, de, dx)
+        bufs.append((de, dx, do))
+    params = [H.default_params(step, shading=cfg["shading"], kernel=kernel,
+                               camera_pos=tuple(cam.eye)) for cam in cams]
+    ctx.synchronize()
+
+    # counted pass (also the first warm-up): per-view sample statistics
+    per_view = []
+    for (de, dx, do), p in zip(bufs, params):
+        per_view.append(ctx.render_device(v, t, de, dx, wimg, himg, p, do, stats=True))
+    samples = sum(s["samples"] for s in per_view)
+    fetched = sum(s["fetched"] for s in per_view)
+    shaded = sum(s["shaded"] for s in per_view)
+    rays = sum(s["rays"] for s in per_view)
+    setup_s = time.perf_counter() - t_setup
+
+    for _ in range(args.warmup):
+        for (de, dx, do), p in zip(bufs, params):
+            ctx.render_device(v, t, de, dx, wimg, himg, p, do)
+    ctx.synchronize()
+
+    ctx.timing_begin(nviews * args.steps)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for (de, dx, do), p in zip(bufs, params):
+            ctx.render_device(v, t, de, dx, wimg, himg, p, do)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    launch_ms = ctx.timing_read()
+    elapsed = t1 - t0
+
+    frames = nviews * args.steps
+    fps = frames / elapsed
+    # algorithmic bytes: 8*sizeof(T) per reconstructed sample, 24*sizeof(T)
+    # more per shaded sample (32-voxel union of the 7 stencils), 48 B per pixel
+    bytes_orbit = fetched * 8 * vsz + shaded * 24 * vsz + 48 * wimg * himg * nviews
+    kernel_s = sum(launch_ms) / 1e3
+    achieved = bytes_orbit * args.steps / kernel_s / 1e9
+    out = {
+        "metric": "frames/s + Gsamples/s at 1024^2, 512^3 u16; learned vs plain at >=45 dB PSNR",
+        "value": round(fps, 4),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.config}: {dims[0]}x{dims[1]}x{dims[2]} "
+                         f"{'u8' if vsz == 1 else 'u16'}, {wimg}x{himg}, step 1/{int(round(1 / step))}, "
+                         f"{'phong' if cfg['shading'] else 'no shading'}, {nviews} orbit views per step"),
+            "kernel": args.kernel,
+            "voxel": "u8" if vsz == 1 else "u16",
+        },
+        "gsamples_per_s": round(samples * args.steps / elapsed / 1e9, 4),
+        "samples_per_frame": samples // nviews,
+        "fetched_per_frame": fetched // nviews,
+        "shaded_per_frame": shaded // nviews,
+        "rays_per_frame": rays // nviews,
+        "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 4),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "setup_s": round(setup_s, 2),
+    }
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_device(b, a, nb)
+        ctx.synchronize()
+        a.free()
+        b.free()
+        out["copy_calibration_bytes"] = nb
+    if not args.no_cpu:
+        de, dx, _ = bufs[0]
+        en = de.download((himg, wimg, 4), np.float32)
+        ex = dx.download((himg, wimg, 4), np.float32)
+        out["cpu_baseline"] = cpu_baseline(vol, keys, tf, en, ex, step, samples / nviews,
+                                           args.cpu_seconds)
+    for bset in bufs:
+        for b in bset:
+            b.free()
+    v.close()
+    t.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,34 @@
+/*
+ * hipdvr_synth.h -- seeded synthetic workloads for the DVR ray march.
+ *
+ * These generators have no counterpart in the reference (Voreen reads real
+ * volumes from disk); they produce the benchmark and test volumes
+ * deterministically from a seed (SplitMix64), on the host, multi-threaded.
+ */
+#ifndef HIPDVR_SYNTH_H
+#define HIPDVR_SYNTH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sum of 8 Gaussian blobs (centres U[0.2,0.8]^3, sigma U[0.05,0.15],
+ * amplitude U[0.3,1.0]) over normalized voxel centres, scaled so the maximum
+ * maps to the type's range (255 / 65535), plus per-voxel uniform noise of
+ * +-2 steps (u8) or +-2*257 (u16), clamped.  dtype: 0 = uint8, 1 = uint16.
+ * out: x-fastest, nx*ny*nz voxels. */
+int hipdvr_synth_blobs(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, int32_t dtype,
+                       void* out);
+
+/* Nested ellipsoid shells (semi-axes 0.62/0.56/0.50 of the unit cube) at 5
+ * densities {0.15, 0.45, 0.70, 0.25, 0.90} from outside in, plus 4-octave fBm
+ * value noise of amplitude 5% of the range; voxels outside the outer shell
+ * (about 30% of the cube) are air, exactly 0.  uint16. */
+int hipdvr_synth_shells(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, uint16_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""ctypes wrapper of the CPU oracle (oracle/libdvr_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py, always as the checker.  The product
+package ``voreen_amd`` never imports it.
+
+The oracle restates Voreen's CPURaycaster
+(voreen/modules/base/processors/render/cpuraycaster.cpp:90-332) and the
+transfer-function, pre-integration and volume-access code it calls; see the
+header of dvr_oracle.c for the per-function reference lines and pinning.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libdvr_oracle.so")
+
+U8, U16 = 0, 1
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+CLASS_TF, CLASS_PREINT = 0, 1
+SHADE_NONE, SHADE_PHONG = 0, 1
+ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
+
+
+class Volume(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("dtype", C.c_int32), ("data", C.c_void_p),
+                ("rwm_scale", C.c_float), ("rwm_offset", C.c_float),
+                ("spacing", C.c_float * 3), ("offset", C.c_float * 3)]
+
+
+class TF(C.Structure):
+    _fields_ = [("rgba8", C.c_void_p), ("width", C.c_int32),
+                ("domain_lo", C.c_float), ("domain_hi", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [("step", C.c_float), ("filter", C.c_int32), ("classification", C.c_int32),
+                ("preint_table", C.c_void_p), ("preint_n", C.c_int32), ("shading", C.c_int32),
+                ("light_pos", C.c_float * 3), ("light_ambient", C.c_float * 3),
+                ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
+                ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
+                ("loop_cap", C.c_int32)]
+
+
+class TFKey(C.Structure):
+    _fields_ = [("intensity", C.c_float), ("split", C.c_int32),
+                ("color_l", C.c_uint8 * 4), ("color_r", C.c_uint8 * 4)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("focus", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy_deg", C.c_float), ("near_dist", C.c_float)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = C.CDLL(_LIB_PATH)
+    P = C.POINTER
+    lib.dvr_splitmix64.argtypes = [P(C.c_uint64)]
+    lib.dvr_splitmix64.restype = C.c_uint64
+    lib.dvr_uniform01.argtypes = [P(C.c_uint64)]
+    lib.dvr_uniform01.restype = C.c_double
+    lib.dvr_tf_build.argtypes = [P(TFKey), C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                 C.c_float, C.c_int32, C.c_void_p]
+    lib.dvr_tf_build.restype = None
+    lib.dvr_opacity_correct.argtypes = [C.c_float, C.c_float]
+    lib.dvr_opacity_correct.restype = C.c_float
+    lib.dvr_preint_build.argtypes = [P(TFKey), C.c_int32, C.c_float, C.c_float, C.c_float,
+                                     C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_void_p]
+    lib.dvr_preint_build.restype = None
+    lib.dvr_entry_exit.argtypes = [P(Camera), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_void_p, C.c_void_p]
+    lib.dvr_entry_exit.restype = None
+    for fn in (lib.dvr_cpu_render, lib.dvr_fp64_render):
+        fn.argtypes = [P(Volume), P(TF), P(Params), C.c_void_p, C.c_void_p, C.c_int32,
+                       C.c_int32, C.c_void_p, C.c_int32]
+        fn.restype = C.c_int64
+    lib.dvr_sample_linear.argtypes = [P(Volume), C.c_float, C.c_float, C.c_float]
+    lib.dvr_sample_linear.restype = C.c_float
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_keys(keys):
+    """keys: list of (intensity, rgba_left[4]) or (intensity, rgba_l, rgba_r)."""
+    arr = (TFKey * max(1, len(keys)))()
+    for i, k in enumerate(keys):
+        arr[i].intensity = float(k[0])
+        cl = k[1]
+        cr = k[2] if len(k) > 2 else k[1]
+        arr[i].split = int(len(k) > 2)
+        for c in range(4):
+            arr[i].color_l[c] = int(cl[c])
+            arr[i].color_r[c] = int(cr[c])
+    return arr, len(keys)
+
+
+def tf_build(keys, width, gamma=1.0, thr=(0.0, 1.0), alpha_mode=ALPHA_USE):
+    arr, n = make_keys(keys)
+    out = np.zeros((width, 4), np.uint8)
+    lib().dvr_tf_build(arr, n, width, gamma, thr[0], thr[1], alpha_mode, _ptr(out))
+    return out
+
+
+def preint_build(keys, n, step, use_integral=False, gamma=1.0, thr=(0.0, 1.0),
+                 alpha_mode=ALPHA_USE):
+    arr, nk = make_keys(keys)
+    out = np.zeros((n, n, 4), np.float32)
+    lib().dvr_preint_build(arr, nk, gamma, thr[0], thr[1], alpha_mode, n, step,
+                           int(use_integral), _ptr(out))
+    return out
+
+
+def opacity_correct(alpha, step):
+    return lib().dvr_opacity_correct(alpha, step)
+
+
+def entry_exit(eye, focus, up, fovy, near, w, h, llf=(0, 0, 0), urb=(1, 1, 1), quantize=False):
+    cam = Camera()
+    cam.eye[:] = [float(v) for v in eye]
+    cam.focus[:] = [float(v) for v in focus]
+    cam.up[:] = [float(v) for v in up]
+    cam.fovy_deg = fovy
+    cam.near_dist = near
+    llf = np.asarray(llf, np.float32)
+    urb = np.asarray(urb, np.float32)
+    en = np.zeros((h, w, 4), np.float32)
+    ex = np.zeros((h, w, 4), np.float32)
+    lib().dvr_entry_exit(C.byref(cam), _ptr(llf), _ptr(urb), w, h, int(quantize), _ptr(en), _ptr(ex))
+    return en, ex
+
+
+def _volume(vol, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0)):
+    vol = np.ascontiguousarray(vol)
+    v = Volume()
+    nz, ny, nx = vol.shape
+    v.dims[:] = [nx, ny, nz]
+    v.dtype = U8 if vol.dtype == np.uint8 else U16
+    v.data = vol.ctypes.data
+    v.rwm_scale, v.rwm_offset = rwm
+    v.spacing[:] = list(spacing)
+    v.offset[:] = list(offset)
+    return v, vol
+
+
+def _params(step, filter_, classification, preint, shading, light):
+    p = Params()
+    p.step = step
+    p.filter = filter_
+    p.classification = classification
+    if preint is not None:
+        p.preint_table = preint.ctypes.data
+        p.preint_n = preint.shape[0]
+    p.shading = shading
+    lt = dict(pos=(2.3, 1.5, 1.5), ambient=(0.4,) * 3, diffuse=(0.8,) * 3, specular=(0.6,) * 3,
+              shininess=60.0, camera=(0.0, 0.0, 3.5))
+    if light:
+        lt.update(light)
+    p.light_pos[:] = list(lt["pos"])
+    p.light_ambient[:] = list(lt["ambient"])
+    p.light_diffuse[:] = list(lt["diffuse"])
+    p.light_specular[:] = list(lt["specular"])
+    p.shininess = lt["shininess"]
+    p.camera_pos[:] = list(lt["camera"])
+    p.loop_cap = 255 * 255
+    return p
+
+
+def render(vol, tf_rgba8, domain, entry, exit_, step, filter_=FILTER_LINEAR,
+           classification=CLASS_TF, preint=None, shading=SHADE_NONE, light=None,
+           rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0),
+           fp64=False, nthreads=0):
+    """Render one frame; vol is a (Z, Y, X) uint8/uint16 array, entry/exit
+    (H, W, 4) float32.  Returns (rgba (H, W, 4), samples)."""
+    v, vol_keep = _volume(vol, rwm, spacing, offset)
+    tf_rgba8 = np.ascontiguousarray(tf_rgba8, np.uint8)
+    t = TF()
+    t.rgba8 = tf_rgba8.ctypes.data
+    t.width = tf_rgba8.shape[0]
+    t.domain_lo, t.domain_hi = domain
+    if preint is not None:
+        preint = np.ascontiguousarray(preint, np.float32)
+    p = _params(step, filter_, classification, preint, shading, light)
+    entry = np.ascontiguousarray(entry, np.float32)
+    exit_ = np.ascontiguousarray(exit_, np.float32)
+    h, w = entry.shape[:2]
+    if fp64:
+        out = np.zeros((h, w, 4), np.float64)
+        n = lib().dvr_fp64_render(C.byref(v), C.byref(t), C.byref(p), _ptr(entry), _ptr(exit_),
+                                  w, h, _ptr(out), nthreads)
+    else:
+        out = np.zeros((h, w, 4), np.float32)
+        n = lib().dvr_cpu_render(C.byref(v), C.byref(t), C.byref(p), _ptr(entry), _ptr(exit_),
+                                 w, h, _ptr(out), nthreads)
+    del vol_keep
+    return out, int(n)
+
+
+def sample_linear(vol, pos):
+    v, keep = _volume(vol)
+    return lib().dvr_sample_linear(C.byref(v), float(pos[0]), float(pos[1]), float(pos[2]))
+
+
+def psnr(a, b):
+    """PSNR = 10*log10(1/MSE) over all RGBA channels of images in [0, 1]."""
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0.0 else 10.0 * np.log10(1.0 / mse)

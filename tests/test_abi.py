@@ -1,0 +1,130 @@
+"""CPU tests of libhipdvr.so: it loads, exports every declared symbol, its
+host-side functions agree with the oracle, and errors come back as codes."""
+import ctypes as C
+import fractions
+import glob
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import fixtures as F  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(hdr).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(hipdvr_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from voreen_amd import hipdvr as H
+    lib = H.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared) == set(H.EXPORTS)
+    assert lib.hipdvr_abi_version() == 1
+
+
+def test_create_on_a_missing_device_returns_an_error():
+    from voreen_amd import hipdvr as H
+    lib = H.load()
+    h = C.c_void_p()
+    rc = lib.hipdvr_create(C.byref(h), 99)
+    assert rc in (H.ERR_INVALID, H.ERR_HIP) and not h
+    assert lib.hipdvr_create(None, 0) == H.ERR_INVALID
+    assert lib.hipdvr_destroy(None) == H.OK
+    assert lib.hipdvr_render(None, None, None, None, None, 1, 1, None, None, None) == H.ERR_INVALID
+
+
+def test_tf_from_keys_matches_oracle(oracle):
+    from voreen_amd import hipdvr as H, workloads as W
+    rng = np.random.default_rng(7)
+    cases = [W.random_keys(s, int(n)) for s, n in zip(range(20), rng.integers(1, 9, 20))]
+    cases += [F.load_tfi("nucleon")["keys"], F.load_tfi("walnut")["keys"], W.shells_keys(5)]
+    cases.append([(0.3, (1, 2, 3, 4), (200, 100, 50, 25)), (0.7, (9, 9, 9, 9), (90, 80, 70, 60))])
+    for keys in cases:
+        for width in (2, 256, 1000, 4096):
+            for kw in (dict(), dict(gamma=0.6), dict(thr=(0.1, 0.85)),
+                       dict(alpha_mode=H.ALPHA_ONE), dict(alpha_mode=H.ALPHA_ZERO)):
+                a = H.tf_from_keys(keys, width, **kw)
+                b = oracle.tf_build(keys, width, **kw)
+                assert np.array_equal(a, b), (keys, width, kw)
+
+
+def test_tf_from_keys_rejects_bad_input():
+    from voreen_amd import hipdvr as H
+    with pytest.raises(H.HipDvrError):
+        H.tf_from_keys([(0.8, (0, 0, 0, 0)), (0.2, (1, 1, 1, 1))], 256)   # unsorted
+    with pytest.raises(H.HipDvrError):
+        H.tf_from_keys([(0.5, (0, 0, 0, 0))], 1)
+
+
+@pytest.mark.parametrize("integral", [False, True])
+def test_preint_build_matches_oracle(oracle, integral):
+    from voreen_amd import hipdvr as H, workloads as W
+    for keys in (W.random_keys(11, 5), F.load_tfi("nucleon")["keys"]):
+        for n, step in ((64, 1 / 128), (256, 1 / 512)):
+            a = H.preint_build(keys, n, step, use_integral=integral)
+            b = oracle.preint_build(keys, n, step, use_integral=integral)
+            assert np.array_equal(a, b)
+
+
+def _round_f32(x):
+    """Correctly rounded (nearest-even) float32 of an exact Fraction."""
+    c = np.float32(float(x))
+    best = None
+    for cand in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+        d = abs(fractions.Fraction(float(cand)) - x)
+        key = (d, int(np.frombuffer(np.float32(cand).tobytes(), np.uint32)[0]) & 1)
+        if best is None or key < best[0]:
+            best = (key, cand)
+    return np.float32(best[1])
+
+
+@pytest.mark.parametrize("d", [255, 65535])
+def test_voxel_normalisation_residual_step_is_exact(d):
+    # The kernels compute v/d as q = v*r; e = fma(-q, d, v); q' = fma(e, r, q)
+    # (march_common.h VoxNorm).  Check it equals the correctly rounded v/d for
+    # every voxel value, with exact rational arithmetic for the fused steps.
+    Fr = fractions.Fraction
+    r = np.float32(1.0) / np.float32(d)
+    fr_r = Fr(float(r))
+    vals = range(0, d + 1) if d == 255 else list(range(0, 4096)) + list(range(4096, d + 1, 7)) + [d]
+    for v in vals:
+        q = np.float32(np.float32(v) * r)
+        e = _round_f32(Fr(v) - Fr(float(q)) * d)
+        q2 = _round_f32(Fr(float(e)) * fr_r + Fr(float(q)))
+        assert q2 == _round_f32(Fr(v, d)), v
+
+
+def test_synth_is_deterministic_and_pinned():
+    from voreen_amd import hipdvr as H, workloads as W
+    a = H.synth_blobs(123, (40, 30, 20), H.U8)
+    b = H.synth_blobs(123, (40, 30, 20), H.U8)
+    assert np.array_equal(a, b) and a.max() >= 250 and a.min() <= 5
+    c = H.synth_blobs(124, (40, 30, 20), H.U8)
+    assert not np.array_equal(a, c)
+    s = W.make_volume("c1", dims=(64, 64, 64))
+    air = np.mean(s == 0)
+    assert 0.25 < air < 0.40
+    assert len(np.unique(s)) > 1000
+
+
+def test_workload_helpers():
+    from voreen_amd import workloads as W
+    assert W.step_for((512, 512, 512), 2.0) == np.float32(1.0) / np.float32(1024.0)
+    llf, urb = W.world_box((128, 96, 114), (0.015625,) * 3)
+    assert np.isclose(urb[0] - llf[0], 1.0) and np.isclose(urb[1] - llf[1], 0.75)
+    keys, tf = W.make_tf("c1")
+    assert tf.shape == (4096, 4) and 0.68 < np.mean(tf[:, 3] == 0) < 0.72

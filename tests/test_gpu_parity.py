@@ -1,0 +1,263 @@
+"""GPU parity: the HIP ray march (through the C-ABI) against the CPU oracle.
+
+Tolerance for the plain path (no shading): max abs RGBA error <= 1e-3 against
+the FP32 CPURaycaster restatement at equal step size; the kernels are written
+to be bit-exact and the tests also report the exact-pixel fraction.  Phong has
+no CPURaycaster counterpart: it is checked against the FP32 restatement of the
+GLSL Phong terms (max abs <= 1e-3 on all but ERT-threshold pixels) and the
+FP64 oracle (PSNR).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import fixtures as F  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def render_both(ctx, oracle, vol, tf, domain, en, ex, step, kernel=None, rwm=(1.0, 0.0),
+                spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0), filter_=1, shading=0,
+                classification=0, preint=None, camera_pos=(0.0, 0.0, 3.5), fp64=False):
+    from voreen_amd import hipdvr as H
+    kernel = H.KERNEL_REFERENCE if kernel is None else kernel
+    v = ctx.volume(vol, rwm=rwm, spacing=spacing, offset=offset)
+    t = ctx.transfunc(tf, domain)
+    if preint is not None:
+        t.set_preint(preint)
+    p = H.default_params(step, filter_=filter_, classification=classification, shading=shading,
+                         kernel=kernel, camera_pos=camera_pos)
+    try:
+        gpu, st = ctx.render(v, t, en, ex, p)
+    finally:
+        v.close()
+        t.close()
+    light = dict(camera=camera_pos)
+    cpu, n = oracle.render(vol, tf, domain, en, ex, step, filter_=filter_,
+                           classification=classification, preint=preint, shading=shading,
+                           light=light, rwm=rwm, spacing=spacing, offset=offset)
+    out = dict(gpu=gpu, cpu=cpu, n=n, stats=st)
+    if fp64:
+        out["fp64"], _ = oracle.render(vol, tf, domain, en, ex, step, filter_=filter_,
+                                       classification=classification, preint=preint,
+                                       shading=shading, light=light, rwm=rwm, spacing=spacing,
+                                       offset=offset, fp64=True)
+    return out
+
+
+def report(name, r, oracle):
+    d = np.abs(r["gpu"] - r["cpu"])
+    exact = float(np.mean(np.all(r["gpu"] == r["cpu"], axis=2)))
+    msg = (f"{name}: max_abs={d.max():.3g} exact_pixels={exact:.4f} "
+           f"psnr_vs_fp32={oracle.psnr(r['gpu'], r['cpu']):.1f}")
+    if "fp64" in r:
+        msg += f" psnr_vs_fp64={oracle.psnr(r['gpu'], r['fp64']):.1f}"
+    print(msg, flush=True)
+    return d
+
+
+def c0_inputs(oracle, size=256, view=0, nviews=1):
+    from voreen_amd import workloads as W
+    vol = W.make_volume("c0")
+    keys, tf = W.make_tf("c0")
+    cam = W.orbit_camera(view, nviews)
+    llf, urb = W.world_box((128, 128, 128))
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, cam.fovy_deg, cam.near_dist, size, size,
+                               llf=llf, urb=urb)
+    return vol, tf, en, ex, W.step_for((128, 128, 128), 2.0)
+
+
+def test_entry_exit_device_matches_oracle(ctx, oracle):
+    from voreen_amd import hipdvr as H, workloads as W
+    cases = [(W.orbit_camera(3, 36), 256, 256, False), (W.orbit_camera(7, 36), 97, 61, True)]
+    inside = H.Camera()
+    inside.eye[:] = [0.1, 0.05, -0.1]
+    inside.focus[:] = [0.0, 0.0, -1.0]
+    inside.up[:] = [0.0, 1.0, 0.0]
+    inside.fovy_deg, inside.near_dist = 50.0, 0.05
+    cases.append((inside, 64, 48, False))
+    llf, urb = W.world_box((128, 96, 114), (0.015625,) * 3)
+    for cam, w, h, q in cases:
+        say(f"entry_exit {w}x{h} quantize={q}")
+        de, dx = ctx.alloc(w * h * 16), ctx.alloc(w * h * 16)
+        ctx.entry_exit_device(cam, llf, urb, w, h, de, dx, quantize=q)
+        ge = de.download((h, w, 4), np.float32)
+        gx = dx.download((h, w, 4), np.float32)
+        de.free()
+        dx.free()
+        ce, cx = oracle.entry_exit(cam.eye, cam.focus, cam.up, cam.fovy_deg, cam.near_dist, w, h,
+                                   llf=llf, urb=urb, quantize=q)
+        assert np.array_equal(ge, ce) and np.array_equal(gx, cx)
+        assert 0.05 < np.mean(ge[..., 3] > 0) <= 1.0
+
+
+def test_c0_full_frame(ctx, oracle):
+    say("c0 128^3 u8 256^2 step 1/256 linear")
+    vol, tf, en, ex, step = c0_inputs(oracle)
+    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, fp64=True)
+    d = report("c0", r, oracle)
+    assert d.max() <= TOL
+    assert r["stats"]["samples"] == r["n"]
+    assert r["stats"]["rays"] == int(np.sum(~np.all((en == 0) & (ex == 0), axis=2)))
+    assert oracle.psnr(r["gpu"], r["fp64"]) > 50.0
+
+
+def test_c0_orbit_views_nearest_and_linear(ctx, oracle):
+    for view in (5, 13, 29):
+        for filt in (0, 1):
+            say(f"c0 view {view}/36 filter {filt} 160^2")
+            vol, tf, en, ex, step = c0_inputs(oracle, size=160, view=view, nviews=36)
+            r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, filter_=filt)
+            assert report(f"c0 v{view} f{filt}", r, oracle).max() <= TOL
+            assert r["stats"]["samples"] == r["n"]
+
+
+def test_nucleon_regression_workspace_setup(ctx, oracle):
+    # modules/base/test/render/cpuraycaster.vws: nucleon, camera :96-103,
+    # canvas 200x200, samplingRate 2, linear filter, TF domain [0, 249]
+    # (real-world units, so rwm scale 255), 12-key TF of :228-270 (== nucleon.tfi).
+    v = F.load_vvd("nucleon")
+    t = F.load_tfi("nucleon")
+    tf = oracle.tf_build(t["keys"], t["width"], t["gamma"], t["threshold"], t["alpha_mode"])
+    sp = np.array(v["spacing"], np.float32)
+    llf = np.array(v["offset"], np.float32) - np.float32(0.5) * sp
+    urb = llf + np.array(v["dims"], np.float32) * sp
+    en, ex = oracle.entry_exit((1.76227701, 1.01016998, 1.77568305),
+                               (-0.023203, 0.019314, 0.012643),
+                               (-0.74486113, 0.22378904, 0.62857014), 45.0, 0.1, 200, 200,
+                               llf=llf, urb=urb, quantize=True)
+    step = float(np.float32(1.0) / (np.float32(41) * np.float32(2.0)))
+    for filt in (1, 0):
+        say(f"nucleon 41^3 u8 200^2 filter {filt}")
+        r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(255.0, 0.0),
+                        filter_=filt, spacing=tuple(sp), offset=v["offset"])
+        assert report(f"nucleon f{filt}", r, oracle).max() <= TOL
+        assert r["stats"]["samples"] == r["n"]
+        assert np.mean(r["gpu"][..., 3] > 0) > 0.05
+
+
+def test_walnut_uint16_noncubic(ctx, oracle):
+    from voreen_amd import workloads as W
+    v = F.load_vvd("walnut")
+    t = F.load_tfi("walnut")
+    tf = oracle.tf_build(t["keys"], t["width"], t["gamma"], t["threshold"], t["alpha_mode"])
+    llf, urb = W.world_box(v["dims"], v["spacing"])
+    for view in (0, 9):
+        say(f"walnut 128x96x114 u16 view {view} 192x144")
+        cam = W.orbit_camera(view, 24, radius=2.5)
+        en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 192, 144, llf=llf, urb=urb)
+        step = W.step_for(v["dims"], 2.0)
+        r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(65535.0, 0.0))
+        assert report(f"walnut v{view}", r, oracle).max() <= TOL
+        assert r["stats"]["samples"] == r["n"]
+        assert np.mean(r["gpu"][..., 3] > 0) > 0.02
+
+
+def test_phong_on_shells(ctx, oracle):
+    from voreen_amd import workloads as W
+    dims = (96, 96, 96)
+    vol = W.make_volume("c1", dims=dims)
+    keys, _ = W.make_tf("c1")
+    tf = oracle.tf_build(keys, 4096)
+    cam = W.orbit_camera(4, 36)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 128, 128, llf=llf, urb=urb)
+    sp = (1.0 / 96,) * 3
+    off = tuple(float(llf[i] + 0.5 / 96) for i in range(3))
+    say("phong shells 96^3 u16 128^2")
+    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, W.step_for(dims, 2.0), shading=1,
+                    spacing=sp, offset=off, camera_pos=tuple(cam.eye), fp64=True)
+    d = report("phong", r, oracle).max(axis=2)
+    assert np.mean(d > TOL) < 0.002
+    assert oracle.psnr(r["gpu"], r["fp64"]) > 45.0
+    assert r["stats"]["shaded"] > 0
+
+
+@pytest.mark.parametrize("integral", [False, True])
+def test_preintegrated_classification(ctx, oracle, integral):
+    vol, tf, en, ex, step = c0_inputs(oracle, size=128)
+    from voreen_amd import workloads as W
+    keys, _ = W.make_tf("c0")
+    table = oracle.preint_build(keys, 256, step, use_integral=integral)
+    say(f"preint integral={integral} 128^2")
+    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, classification=1, preint=table)
+    assert report(f"preint {integral}", r, oracle).max() <= TOL
+
+
+def test_edge_cases(ctx, oracle):
+    from voreen_amd import workloads as W
+    rng = np.random.default_rng(5)
+    keys = W.random_keys(99, 5)
+    tf256 = oracle.tf_build(keys, 256)
+    # degenerate rays, background, ragged image
+    h, w = 23, 37
+    en = rng.uniform(0, 1, (h, w, 4)).astype(np.float32)
+    ex = rng.uniform(0, 1, (h, w, 4)).astype(np.float32)
+    en[..., 3] = ex[..., 3] = 1.0
+    en[0:3] = 0.0
+    ex[0:3] = 0.0                     # background rows
+    ex[5, :] = en[5, :]               # degenerate rays
+    vol = rng.integers(0, 256, (9, 11, 13), dtype=np.uint8)
+    say("edge: ragged 37x23 with background and degenerate rays")
+    r = render_both(ctx, oracle, vol, tf256, (0.0, 1.0), en, ex, 1 / 26)
+    assert report("ragged", r, oracle).max() <= TOL
+    assert np.all(r["gpu"][0:3] == 0.0)
+    # tiny volumes and a 1x1 image
+    for shape in ((1, 1, 1), (5, 3, 2), (2, 1, 7)):
+        v = rng.integers(0, 65536, shape, dtype=np.uint16)
+        e1 = np.array([[[0.1, 0.2, 0.3, 1.0]]], np.float32)
+        x1 = np.array([[[0.9, 0.7, 0.6, 1.0]]], np.float32)
+        say(f"edge: volume {shape} 1x1")
+        r = render_both(ctx, oracle, v, oracle.tf_build(keys, 4096), (0.0, 1.0), e1, x1, 1 / 14)
+        assert report(f"tiny {shape}", r, oracle).max() <= TOL
+        assert r["stats"]["samples"] == r["n"]
+    # loop cap, TF width 2, degenerate domain
+    e1 = np.array([[[0.0, 0.5, 0.5, 1.0]]], np.float32)
+    x1 = np.array([[[1.0, 0.5, 0.5, 1.0]]], np.float32)
+    zero_tf = np.zeros((256, 4), np.uint8)
+    say("edge: loop cap")
+    r = render_both(ctx, oracle, vol, zero_tf, (0.0, 1.0), e1, x1, 1e-6)
+    assert r["stats"]["samples"] == 255 * 255 == r["n"]
+    say("edge: TF width 2, domain lo == hi")
+    tf2 = np.array([[255, 0, 0, 30], [0, 255, 0, 200]], np.uint8)
+    for dom in ((0.0, 1.0), (0.4, 0.4), (0.2, 0.3)):
+        r = render_both(ctx, oracle, vol, tf2, dom, en, ex, 1 / 40)
+        assert report(f"tf2 {dom}", r, oracle).max() <= TOL
+
+
+def test_invalid_arguments_return_codes(ctx):
+    from voreen_amd import hipdvr as H
+    vol = np.zeros((4, 4, 4), np.uint8)
+    v = ctx.volume(vol)
+    t = ctx.transfunc(np.zeros((256, 4), np.uint8))
+    en = np.zeros((2, 2, 4), np.float32)
+    say("invalid arguments")
+    try:
+        with pytest.raises(H.HipDvrError) as e:
+            ctx.render(v, t, en, en, H.default_params(0.0))
+        assert e.value.code == H.ERR_INVALID
+        with pytest.raises(H.HipDvrError) as e:
+            ctx.render(v, t, en, en, H.default_params(0.1, classification=H.CLASS_PREINT))
+        assert e.value.code == H.ERR_INVALID
+        p = H.default_params(0.1)
+        p.compositing = 5
+        with pytest.raises(H.HipDvrError) as e:
+            ctx.render(v, t, en, en, p)
+        assert e.value.code == H.ERR_UNSUPPORTED
+        with pytest.raises(H.HipDvrError):
+            ctx.transfunc(np.zeros((5000, 4), np.uint8))
+        with pytest.raises(H.HipDvrError):
+            ctx.transfunc(np.zeros((256, 4), np.uint8), domain=(1.0, 0.0))
+    finally:
+        v.close()
+        t.close()

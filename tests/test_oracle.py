@@ -1,0 +1,378 @@
+"""Pinning the CPU oracle (oracle/dvr_oracle.c) on the CPU.
+
+Each check is chosen so a plausible mistake fails it: transposed or dropped
+trilinear weights fail the affine-field test, a wrong compositing order or a
+missing opacity correction fails the closed forms, an off-by-one in the loop
+fails the sample-count tests.  Reference lines are relative to voreen/.
+"""
+import hashlib
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import fixtures as F  # noqa: E402
+
+f32 = np.float32
+
+
+def _unit_ray_x(w=1, h=1, y=0.5, z=0.5):
+    en = np.zeros((h, w, 4), np.float32)
+    ex = np.zeros((h, w, 4), np.float32)
+    en[...] = (0.0, y, z, 1.0)
+    ex[...] = (1.0, y, z, 1.0)
+    return en, ex
+
+
+def _const_tf(width, rgba):
+    t = np.zeros((width, 4), np.uint8)
+    t[:] = rgba
+    return t
+
+
+# --------------------------------------------------------------------------
+# trilinear reconstruction, volumeram.cpp:78-97
+# --------------------------------------------------------------------------
+
+def test_trilinear_integer_positions_return_the_voxel(oracle):
+    rng = np.random.default_rng(1)
+    vol = rng.integers(0, 65536, size=(5, 4, 3), dtype=np.uint16)
+    for z in range(5):
+        for y in range(4):
+            for x in range(3):
+                got = oracle.sample_linear(vol, (x, y, z))
+                assert got == f32(vol[z, y, x]) / f32(65535.0)
+
+
+def test_trilinear_reproduces_affine_fields(oracle):
+    # f = 3x + 101y + 1009z + 7 on an 8x7x6 grid: trilinear interpolation is
+    # exact on affine fields; swapping any two weights breaks it.
+    nx, ny, nz = 8, 7, 6
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vol = (3 * x + 101 * y + 1009 * z + 7).astype(np.uint16)
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        p = rng.uniform(0, [nx - 1, ny - 1, nz - 1]).astype(np.float32)
+        exact = (3.0 * p[0] + 101.0 * p[1] + 1009.0 * p[2] + 7.0) / 65535.0
+        got = oracle.sample_linear(vol, p)
+        assert abs(got - exact) <= 2e-6 * max(1.0, exact), (p, got, exact)
+
+
+def test_trilinear_matches_fp64_brute_force(oracle):
+    rng = np.random.default_rng(3)
+    for trial in range(40):
+        nx, ny, nz = rng.integers(1, 6, size=3)
+        vol = rng.integers(0, 256, size=(nz, ny, nx), dtype=np.uint8)
+        for _ in range(20):
+            p = rng.uniform(-0.5, [nx, ny, nz]).astype(np.float32)  # includes out-of-range
+            q = np.clip(p.astype(np.float64), 0, None)
+            lo = np.floor(q).astype(int)
+            fr = q - lo
+            hi = np.minimum(np.ceil(q).astype(int), [nx - 1, ny - 1, nz - 1])
+            lo = np.minimum(lo, [nx - 1, ny - 1, nz - 1])
+            ref = 0.0
+            for cz in (0, 1):
+                for cy in (0, 1):
+                    for cx in (0, 1):
+                        wgt = ((fr[0] if cx else 1 - fr[0]) * (fr[1] if cy else 1 - fr[1]) *
+                               (fr[2] if cz else 1 - fr[2]))
+                        ix = hi[0] if cx else lo[0]
+                        iy = hi[1] if cy else lo[1]
+                        iz = hi[2] if cz else lo[2]
+                        ref += wgt * vol[iz, iy, ix] / 255.0
+            got = oracle.sample_linear(vol, p)
+            assert abs(got - ref) < 1e-6, (trial, p, got, ref)
+
+
+# --------------------------------------------------------------------------
+# transfer function table, transfunc1d.cpp:85-116, transfunc1dkeys.cpp:119-154
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("width", [256, 1024, 4096])
+def test_standard_ramp_closed_form(oracle, width):
+    # setToStandardFunc (transfunc1dkeys.cpp:413-421): (0, rgba 0) .. (1, rgba 255)
+    tf = oracle.tf_build([(0.0, (0, 0, 0, 0)), (1.0, (255, 255, 255, 255))], width)
+    exact = 255.0 * np.arange(width) / (width - 1)
+    fl = np.floor(exact).astype(int)
+    got = tf[:, 0].astype(int)
+    # byte truncation: the float path may land one ulp below an integer
+    assert np.all((got == fl) | ((got == fl - 1) & (exact == np.round(exact))))
+    assert got[0] == 0 and got[-1] == 255
+    assert np.all(np.diff(got) >= 0)
+    assert np.array_equal(tf[:, 0], tf[:, 3])
+
+
+def test_piecewise_linear_between_keys(oracle):
+    keys = [(0.2, (10, 200, 30, 0)), (0.5, (250, 0, 130, 200)), (0.9, (0, 100, 255, 40))]
+    width = 1024
+    tf = oracle.tf_build(keys, width).astype(int)
+    xs = np.arange(width) / (width - 1)
+    for c in range(4):
+        ref = np.interp(xs, [k[0] for k in keys], [k[1][c] for k in keys])
+        assert np.all(np.abs(tf[:, c] - ref) <= 1.0 + 1e-9)
+        assert np.all(tf[:, c] <= np.floor(ref + 1e-4) + 0)  # truncation never rounds up
+    assert np.all(tf[xs <= 0.2] == np.array(keys[0][1]))
+    assert np.all(tf[xs > 0.9] == np.array(keys[-1][1]))
+
+
+def test_split_key_and_constant(oracle):
+    keys = [(0.5, (255, 0, 0, 100), (0, 0, 255, 200))]
+    tf = oracle.tf_build(keys, 256)
+    x = np.arange(256) / 255.0
+    assert np.all(tf[x <= 0.5] == (255, 0, 0, 100))     # left of the only key: colorL
+    assert np.all(tf[x > 0.5] == (0, 0, 255, 200))      # right of it: colorR
+    const = oracle.tf_build([(0.3, (7, 8, 9, 10))], 300)
+    assert np.all(const == (7, 8, 9, 10))
+
+
+def test_threshold_and_alpha_modes(oracle):
+    keys = [(0.0, (0, 0, 0, 0)), (1.0, (255, 255, 255, 255))]
+    tf = oracle.tf_build(keys, 256, thr=(0.25, 0.75))
+    lo, hi = int(np.floor(0.25 * 256 + 0.5)), int(np.floor(0.75 * 256 + 0.5))
+    idx = np.arange(256)
+    assert np.all(tf[(idx < lo) | (idx > hi)] == 0)
+    assert np.all(tf[lo:hi + 1, 0] > 0)
+    one = oracle.tf_build(keys, 256, alpha_mode=oracle.ALPHA_ONE)
+    assert one[0, 3] == 0 and np.all(one[1:, 3] == 255)
+    zero = oracle.tf_build(keys, 256, alpha_mode=oracle.ALPHA_ZERO)
+    assert np.all(zero[:, 3] == 0)
+
+
+def test_opacity_correction_overload_is_immaterial(oracle):
+    # cpuraycaster.cpp:291 passes floats to pow; whether the float or the
+    # double overload is picked must not change any table entry at the steps
+    # the workloads use.
+    for step in (1 / 256, 1 / 1024, 1 / 4096, 1 / 82, 1 / 256 / 4):
+        e = f32(step) * f32(200.0)
+        for k in range(256):
+            a = f32(k) / f32(255.0)
+            viaf = oracle.opacity_correct(float(a), float(f32(step)))
+            viad = f32(1.0) - f32(math.pow(float(f32(1.0) - a), float(e)))
+            assert viaf == viad, (step, k)
+
+
+# --------------------------------------------------------------------------
+# ray march closed forms, cpuraycaster.cpp:210-326
+# --------------------------------------------------------------------------
+
+def test_sample_count_of_a_full_ray(oracle):
+    vol = np.zeros((4, 4, 4), np.uint8)
+    tf = _const_tf(256, (0, 0, 0, 0))
+    en, ex = _unit_ray_x()
+    _, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 256)
+    assert n == 257                      # floor(tend/step) + 1 with tend = 1 exactly
+    _, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 100)
+    t, k = f32(0), 0                     # float accumulation of t (cpuraycaster.cpp:315)
+    while True:
+        k += 1
+        t = f32(t + f32(1 / 100))
+        if t > f32(1.0):
+            break
+    assert n == k and abs(n - 101) <= 1
+
+
+def test_constant_volume_closed_form_and_ert(oracle):
+    vol = np.full((8, 8, 8), 77, np.uint8)
+    for alpha8, rgb in ((3, (200, 100, 50)), (40, (10, 250, 128))):
+        tf = _const_tf(256, rgb + (alpha8,))
+        step = 1 / 256
+        en, ex = _unit_ray_x()
+        img, n = oracle.render(vol, tf, (0, 1), en, ex, step)
+        a = alpha8 / 255.0
+        ac = 1 - (1 - a) ** (step * 200)
+        k = np.arange(1, 258)
+        A = 1 - (1 - ac) ** k
+        hit = np.nonzero(A >= 0.95)[0]
+        n_exp = int(hit[0]) + 1 if hit.size else 257
+        assert n == n_exp
+        A_n = A[n_exp - 1]
+        if hit.size:
+            assert img[0, 0, 3] == 1.0                      # ERT sets alpha to 1
+        else:
+            assert abs(img[0, 0, 3] - A_n) < 1e-5
+        for c in range(3):
+            assert abs(img[0, 0, c] - rgb[c] / 255.0 * A_n) < 1e-5
+
+
+def test_degenerate_ray_takes_one_sample(oracle):
+    vol = np.full((4, 4, 4), 200, np.uint8)
+    tf = _const_tf(256, (255, 0, 0, 128))
+    en = np.array([[[0.3, 0.4, 0.5, 1.0]]], np.float32)
+    img, n = oracle.render(vol, tf, (0, 1), en, en.copy(), 1 / 64)
+    assert n == 1
+    ac = 1 - (1 - 128 / 255) ** (200 / 64)
+    assert abs(img[0, 0, 3] - ac) < 1e-6 and abs(img[0, 0, 0] - ac) < 1e-6
+
+
+def test_background_pixels_are_zero(oracle):
+    vol = np.full((4, 4, 4), 255, np.uint8)
+    tf = _const_tf(256, (255, 255, 255, 255))
+    en = np.zeros((3, 3, 4), np.float32)
+    ex = np.zeros((3, 3, 4), np.float32)
+    en[1, 1] = ex[1, 1] = (0.5, 0.5, 0.5, 1.0)      # only the centre pixel is marched
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 16)
+    assert n == 1
+    mask = np.ones((3, 3), bool)
+    mask[1, 1] = False
+    assert np.all(img[mask] == 0.0) and img[1, 1, 3] == 1.0
+
+
+def test_first_hit_of_an_opaque_slab(oracle):
+    # voxels with x >= 10 are 255, the rest 0; only TF entry 255 is opaque.
+    vol = np.zeros((4, 4, 32), np.uint8)
+    vol[:, :, 10:] = 255
+    tf = _const_tf(256, (0, 0, 0, 0))
+    tf[255] = (30, 60, 90, 255)
+    en, ex = _unit_ray_x()
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 64)
+    assert np.allclose(img[0, 0], [30 / 255, 60 / 255, 90 / 255, 1.0], atol=1e-7)
+    # the first sample whose trilinear value rounds to index 255: voxel x >= 9 + 254.5/255
+    t_hit = ((9 + 254.5 / 255) + 0.5) / 32
+    assert abs((n - 1) - math.ceil(t_hit * 64)) <= 1
+
+
+def test_rwm_and_domain_mapping(oracle):
+    # rw = n*scale + offset, then (rw - lo)/(hi - lo) with clamps
+    # (realworldmappingmetadata.cpp:65-67, transfuncbase.cpp:244-259)
+    vol = np.full((4, 4, 4), 51, np.uint8)        # n = 0.2
+    tf = np.zeros((256, 4), np.uint8)
+    tf[:, 3] = 255
+    tf[:, 0] = np.arange(256)
+    en, ex = _unit_ray_x()
+    img, _ = oracle.render(vol, tf, (0.0, 100.0), en, ex, 1 / 16, rwm=(255.0, 10.0))
+    idx = math.floor((51.0 + 10.0) / 100.0 * 255 + 0.5)
+    assert abs(img[0, 0, 0] - idx / 255.0) < 1e-7
+    img, _ = oracle.render(vol, tf, (0.5, 0.5), en, ex, 1 / 16)       # lo == hi -> 0
+    assert img[0, 0, 0] == 0.0
+    img, _ = oracle.render(vol, tf, (0.0, 0.1), en, ex, 1 / 16)       # rw > hi -> 1
+    assert abs(img[0, 0, 0] - 1.0) < 1e-7
+
+
+def test_loop_cap(oracle):
+    vol = np.zeros((2, 2, 2), np.uint8)
+    tf = _const_tf(256, (0, 0, 0, 0))
+    en, ex = _unit_ray_x()
+    _, n = oracle.render(vol, tf, (0, 1), en, ex, 1e-6)
+    assert n == 255 * 255
+
+
+def test_nearest_filter_picks_iround(oracle):
+    vol = np.zeros((1, 1, 8), np.uint8)
+    vol[0, 0, :] = np.arange(8) * 30
+    tf = np.zeros((256, 4), np.uint8)
+    tf[:, 0] = np.arange(256)
+    tf[:, 3] = 255
+    for xv in (0.2, 2.49, 2.5, 6.9):
+        s = (xv + 0.5) / 8
+        en = np.array([[[s, 0.5, 0.5, 1.0]]], np.float32)
+        img, _ = oracle.render(vol, tf, (0, 1), en, en.copy(), 1 / 8, filter_=oracle.FILTER_NEAREST)
+        vx = np.float32(np.float32(8) * np.float32(s)) - np.float32(0.5)
+        assert abs(img[0, 0, 0] - vol[0, 0, int(math.floor(vx + 0.5))] / 255.0) < 1e-7
+
+
+def test_fp64_envelope_on_blobs(oracle):
+    from voreen_amd import workloads as W
+    vol = W.make_volume("c0", dims=(48, 40, 36))
+    keys, tf = W.make_tf("c0")
+    en, ex = oracle.entry_exit((0.4, 0.5, 3.0), (0.5, 0.5, 0.5), (0, 1, 0), 30.0, 0.1, 64, 64)
+    img32, n32 = oracle.render(vol, tf, (0, 1), en, ex, 1 / 96)
+    img64, n64 = oracle.render(vol, tf, (0, 1), en, ex, 1 / 96, fp64=True)
+    assert oracle.psnr(img32, img64) > 60.0
+    assert np.mean(np.abs(img32 - img64).max(axis=2) > 1e-3) < 0.02
+
+
+# --------------------------------------------------------------------------
+# pre-integration, preintegrationtable.cpp:67-204
+# --------------------------------------------------------------------------
+
+def test_preint_constant_tf_closed_forms(oracle):
+    keys = [(0.5, (100, 150, 200, 60))]
+    step = 1 / 128
+    n = 64
+    a = 60 / 255.0
+    ac = 1 - (1 - a) ** (step * 200)
+    exact = oracle.preint_build(keys, n, step)
+    assert np.allclose(exact[..., 3], ac, atol=2e-6)
+    assert np.allclose(exact[..., :3], np.array([100, 150, 200]) / 255.0, atol=2e-5)
+    integ = oracle.preint_build(keys, n, step, use_integral=True)
+    ai = 1 - math.exp(-a * 200 * step)
+    off = ~np.eye(n, dtype=bool)
+    assert np.allclose(integ[off][:, 3], ai, atol=2e-6)
+    rgb = np.array([100, 150, 200]) / 255.0 * a * 200 * step / ai
+    assert np.allclose(integ[off][:, :3], np.clip(rgb, 0, 1), atol=2e-5)
+    assert np.allclose(np.diagonal(integ[..., 3]), ac, atol=2e-6)
+
+
+# --------------------------------------------------------------------------
+# entry/exit points of the unit cube
+# --------------------------------------------------------------------------
+
+def test_entry_exit_geometry(oracle):
+    w, h = 48, 40
+    eye, focus, up = (1.7, 1.1, 2.3), (0.5, 0.45, 0.5), (0.0, 1.0, 0.2)
+    en, ex = oracle.entry_exit(eye, focus, up, 35.0, 0.05, w, h)
+    hit = en[..., 3] > 0
+    assert 0.1 < hit.mean() < 0.95
+    assert np.all(en[~hit] == 0) and np.all(ex[~hit] == 0)
+    on_face = lambda p: np.min(np.minimum(np.abs(p[..., :3]), np.abs(p[..., :3] - 1)), axis=-1)
+    assert np.all(on_face(en[hit]) < 1e-5) and np.all(on_face(ex[hit]) < 1e-5)
+    # exit - entry is parallel to the pixel ray and points away from the eye
+    eye_a = np.array(eye)
+    seg = (ex[..., :3] - en[..., :3])[hit].astype(np.float64)
+    toe = (en[..., :3][hit] - eye_a)
+    cr = np.cross(seg, toe)
+    assert np.all(np.linalg.norm(cr, axis=1) <= 1e-4 * (1 + np.linalg.norm(seg, axis=1)))
+    assert np.all(np.sum(seg * toe, axis=1) >= -1e-6)
+    # brute force hit mask: march each pixel ray through the cube
+    f = np.array(focus) - eye_a
+    f /= np.linalg.norm(f)
+    r = np.cross(f, up)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    th = math.tan(math.radians(35.0) / 2)
+    ys, xs = np.mgrid[0:h, 0:w]
+    nx = (2 * (xs + 0.5)) / w - 1
+    ny = (2 * (ys + 0.5)) / h - 1
+    d = f + (nx[..., None] * th * w / h) * r + (ny[..., None] * th) * u
+    ts = np.linspace(0.05, 6, 4000)
+    pts = eye_a + d[..., None, :] * ts[:, None]
+    inside = np.all((pts >= 0) & (pts <= 1), axis=-1).any(axis=-1)
+    assert np.mean(inside != hit) < 0.01
+
+
+def test_entry_exit_camera_inside_uses_near_plane(oracle):
+    en, ex = oracle.entry_exit((0.5, 0.5, 0.5), (0.5, 0.5, 0.0), (0, 1, 0), 40.0, 0.1, 9, 9)
+    assert np.all(en[..., 3] == 1.0)
+    c = en[4, 4, :3]
+    assert abs(c[2] - 0.4) < 1e-6 and abs(c[0] - 0.5) < 1e-6 and abs(c[1] - 0.5) < 1e-6
+    assert abs(ex[4, 4, 2]) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# reference-held fixtures
+# --------------------------------------------------------------------------
+
+def test_bundled_volumes_match_their_metadata():
+    for name in ("nucleon", "walnut"):
+        v = F.load_vvd(name)
+        with open(v["raw_path"], "rb") as fh:
+            assert hashlib.md5(fh.read()).hexdigest() == v["derived"]["hash"]
+        rng = 255.0 if v["data"].dtype == np.uint8 else 65535.0
+        assert abs(v["data"].max() / rng - v["derived"]["max"]) < 1e-7
+        assert abs(v["data"].min() / rng - v["derived"]["min"]) < 1e-7
+
+
+def test_bundled_tfs_build(oracle):
+    for name in ("nucleon", "walnut"):
+        t = F.load_tfi(name)
+        tf = oracle.tf_build(t["keys"], t["width"], t["gamma"], t["threshold"], t["alpha_mode"])
+        keys = t["keys"]
+        x = np.arange(t["width"]) / (t["width"] - 1)
+        assert np.all(tf[x <= keys[0][0]] == keys[0][1])
+        assert np.all(tf[x > keys[-1][0]] == keys[-1][1])
+        for c in range(4):
+            ref = np.interp(x, [k[0] for k in keys], [k[1][c] for k in keys])
+            assert np.all(np.abs(tf[:, c].astype(float) - ref) <= 1.0 + 1e-6)

@@ -1,0 +1,499 @@
+// api.hip -- the C-ABI of libhipdvr.so (declared in include/hipdvr.h).
+//
+// Owns device objects behind opaque handles: a context (device + stream +
+// events + counters), volumes (linear and bricked copies in HBM), transfer
+// functions (RGBA8 table + optional pre-integration table).  No C++ exception
+// crosses the ABI: every entry point catches and returns a status, and the
+// message is kept per context.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hipdvr.h"
+#include "errors.h"
+#include "kernels.h"
+#include "volume.h"
+
+using namespace hipdvr;
+
+struct hipdvr_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* d_counters = nullptr;   // 4 x u64
+    std::string last_error;
+    // per-launch timing (hipdvr_timing_begin/read)
+    std::vector<hipEvent_t> tev;                 // 2 per slot
+    uint32_t tcap = 0, tcount = 0;
+};
+
+struct hipdvr_tf {
+    hipdvr_ctx* ctx = nullptr;
+    uint32_t width = 0;
+    float domain_lo = 0.f, domain_hi = 1.f;
+    uint32_t* d_rgba = nullptr;
+    std::vector<uint8_t> host_rgba;
+    float4* d_preint = nullptr;
+    uint32_t preint_n = 0;
+};
+
+namespace {
+
+template <typename F>
+int guarded(hipdvr_ctx* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return HIPDVR_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return HIPDVR_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        return HIPDVR_ERR_HIP;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown error";
+        return HIPDVR_ERR_HIP;
+    }
+}
+
+void set_device(hipdvr_ctx* ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
+
+// Opacity correction per alpha byte: cpuraycaster.cpp:291 with the TF colour
+// byte/255.f of :330, in float with the float overload of pow.
+void fill_alpha_table(float step, float* out) {
+    const float e = step * 200.0f;
+    for (int k = 0; k < 256; ++k) {
+        float a = (float)k / 255.f;
+        out[k] = 1.f - powf(1.f - a, e);
+    }
+}
+
+void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
+    if (!p) fail(HIPDVR_ERR_INVALID, "params is NULL");
+    if (!(p->step > 0.f) || !std::isfinite(p->step)) fail(HIPDVR_ERR_INVALID, "step must be > 0");
+    if (p->filter != HIPDVR_FILTER_NEAREST && p->filter != HIPDVR_FILTER_LINEAR)
+        fail(HIPDVR_ERR_INVALID, "unknown filter mode");
+    if (p->classification != HIPDVR_CLASS_TF && p->classification != HIPDVR_CLASS_PREINT)
+        fail(HIPDVR_ERR_UNSUPPORTED, "unsupported classification mode");
+    if (p->classification == HIPDVR_CLASS_PREINT && (!tf->d_preint || tf->preint_n < 2))
+        fail(HIPDVR_ERR_INVALID, "pre-integrated classification needs a table (hipdvr_tf_set_preint)");
+    if (p->compositing != HIPDVR_COMPOSITE_DVR)
+        fail(HIPDVR_ERR_UNSUPPORTED, "only DVR compositing is implemented");
+    if (p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
+        fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
+    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_BRICKED)
+        fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
+}
+
+void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf,
+                     const hipdvr_params* p, const float* d_entry, const float* d_exit,
+                     float* d_out, uint32_t w, uint32_t h, unsigned long long* counters) {
+    std::memset(&a, 0, sizeof(a));
+    const VolumeDesc& v = vol->desc;
+    a.vox = vol->d_linear;
+    a.bricks = vol->d_bricks;
+    a.brick_log2 = v.brick_log2;
+    for (int i = 0; i < 3; ++i) {
+        a.dims[i] = v.dims[i];
+        a.nbricks[i] = vol->nbricks[i];
+        a.fdims[i] = (float)v.dims[i];
+        a.fdm1[i] = (float)v.dims[i] - 1.0f;
+        a.spacing[i] = v.spacing[i];
+        a.offset[i] = v.offset[i];
+        a.grad_off[i] = 1.f / (float)v.dims[i];
+        a.grad_scale[i] = (1.f / v.spacing[i]) * 0.5f;
+        a.light_pos[i] = p->light_pos[i];
+        a.light_amb[i] = p->light_ambient[i];
+        a.light_dif[i] = p->light_diffuse[i];
+        a.light_spe[i] = p->light_specular[i];
+        a.cam_pos[i] = p->camera_pos[i];
+    }
+    a.shininess = p->shininess;
+    a.rwm_scale = v.rwm_scale;
+    a.rwm_offset = v.rwm_offset;
+    a.dom_lo = tf->domain_lo;
+    a.dom_hi = tf->domain_hi;
+    a.tf_width = (int32_t)tf->width;
+    a.tf = tf->d_rgba;
+    a.preint = tf->d_preint;
+    a.preint_n = (int32_t)tf->preint_n;
+    a.step = p->step;
+    a.loop_cap = p->loop_cap > 0 ? p->loop_cap : kDefaultLoopCap;
+    a.entry = (const float4*)d_entry;
+    a.exit = (const float4*)d_exit;
+    a.out = (float4*)d_out;
+    a.w = (int32_t)w;
+    a.h = (int32_t)h;
+    a.counters = counters;
+    fill_alpha_table(p->step, a.alpha_corr);
+}
+
+void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_tf* tf,
+                        const float* d_entry, const float* d_exit, uint32_t w, uint32_t h,
+                        const hipdvr_params* p, float* d_out, hipdvr_stats* stats) {
+    if (!vol || !tf || !d_entry || !d_exit || !d_out) fail(HIPDVR_ERR_INVALID, "NULL argument");
+    if (vol->ctx != ctx || tf->ctx != ctx) fail(HIPDVR_ERR_INVALID, "handle from another context");
+    if (w == 0 || h == 0) fail(HIPDVR_ERR_INVALID, "empty image");
+    validate_params(p, tf);
+    set_device(ctx);
+    const bool count = stats != nullptr;
+    if (count) launch_zero_u64(ctx->d_counters, 4, ctx->stream);
+    MarchArgs a;
+    fill_march_args(a, vol, tf, p, d_entry, d_exit, d_out, w, h, ctx->d_counters);
+    const bool phong = p->shading == HIPDVR_SHADE_PHONG;
+    int kernel = p->kernel;
+    if (kernel == HIPDVR_KERNEL_AUTO) kernel = HIPDVR_KERNEL_REFERENCE;
+    if (count) hip_check(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
+    const bool timed = ctx->tcount < ctx->tcap;
+    if (timed) hip_check(hipEventRecord(ctx->tev[2 * ctx->tcount], ctx->stream), "hipEventRecord");
+    if (kernel == HIPDVR_KERNEL_REFERENCE) {
+        launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
+    } else {
+        fail(HIPDVR_ERR_UNSUPPORTED, "bricked kernel not available in this build");
+    }
+    hip_check(hipGetLastError(), "march kernel launch");
+    if (timed) {
+        hip_check(hipEventRecord(ctx->tev[2 * ctx->tcount + 1], ctx->stream), "hipEventRecord");
+        ++ctx->tcount;
+    }
+    if (count) {
+        hip_check(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
+        unsigned long long c[4];
+        hip_check(hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream),
+                  "hipMemcpyAsync(counters)");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+        stats->rays = c[0];
+        stats->samples = c[1];
+        stats->fetched = c[2];
+        stats->shaded = c[3];
+        stats->kernel_ms = ms;
+    }
+}
+
+// Camera -> texture-space ray-generation constants.  Double precision, then
+// rounded to float; the per-pixel part runs in float in the kernel.
+RayGen raygen_setup(const hipdvr_camera* cam, const float llf[3], const float urb[3], uint32_t w,
+                    uint32_t h, int quantize) {
+    double f[3], up[3], r[3], u[3], n;
+    for (int i = 0; i < 3; ++i) {
+        f[i] = (double)cam->focus[i] - cam->eye[i];
+        up[i] = cam->up[i];
+    }
+    n = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (int i = 0; i < 3; ++i) f[i] /= n;
+    r[0] = f[1] * up[2] - f[2] * up[1];
+    r[1] = f[2] * up[0] - f[0] * up[2];
+    r[2] = f[0] * up[1] - f[1] * up[0];
+    n = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int i = 0; i < 3; ++i) r[i] /= n;
+    u[0] = r[1] * f[2] - r[2] * f[1];
+    u[1] = r[2] * f[0] - r[0] * f[2];
+    u[2] = r[0] * f[1] - r[1] * f[0];
+    const double th = std::tan((double)cam->fovy_deg * 0.5 * 3.14159265358979323846 / 180.0);
+    const double aspect = (double)w / (double)h;
+    RayGen g;
+    for (int i = 0; i < 3; ++i) {
+        const double inv = 1.0 / ((double)urb[i] - (double)llf[i]);
+        g.o[i] = (float)(((double)cam->eye[i] - (double)llf[i]) * inv);
+        g.f[i] = (float)(f[i] * inv);
+        g.r[i] = (float)(r[i] * th * aspect * inv);
+        g.u[i] = (float)(u[i] * th * inv);
+    }
+    g.near_t = cam->near_dist;
+    g.w = (int32_t)w;
+    g.h = (int32_t)h;
+    g.quantize = quantize;
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipdvr_abi_version(void) { return HIPDVR_ABI_VERSION; }
+
+const char* hipdvr_last_error(const hipdvr_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : "NULL context";
+}
+
+int hipdvr_create(hipdvr_ctx** out, int device) {
+    if (!out) return HIPDVR_ERR_INVALID;
+    *out = nullptr;
+    hipdvr_ctx* ctx = new (std::nothrow) hipdvr_ctx();
+    if (!ctx) return HIPDVR_ERR_NOMEM;
+    int rc = guarded(ctx, [&] {
+        int n = 0;
+        hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device < 0 || device >= n) fail(HIPDVR_ERR_INVALID, "device index out of range");
+        ctx->device = device;
+        set_device(ctx);
+        hip_check(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+        ctx->stream = ctx->own_stream;
+        hip_check(hipEventCreate(&ctx->ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&ctx->ev1), "hipEventCreate");
+        hip_check(hipMalloc((void**)&ctx->d_counters, 4 * sizeof(unsigned long long)), "hipMalloc(counters)");
+    });
+    if (rc != HIPDVR_OK) {
+        std::fprintf(stderr, "hipdvr_create: %s\n", ctx->last_error.c_str());
+        hipdvr_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return HIPDVR_OK;
+}
+
+int hipdvr_destroy(hipdvr_ctx* ctx) {
+    if (!ctx) return HIPDVR_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return HIPDVR_OK;
+}
+
+int hipdvr_set_stream(hipdvr_ctx* ctx, void* stream) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return HIPDVR_OK;
+}
+
+int hipdvr_synchronize(hipdvr_ctx* ctx) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int hipdvr_volume_create(hipdvr_ctx* ctx, const hipdvr_volume_desc* desc, const void* host_voxels,
+                         hipdvr_volume** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!out || !desc || !host_voxels) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        *out = volume_create(ctx, ctx->stream, desc, host_voxels, /*device=*/false);
+    });
+}
+
+int hipdvr_volume_create_device(hipdvr_ctx* ctx, const hipdvr_volume_desc* desc,
+                                const void* device_voxels, hipdvr_volume** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!out || !desc || !device_voxels) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        *out = volume_create(ctx, ctx->stream, desc, device_voxels, /*device=*/true);
+    });
+}
+
+int hipdvr_volume_destroy(hipdvr_volume* vol) {
+    if (!vol) return HIPDVR_OK;
+    volume_destroy(vol);
+    return HIPDVR_OK;
+}
+
+int hipdvr_volume_device_ptr(const hipdvr_volume* vol, const void** out) {
+    if (!vol || !out) return HIPDVR_ERR_INVALID;
+    *out = vol->d_linear;
+    return HIPDVR_OK;
+}
+
+int hipdvr_tf_create(hipdvr_ctx* ctx, const uint8_t* rgba8, uint32_t width, float domain_lo,
+                     float domain_hi, hipdvr_tf** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!out || !rgba8) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (width < 2 || width > (uint32_t)kMaxTfWidth)
+            fail(HIPDVR_ERR_INVALID, "TF width must be in [2, 4096]");
+        if (!(domain_lo <= domain_hi)) fail(HIPDVR_ERR_INVALID, "invalid transfer function domain");
+        set_device(ctx);
+        hipdvr_tf* tf = new hipdvr_tf();
+        tf->ctx = ctx;
+        tf->width = width;
+        tf->domain_lo = domain_lo;
+        tf->domain_hi = domain_hi;
+        tf->host_rgba.assign(rgba8, rgba8 + 4 * (size_t)width);
+        hipError_t e = hipMalloc((void**)&tf->d_rgba, 4 * (size_t)width);
+        if (e == hipSuccess)
+            e = hipMemcpy(tf->d_rgba, rgba8, 4 * (size_t)width, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (tf->d_rgba) (void)hipFree(tf->d_rgba);
+            delete tf;
+            hip_check(e, "TF upload");
+        }
+        *out = tf;
+    });
+}
+
+int hipdvr_tf_destroy(hipdvr_tf* tf) {
+    if (!tf) return HIPDVR_OK;
+    (void)hipSetDevice(tf->ctx->device);
+    if (tf->d_rgba) (void)hipFree(tf->d_rgba);
+    if (tf->d_preint) (void)hipFree(tf->d_preint);
+    delete tf;
+    return HIPDVR_OK;
+}
+
+int hipdvr_tf_set_preint(hipdvr_tf* tf, const float* table, uint32_t n) {
+    if (!tf) return HIPDVR_ERR_INVALID;
+    return guarded(tf->ctx, [&] {
+        if (!table || n < 2 || n > 4096) fail(HIPDVR_ERR_INVALID, "pre-integration table size must be in [2, 4096]");
+        set_device(tf->ctx);
+        if (tf->d_preint) {
+            (void)hipFree(tf->d_preint);
+            tf->d_preint = nullptr;
+            tf->preint_n = 0;
+        }
+        const size_t bytes = (size_t)n * n * sizeof(float4);
+        hip_check(hipMalloc((void**)&tf->d_preint, bytes), "hipMalloc(preint)");
+        hip_check(hipMemcpy(tf->d_preint, table, bytes, hipMemcpyHostToDevice), "preint upload");
+        tf->preint_n = n;
+    });
+}
+
+int hipdvr_render_device(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_tf* tf,
+                         const float* d_entry, const float* d_exit, uint32_t w, uint32_t h,
+                         const hipdvr_params* prm, float* d_out, hipdvr_stats* stats) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] { render_device_impl(ctx, vol, tf, d_entry, d_exit, w, h, prm, d_out, stats); });
+}
+
+int hipdvr_render(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_tf* tf,
+                  const float* entry, const float* exit_, uint32_t w, uint32_t h,
+                  const hipdvr_params* prm, float* out, hipdvr_stats* stats) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!entry || !exit_ || !out) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (w == 0 || h == 0) fail(HIPDVR_ERR_INVALID, "empty image");
+        set_device(ctx);
+        const size_t bytes = (size_t)w * h * 4 * sizeof(float);
+        float *de = nullptr, *dx = nullptr, *dout = nullptr;
+        struct Bufs {
+            float **a, **b, **c;
+            ~Bufs() {
+                if (*a) (void)hipFree(*a);
+                if (*b) (void)hipFree(*b);
+                if (*c) (void)hipFree(*c);
+            }
+        } guard{&de, &dx, &dout};
+        hip_check(hipMalloc((void**)&de, bytes), "hipMalloc(entry)");
+        hip_check(hipMalloc((void**)&dx, bytes), "hipMalloc(exit)");
+        hip_check(hipMalloc((void**)&dout, bytes), "hipMalloc(out)");
+        hip_check(hipMemcpyAsync(de, entry, bytes, hipMemcpyHostToDevice, ctx->stream), "upload entry");
+        hip_check(hipMemcpyAsync(dx, exit_, bytes, hipMemcpyHostToDevice, ctx->stream), "upload exit");
+        render_device_impl(ctx, vol, tf, de, dx, w, h, prm, dout, stats);
+        hip_check(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream), "download out");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int hipdvr_entry_exit_device(hipdvr_ctx* ctx, const hipdvr_camera* cam, const float llf[3],
+                             const float urb[3], uint32_t w, uint32_t h, int32_t quantize16,
+                             float* d_entry, float* d_exit) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!cam || !llf || !urb || !d_entry || !d_exit) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (w == 0 || h == 0) fail(HIPDVR_ERR_INVALID, "empty image");
+        for (int i = 0; i < 3; ++i)
+            if (!(urb[i] > llf[i])) fail(HIPDVR_ERR_INVALID, "proxy box must have urb > llf");
+        set_device(ctx);
+        RayGen g = raygen_setup(cam, llf, urb, w, h, quantize16 ? 1 : 0);
+        launch_entry_exit(g, (float4*)d_entry, (float4*)d_exit, ctx->stream);
+        hip_check(hipGetLastError(), "entry/exit kernel launch");
+    });
+}
+
+int hipdvr_timing_begin(hipdvr_ctx* ctx, uint32_t capacity) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (capacity > (1u << 16)) fail(HIPDVR_ERR_INVALID, "timing capacity too large");
+        set_device(ctx);
+        while (ctx->tev.size() < 2 * (size_t)capacity) {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            ctx->tev.push_back(e);
+        }
+        ctx->tcap = capacity;
+        ctx->tcount = 0;
+    });
+}
+
+int hipdvr_timing_read(hipdvr_ctx* ctx, float* out_ms, uint32_t max, uint32_t* n) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!n || (max && !out_ms)) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        const uint32_t k = ctx->tcount < max ? ctx->tcount : max;
+        for (uint32_t i = 0; i < k; ++i)
+            hip_check(hipEventElapsedTime(&out_ms[i], ctx->tev[2 * i], ctx->tev[2 * i + 1]),
+                      "hipEventElapsedTime");
+        *n = ctx->tcount;
+        ctx->tcap = 0;
+        ctx->tcount = 0;
+    });
+}
+
+int hipdvr_copy_device(hipdvr_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!d_dst || !d_src || bytes % 16) fail(HIPDVR_ERR_INVALID, "copy needs 16-byte multiples");
+        set_device(ctx);
+        launch_copy16(d_dst, d_src, bytes / 16, ctx->stream);
+        hip_check(hipGetLastError(), "copy kernel launch");
+    });
+}
+
+int hipdvr_buffer_alloc(hipdvr_ctx* ctx, size_t bytes, void** d_ptr) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!d_ptr || bytes == 0) fail(HIPDVR_ERR_INVALID, "bad buffer request");
+        set_device(ctx);
+        *d_ptr = nullptr;
+        hip_check(hipMalloc(d_ptr, bytes), "hipMalloc");
+    });
+}
+
+int hipdvr_buffer_free(hipdvr_ctx* ctx, void* d_ptr) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        if (d_ptr) hip_check(hipFree(d_ptr), "hipFree");
+    });
+}
+
+int hipdvr_memcpy_h2d(hipdvr_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        hip_check(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int hipdvr_memcpy_d2h(hipdvr_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        hip_check(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+}  // extern "C"

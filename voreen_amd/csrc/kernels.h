@@ -1,0 +1,30 @@
+// kernels.h -- launchers of the HIP kernels (internal to libhipdvr.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "march_common.h"
+
+namespace hipdvr {
+
+// Ray-generation constants of the unit-cube proxy, texture space.
+struct RayGen {
+    float o[3];     // camera position
+    float f[3];     // view direction
+    float r[3];     // right * tan(fovy/2) * aspect
+    float u[3];     // up * tan(fovy/2)
+    float near_t;   // near-plane ray parameter
+    int32_t w, h;
+    int32_t quantize;
+};
+
+void launch_march_ref(const MarchArgs& a, int dtype, int filter, int classif, bool phong,
+                      bool count, hipStream_t st);
+// Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+// Streaming copy of n 16-byte vectors (bandwidth calibration).
+void launch_copy16(void* dst, const void* src, size_t n, hipStream_t st);
+void launch_entry_exit(const RayGen& g, float4* entry, float4* exit, hipStream_t st);
+
+}  // namespace hipdvr

@@ -1,0 +1,249 @@
+// march_common.h -- per-sample arithmetic of the ray march, shared by the
+// reference-layout and bricked kernels.
+//
+// Every float expression follows the evaluation order of Voreen's
+// CPURaycaster (modules/base/processors/render/cpuraycaster.cpp:210-332) and
+// VolumeRAM::getVoxelNormalizedLinear (src/core/datastructures/volume/
+// volumeram.cpp:78-97).  The translation units that include this header are
+// compiled with -ffp-contract=off, so no multiply-add is fused unless written
+// as an explicit fmaf().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hipdvr {
+
+constexpr float kErtOpacity = 0.95f;        // mod_raysetup.frag:36, cpuraycaster.cpp:310
+constexpr int kMaxTfWidth = 4096;           // transfunc1dkeysproperty.cpp:95
+constexpr int kDefaultLoopCap = 255 * 255;  // cpuraycaster.cpp:239
+
+// Arguments of one march launch (plain data; passed by value as kernarg).
+struct MarchArgs {
+    const void* vox;            // linear voxels (reference kernel)
+    const void* bricks;         // bricked voxels with +1 apron (bricked kernel)
+    const uint32_t* brick_occ;  // per-brick occupancy bits for this TF (bricked kernel)
+    uint32_t dims[3];
+    uint32_t nbricks[3];
+    uint32_t brick_log2;
+    float fdims[3];             // (float)dims
+    float fdm1[3];              // (float)dims - 1
+    float rwm_scale, rwm_offset;
+    float dom_lo, dom_hi;
+    int32_t tf_width;
+    const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
+    const float4* preint;       // N*N, [fe*N + fs]
+    int32_t preint_n;
+    float step;
+    int32_t loop_cap;
+    // Phong
+    float spacing[3], offset[3];
+    float grad_off[3];          // 1/dims (datasetDimensionsRCP_)
+    float grad_scale[3];        // (1/spacing) * 0.5
+    float light_pos[3], light_amb[3], light_dif[3], light_spe[3], shininess;
+    float cam_pos[3];
+    // images
+    const float4* entry;
+    const float4* exit;
+    float4* out;
+    int32_t w, h;
+    unsigned long long* counters;  // [rays, samples, fetched, shaded]
+    float alpha_corr[256];      // 1 - pow(1 - a/255, step*200) per alpha byte
+};
+
+// v / 255.f and v / 65535.f, correctly rounded, as q = v*r corrected by one
+// fused residual step; exact for every 8- and 16-bit voxel value (checked
+// exhaustively on the host by the test suite).
+template <typename T> struct VoxNorm;
+template <> struct VoxNorm<uint8_t> {
+    static __device__ __forceinline__ float apply(uint32_t v) {
+        const float d = 255.0f, r = 1.0f / 255.0f;
+        float fv = (float)v;
+        float q = fv * r;
+        float e = __builtin_fmaf(-q, d, fv);
+        return __builtin_fmaf(e, r, q);
+    }
+};
+template <> struct VoxNorm<uint16_t> {
+    static __device__ __forceinline__ float apply(uint32_t v) {
+        const float d = 65535.0f, r = 1.0f / 65535.0f;
+        float fv = (float)v;
+        float q = fv * r;
+        float e = __builtin_fmaf(-q, d, fv);
+        return __builtin_fmaf(e, r, q);
+    }
+};
+
+__device__ __forceinline__ float smaxf(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float sminf(float a, float b) { return (b < a) ? b : a; }
+// tgt::iround (ext/tgt/tgt_math.h:213-225)
+__device__ __forceinline__ int iround_tgt(float f) { return (int)floorf(f + 0.5f); }
+
+// Texture -> clamped voxel coordinate (cpuraycaster.cpp:243 with
+// VolumeBase::getTextureToVoxelMatrix, volumebase.cpp:221-224).
+__device__ __forceinline__ float tex_to_voxel(float s, float fdim, float fdm1) {
+    float v = (fdim * s) + (-0.5f);
+    return sminf(smaxf(v, 0.0f), fdm1);
+}
+
+// Trilinear weights and corner coordinates (volumeram.cpp:79-86).
+struct Corners {
+    uint32_t lo[3];
+    uint32_t d[3];      // hi - lo, 0 or 1
+    float p[3];
+};
+
+__device__ __forceinline__ void corners(const float vp[3], const uint32_t dims[3], Corners& c) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float pa = smaxf(vp[i], 0.0f);
+        c.p[i] = pa - floorf(pa);
+        uint32_t lo = (uint32_t)pa;
+        uint32_t hi = (uint32_t)ceilf(pa);
+        uint32_t m = dims[i] - 1;
+        hi = hi < m ? hi : m;
+        lo = lo < m ? lo : m;
+        c.lo[i] = lo;
+        c.d[i] = hi - lo;
+    }
+}
+
+// The 8-term sum in the reference order llB, lrB, urB, ulB, llF, lrF, urF, ulF.
+__device__ __forceinline__ float trilerp(const float v[8], const float p[3]) {
+    float ix = 1.f - p[0], iy = 1.f - p[1], iz = 1.f - p[2];
+    float s = v[0] * ix * iy * iz;
+    s = s + v[1] * p[0] * iy * iz;
+    s = s + v[2] * p[0] * p[1] * iz;
+    s = s + v[3] * ix * p[1] * iz;
+    s = s + v[4] * ix * iy * p[2];
+    s = s + v[5] * p[0] * iy * p[2];
+    s = s + v[6] * p[0] * p[1] * p[2];
+    s = s + v[7] * ix * p[1] * p[2];
+    return s;
+}
+
+// TransFuncBase::realWorldToNormalized (transfuncbase.cpp:244-259).
+__device__ __forceinline__ float rw_to_norm(float rw, float lo, float hi) {
+    if (rw < lo) return 0.f;
+    if (rw > hi) return 1.f;
+    if (lo == hi) return 0.f;
+    return (rw - lo) / (hi - lo);
+}
+
+// Intensity -> TF table index (cpuraycaster.cpp:254,266 and :328-332).
+__device__ __forceinline__ int tf_index(float inten, const MarchArgs& a) {
+    float rw = (inten * a.rwm_scale) + a.rwm_offset;
+    float x = rw_to_norm(rw, a.dom_lo, a.dom_hi);
+    int w1 = a.tf_width - 1;
+    int idx = iround_tgt(x * (float)w1);
+    return idx < 0 ? 0 : (idx > w1 ? w1 : idx);
+}
+
+__device__ __forceinline__ float norm_rw(float inten, const MarchArgs& a) {
+    float rw = (inten * a.rwm_scale) + a.rwm_offset;
+    return rw_to_norm(rw, a.dom_lo, a.dom_hi);
+}
+
+__device__ __forceinline__ int preint_index(float x, int n) {
+    int i = iround_tgt(x * (float)(n - 1));
+    return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+
+// byte / 255.f, correctly rounded (same residual step as VoxNorm<uint8_t>).
+__device__ __forceinline__ float byte_norm(uint32_t b) { return VoxNorm<uint8_t>::apply(b); }
+
+// Phong shading, mod_shading.frag:134-151 with ka = kd = TF colour, ks = 1;
+// a zero gradient gives N = 0.
+__device__ __forceinline__ void phong(const MarchArgs& a, const float g[3], const float pos[3],
+                                      float c[3]) {
+    float gl = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    float N[3] = {0.f, 0.f, 0.f};
+    if (gl > 0.f) {
+        float r = 1.f / sqrtf(gl);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) N[i] = g[i] * r;
+    }
+    float L[3], V[3], H[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { L[i] = a.light_pos[i] - pos[i]; V[i] = a.cam_pos[i] - pos[i]; }
+    float vl = sqrtf(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    float rv = 1.f / vl;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) V[i] *= rv;
+    float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) L[i] /= d;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) H[i] = V[i] + L[i];
+    float hl = sqrtf(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+    float rh = 1.f / hl;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) H[i] *= rh;
+    float ndl = fmaxf(N[0] * L[0] + N[1] * L[1] + N[2] * L[2], 0.f);
+    float ndh = fmaxf(N[0] * H[0] + N[1] * H[1] + N[2] * H[2], 0.f);
+    float sp = powf(ndh, a.shininess);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float amb = c[i] * a.light_amb[i];
+        float dif = (c[i] * a.light_dif[i]) * ndl;
+        float spe = (1.0f * a.light_spe[i]) * sp;
+        c[i] = ((0.0f + amb) + dif) + spe;
+    }
+}
+
+// fixClipBorderGradient, mod_gradients.frag:39-46 (neighbours clamp to edge).
+__device__ __forceinline__ void clip_border_gradient(const MarchArgs& a, int x, int y,
+                                                     const float s[3], const float dir[3],
+                                                     float g[3]) {
+    int x2 = x + 2 < a.w ? x + 2 : a.w - 1;
+    int y2 = y + 2 < a.h ? y + 2 : a.h - 1;
+    float4 e0 = a.entry[(size_t)y * a.w + x2];
+    float4 e1 = a.entry[(size_t)y2 * a.w + x];
+    float v0[3] = {e0.x - s[0], e0.y - s[1], e0.z - s[2]};
+    float v1[3] = {e1.x - s[0], e1.y - s[1], e1.z - s[2]};
+    float l0 = sqrtf(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+    float l1 = sqrtf(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+    if (l0 > 0.f) { float r = 1.f / l0; v0[0] *= r; v0[1] *= r; v0[2] *= r; }
+    if (l1 > 0.f) { float r = 1.f / l1; v1[0] *= r; v1[1] *= r; v1[2] *= r; }
+    g[0] = v0[1] * v1[2] - v0[2] * v1[1];
+    g[1] = v0[2] * v1[0] - v0[0] * v1[2];
+    g[2] = v0[0] * v1[1] - v0[1] * v1[0];
+    float dd = g[0] * dir[0] + g[1] * dir[1] + g[2] * dir[2];
+    if (!(dd < 0.f)) { g[0] = -g[0]; g[1] = -g[1]; g[2] = -g[2]; }
+}
+
+// Ray setup, cpuraycaster.cpp:218-231.
+__device__ __forceinline__ float ray_setup(const float first[3], const float last[3], float step,
+                                           float dir[3]) {
+    dir[0] = last[0] - first[0];
+    dir[1] = last[1] - first[1];
+    dir[2] = last[2] - first[2];
+    bool dz = (dir[0] == 0.f && dir[1] == 0.f && dir[2] == 0.f);
+    bool lnz = !(last[0] == 0.f && last[1] == 0.f && last[2] == 0.f);
+    bool fnz = !(first[0] == 0.f && first[1] == 0.f && first[2] == 0.f);
+    if (dz && lnz && fnz) return step / 2.0f;
+    float lsq = 0.0f;
+    lsq += dir[0] * dir[0];
+    lsq += dir[1] * dir[1];
+    lsq += dir[2] * dir[2];
+    float len = sqrtf(lsq);
+    float rl = 1.0f / len;
+    dir[0] *= rl;
+    dir[1] *= rl;
+    dir[2] *= rl;
+    return len;
+}
+
+__device__ __forceinline__ bool is_background(float4 f, float4 l) {
+    return f.x == 0.f && f.y == 0.f && f.z == 0.f && f.w == 0.f && l.x == 0.f && l.y == 0.f &&
+           l.z == 0.f && l.w == 0.f;
+}
+
+// 64-lane sum of a per-lane counter; lane 0 adds it to the global counter.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+}  // namespace hipdvr

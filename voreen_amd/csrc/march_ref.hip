@@ -1,0 +1,214 @@
+// march_ref.hip -- reference-layout ray march: linear (x-fastest) volume, one
+// lane per ray, 8x8 pixel tile per wavefront, 16x16 pixels per workgroup.
+//
+// Restates CPURaycaster::directRendering
+// (modules/base/processors/render/cpuraycaster.cpp:210-326) sample by sample:
+// same sample positions (t accumulated in float), same trilinear order, same
+// TF indexing and opacity correction, same ERT.  The TF (RGBA8, <= 4096
+// entries) and the 256-entry opacity-correction table sit in LDS.
+#include "march_common.h"
+#include "kernels.h"
+
+namespace hipdvr {
+
+template <typename T>
+struct LinearFetch {
+    const T* p;
+    uint32_t X;
+    uint64_t XY;
+    __device__ __forceinline__ void fetch8(const Corners& c, float v[8]) const {
+        uint64_t base = (uint64_t)c.lo[2] * XY + (uint64_t)c.lo[1] * X + c.lo[0];
+        uint64_t dx = c.d[0], dy = (uint64_t)c.d[1] * X, dz = (uint64_t)c.d[2] * XY;
+        uint32_t r0 = p[base];
+        uint32_t r1 = p[base + dx];
+        uint32_t r2 = p[base + dx + dy];
+        uint32_t r3 = p[base + dy];
+        uint32_t r4 = p[base + dz];
+        uint32_t r5 = p[base + dx + dz];
+        uint32_t r6 = p[base + dx + dy + dz];
+        uint32_t r7 = p[base + dy + dz];
+        v[0] = VoxNorm<T>::apply(r0);
+        v[1] = VoxNorm<T>::apply(r1);
+        v[2] = VoxNorm<T>::apply(r2);
+        v[3] = VoxNorm<T>::apply(r3);
+        v[4] = VoxNorm<T>::apply(r4);
+        v[5] = VoxNorm<T>::apply(r5);
+        v[6] = VoxNorm<T>::apply(r6);
+        v[7] = VoxNorm<T>::apply(r7);
+    }
+    __device__ __forceinline__ float nearest(const float vp[3]) const {
+        uint32_t x = (uint32_t)iround_tgt(vp[0]);
+        uint32_t y = (uint32_t)iround_tgt(vp[1]);
+        uint32_t z = (uint32_t)iround_tgt(vp[2]);
+        return VoxNorm<T>::apply(p[(uint64_t)z * XY + (uint64_t)y * X + x]);
+    }
+};
+
+template <typename T, int FILTER>
+__device__ __forceinline__ float reconstruct(const LinearFetch<T>& f, const MarchArgs& a,
+                                             const float s[3]) {
+    float vp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+    if (FILTER == 0) return f.nearest(vp);
+    Corners c;
+    corners(vp, a.dims, c);
+    float v[8];
+    f.fetch8(c, v);
+    return trilerp(v, c.p);
+}
+
+template <typename T, int FILTER, int CLASSIF, bool PHONG, bool COUNT>
+__global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    const int tid = threadIdx.x;
+    if (CLASSIF == 0) {
+        for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
+        s_alpha[tid] = a.alpha_corr[tid];
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool inside = (px < a.w) && (py < a.h);
+
+    LinearFetch<T> f;
+    f.p = (const T*)a.vox;
+    f.X = a.dims[0];
+    f.XY = (uint64_t)a.dims[0] * a.dims[1];
+
+    unsigned long long n_samples = 0, n_shaded = 0, n_rays = 0;
+    if (inside) {
+        const size_t pix = (size_t)py * a.w + px;
+        const float4 e = a.entry[pix];
+        const float4 l = a.exit[pix];
+        float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!is_background(e, l)) {
+            n_rays = 1;
+            const float first[3] = {e.x, e.y, e.z};
+            const float last[3] = {l.x, l.y, l.z};
+            float dir[3];
+            const float tend = ray_setup(first, last, a.step, dir);
+            const int cap = a.loop_cap;
+            float t = 0.0f, r = 0.f, g = 0.f, b = 0.f, A = 0.f, last_x = 0.f;
+            bool finished = false;
+            for (int loop = 0; !finished && loop < cap; ++loop) {
+                ++n_samples;
+                const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1],
+                                    first[2] + t * dir[2]};
+                const float inten = reconstruct<T, FILTER>(f, a, s);
+                float cr, cg, cb, ca;
+                if (CLASSIF == 1) {
+                    const float x = norm_rw(inten, a);
+                    const int n = a.preint_n;
+                    const float4 c = a.preint[(size_t)preint_index(x, n) * n + preint_index(last_x, n)];
+                    cr = c.x; cg = c.y; cb = c.z; ca = c.w;
+                    last_x = x;
+                } else {
+                    const uint32_t c8 = s_tf[tf_index(inten, a)];
+                    ca = s_alpha[c8 >> 24];
+                    cr = byte_norm(c8 & 0xffu);
+                    cg = byte_norm((c8 >> 8) & 0xffu);
+                    cb = byte_norm((c8 >> 16) & 0xffu);
+                }
+                if (ca > 0.0f) {
+                    if (PHONG) {
+                        float gr[3];
+                        if (t == 0.0f) {
+                            clip_border_gradient(a, px, py, s, dir, gr);
+                        } else {
+                            float vals[6];
+#pragma unroll
+                            for (int k = 0; k < 6; ++k) {
+                                const int axis = k % 3;
+                                const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
+                                float q[3] = {s[0], s[1], s[2]};
+                                q[axis] = s[axis] + off;
+                                q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
+                                q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
+                                vals[k] = reconstruct<T, FILTER>(f, a, q);
+                            }
+#pragma unroll
+                            for (int i = 0; i < 3; ++i)
+                                gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+                        }
+                        float pos[3], c[3] = {cr, cg, cb};
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+                            pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
+                        phong(a, gr, pos, c);
+                        cr = c[0]; cg = c[1]; cb = c[2];
+                        ++n_shaded;
+                    }
+                    const float wgt = (1.0f - A) * ca;
+                    r = r + wgt * cr;
+                    g = g + wgt * cg;
+                    b = b + wgt * cb;
+                    A = A + (1.0f - A) * ca;
+                }
+                if (A >= kErtOpacity) {
+                    A = 1.0f;
+                    finished = true;
+                }
+                t += a.step;
+                finished = finished || (t > tend);
+            }
+            res = make_float4(r, g, b, A);
+        }
+        a.out[pix] = res;
+    }
+    if (COUNT) {
+        wave_add(a.counters + 0, n_rays);
+        wave_add(a.counters + 1, n_samples);
+        wave_add(a.counters + 2, n_samples);
+        wave_add(a.counters + 3, n_shaded);
+    }
+}
+
+template <typename T, int FILTER, int CLASSIF, bool PHONG>
+static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
+    dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
+    if (count)
+        hipLaunchKernelGGL((march_ref_kernel<T, FILTER, CLASSIF, PHONG, true>), grid, dim3(256), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL((march_ref_kernel<T, FILTER, CLASSIF, PHONG, false>), grid, dim3(256), 0,
+                           st, a);
+}
+
+template <typename T>
+static void launch_dtype(const MarchArgs& a, int filter, int classif, bool phong, bool count,
+                         hipStream_t st) {
+    if (filter == 0) {
+        if (classif == 0) {
+            if (phong) launch_t<T, 0, 0, true>(a, count, st); else launch_t<T, 0, 0, false>(a, count, st);
+        } else {
+            if (phong) launch_t<T, 0, 1, true>(a, count, st); else launch_t<T, 0, 1, false>(a, count, st);
+        }
+    } else {
+        if (classif == 0) {
+            if (phong) launch_t<T, 1, 0, true>(a, count, st); else launch_t<T, 1, 0, false>(a, count, st);
+        } else {
+            if (phong) launch_t<T, 1, 1, true>(a, count, st); else launch_t<T, 1, 1, false>(a, count, st);
+        }
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+}
+
+void launch_march_ref(const MarchArgs& a, int dtype, int filter, int classif, bool phong,
+                      bool count, hipStream_t st) {
+    if (dtype == 0) launch_dtype<uint8_t>(a, filter, classif, phong, count, st);
+    else launch_dtype<uint16_t>(a, filter, classif, phong, count, st);
+}
+
+}  // namespace hipdvr

@@ -1,0 +1,362 @@
+"""ctypes binding of libhipdvr.so (the C-ABI declared in include/hipdvr.h).
+
+The library is built in-tree by ``python -m voreen_amd.build``.  Loading fails
+loudly (``HipDvrError``) when it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhipdvr.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+U8, U16 = 0, 1
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+CLASS_TF, CLASS_PREINT = 0, 1
+COMPOSITE_DVR = 0
+SHADE_NONE, SHADE_PHONG = 0, 1
+ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED = 0, 1, 2
+
+# every symbol include/hipdvr.h and include/hipdvr_synth.h declare
+EXPORTS = (
+    "hipdvr_create", "hipdvr_destroy", "hipdvr_last_error", "hipdvr_abi_version",
+    "hipdvr_set_stream", "hipdvr_synchronize",
+    "hipdvr_volume_create", "hipdvr_volume_create_device", "hipdvr_volume_destroy",
+    "hipdvr_volume_device_ptr",
+    "hipdvr_tf_from_keys", "hipdvr_tf_create", "hipdvr_tf_destroy", "hipdvr_tf_set_preint",
+    "hipdvr_preint_build",
+    "hipdvr_render", "hipdvr_render_device", "hipdvr_entry_exit_device",
+    "hipdvr_buffer_alloc", "hipdvr_buffer_free", "hipdvr_memcpy_h2d", "hipdvr_memcpy_d2h",
+    "hipdvr_synth_blobs", "hipdvr_synth_shells",
+    "hipdvr_timing_begin", "hipdvr_timing_read", "hipdvr_copy_device",
+)
+
+
+class HipDvrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hipdvr error {code}: {msg}")
+        self.code = code
+
+
+class VolumeDesc(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 3), ("dtype", C.c_int32), ("rwm_scale", C.c_float),
+                ("rwm_offset", C.c_float), ("spacing", C.c_float * 3), ("offset", C.c_float * 3),
+                ("brick_log2", C.c_uint32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("step", C.c_float), ("filter", C.c_int32), ("classification", C.c_int32),
+                ("compositing", C.c_int32), ("shading", C.c_int32),
+                ("light_pos", C.c_float * 3), ("light_ambient", C.c_float * 3),
+                ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
+                ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
+                ("loop_cap", C.c_int32), ("kernel", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("fetched", C.c_uint64),
+                ("shaded", C.c_uint64), ("kernel_ms", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TFKey(C.Structure):
+    _fields_ = [("intensity", C.c_float), ("split", C.c_int32),
+                ("color_l", C.c_uint8 * 4), ("color_r", C.c_uint8 * 4)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("focus", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy_deg", C.c_float), ("near_dist", C.c_float)]
+
+
+_lib = None
+
+
+def load():
+    """Load libhipdvr.so; raises HipDvrError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipDvrError(ERR_HIP, f"{LIB_PATH} is missing: run `python -m voreen_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    P, vp, i32, u32 = C.POINTER, C.c_void_p, C.c_int32, C.c_uint32
+    sig = {
+        "hipdvr_create": ([P(vp), C.c_int], C.c_int),
+        "hipdvr_destroy": ([vp], C.c_int),
+        "hipdvr_last_error": ([vp], C.c_char_p),
+        "hipdvr_abi_version": ([], C.c_int),
+        "hipdvr_set_stream": ([vp, vp], C.c_int),
+        "hipdvr_synchronize": ([vp], C.c_int),
+        "hipdvr_volume_create": ([vp, P(VolumeDesc), vp, P(vp)], C.c_int),
+        "hipdvr_volume_create_device": ([vp, P(VolumeDesc), vp, P(vp)], C.c_int),
+        "hipdvr_volume_destroy": ([vp], C.c_int),
+        "hipdvr_volume_device_ptr": ([vp, P(vp)], C.c_int),
+        "hipdvr_tf_from_keys": ([P(TFKey), i32, i32, C.c_float, C.c_float, C.c_float, i32, vp], C.c_int),
+        "hipdvr_tf_create": ([vp, vp, u32, C.c_float, C.c_float, P(vp)], C.c_int),
+        "hipdvr_tf_destroy": ([vp], C.c_int),
+        "hipdvr_tf_set_preint": ([vp, vp, u32], C.c_int),
+        "hipdvr_preint_build": ([P(TFKey), i32, C.c_float, C.c_float, C.c_float, i32, u32,
+                                 C.c_float, i32, vp], C.c_int),
+        "hipdvr_render": ([vp, vp, vp, vp, vp, u32, u32, P(Params), vp, P(Stats)], C.c_int),
+        "hipdvr_render_device": ([vp, vp, vp, vp, vp, u32, u32, P(Params), vp, P(Stats)], C.c_int),
+        "hipdvr_entry_exit_device": ([vp, P(Camera), vp, vp, u32, u32, i32, vp, vp], C.c_int),
+        "hipdvr_buffer_alloc": ([vp, C.c_size_t, P(vp)], C.c_int),
+        "hipdvr_buffer_free": ([vp, vp], C.c_int),
+        "hipdvr_memcpy_h2d": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "hipdvr_memcpy_d2h": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "hipdvr_synth_blobs": ([C.c_uint64, u32, u32, u32, i32, vp], C.c_int),
+        "hipdvr_synth_shells": ([C.c_uint64, u32, u32, u32, vp], C.c_int),
+        "hipdvr_timing_begin": ([vp, u32], C.c_int),
+        "hipdvr_timing_read": ([vp, C.POINTER(C.c_float), u32, C.POINTER(u32)], C.c_int),
+        "hipdvr_copy_device": ([vp, vp, vp, C.c_size_t], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_keys(keys):
+    """keys: sequence of (intensity, rgba) or (intensity, rgba_left, rgba_right)."""
+    arr = (TFKey * max(1, len(keys)))()
+    for i, k in enumerate(keys):
+        arr[i].intensity = float(k[0])
+        cl = k[1]
+        cr = k[2] if len(k) > 2 else k[1]
+        arr[i].split = int(len(k) > 2)
+        for c in range(4):
+            arr[i].color_l[c] = int(cl[c])
+            arr[i].color_r[c] = int(cr[c])
+    return arr, len(keys)
+
+
+def tf_from_keys(keys, width, gamma=1.0, thr=(0.0, 1.0), alpha_mode=ALPHA_USE):
+    """RGBA8 table of a 1D key TF (TransFunc1D::createTextureData semantics)."""
+    arr, n = make_keys(keys)
+    out = np.zeros((width, 4), np.uint8)
+    rc = load().hipdvr_tf_from_keys(arr, n, width, gamma, thr[0], thr[1], alpha_mode, _ptr(out))
+    if rc != OK:
+        raise HipDvrError(rc, "hipdvr_tf_from_keys: invalid keys or width")
+    return out
+
+
+def preint_build(keys, n, step, use_integral=False, gamma=1.0, thr=(0.0, 1.0),
+                 alpha_mode=ALPHA_USE):
+    """N x N x RGBA float pre-integration table, element [fe, fs]."""
+    arr, nk = make_keys(keys)
+    out = np.zeros((n, n, 4), np.float32)
+    rc = load().hipdvr_preint_build(arr, nk, gamma, thr[0], thr[1], alpha_mode, n, step,
+                                    int(use_integral), _ptr(out))
+    if rc != OK:
+        raise HipDvrError(rc, "hipdvr_preint_build: invalid arguments")
+    return out
+
+
+def synth_blobs(seed, dims, dtype=U8):
+    nx, ny, nz = dims
+    out = np.empty((nz, ny, nx), np.uint8 if dtype == U8 else np.uint16)
+    rc = load().hipdvr_synth_blobs(seed, nx, ny, nz, dtype, _ptr(out))
+    if rc != OK:
+        raise HipDvrError(rc, "hipdvr_synth_blobs")
+    return out
+
+
+def synth_shells(seed, dims):
+    nx, ny, nz = dims
+    out = np.empty((nz, ny, nx), np.uint16)
+    rc = load().hipdvr_synth_shells(seed, nx, ny, nz, _ptr(out))
+    if rc != OK:
+        raise HipDvrError(rc, "hipdvr_synth_shells")
+    return out
+
+
+def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading=SHADE_NONE,
+                   kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5)):
+    """Render parameters with Voreen's VolumeRenderer light defaults
+    (src/core/processors/volumerenderer.cpp:41-54)."""
+    p = Params()
+    p.step = step
+    p.filter = filter_
+    p.classification = classification
+    p.compositing = COMPOSITE_DVR
+    p.shading = shading
+    lt = dict(pos=(2.3, 1.5, 1.5), ambient=(0.4,) * 3, diffuse=(0.8,) * 3, specular=(0.6,) * 3,
+              shininess=60.0)
+    if light:
+        lt.update(light)
+    p.light_pos[:] = list(lt["pos"])
+    p.light_ambient[:] = list(lt["ambient"])
+    p.light_diffuse[:] = list(lt["diffuse"])
+    p.light_specular[:] = list(lt["specular"])
+    p.shininess = lt["shininess"]
+    p.camera_pos[:] = list(camera_pos)
+    p.loop_cap = 0
+    p.kernel = kernel
+    return p
+
+
+class Context:
+    """One device context: stream, events, counters (hipdvr_create)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        rc = self.lib.hipdvr_create(C.byref(self.h), device)
+        if rc != OK:
+            raise HipDvrError(rc, f"hipdvr_create(device={device}) failed")
+
+    def check(self, rc, what):
+        if rc != OK:
+            msg = self.lib.hipdvr_last_error(self.h)
+            raise HipDvrError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.hipdvr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        self.check(self.lib.hipdvr_synchronize(self.h), "hipdvr_synchronize")
+
+    # ---- objects ----
+    def volume(self, voxels, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0),
+               brick_log2=4):
+        return Volume(self, voxels, rwm, spacing, offset, brick_log2)
+
+    def transfunc(self, rgba8, domain=(0.0, 1.0)):
+        return TransFunc(self, rgba8, domain)
+
+    # ---- buffers ----
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.hipdvr_buffer_alloc(self.h, nbytes, C.byref(p)), "hipdvr_buffer_alloc")
+        return DeviceBuffer(self, p, nbytes)
+
+    def timing_begin(self, capacity):
+        self.check(self.lib.hipdvr_timing_begin(self.h, capacity), "hipdvr_timing_begin")
+
+    def timing_read(self, max_n=65536):
+        buf = (C.c_float * max_n)()
+        n = C.c_uint32()
+        self.check(self.lib.hipdvr_timing_read(self.h, buf, max_n, C.byref(n)), "hipdvr_timing_read")
+        return list(buf[:min(n.value, max_n)])
+
+    def copy_device(self, dst, src, nbytes):
+        self.check(self.lib.hipdvr_copy_device(self.h, dst.ptr, src.ptr, nbytes), "hipdvr_copy_device")
+
+    # ---- rendering ----
+    def render(self, vol, tf, entry, exit_, params, stats=True):
+        entry = np.ascontiguousarray(entry, np.float32)
+        exit_ = np.ascontiguousarray(exit_, np.float32)
+        if entry.shape != exit_.shape or entry.ndim != 3 or entry.shape[2] != 4:
+            raise ValueError("entry/exit must be (H, W, 4) float32 arrays of equal shape")
+        h, w = entry.shape[:2]
+        out = np.zeros((h, w, 4), np.float32)
+        st = Stats()
+        rc = self.lib.hipdvr_render(self.h, vol.h, tf.h, _ptr(entry), _ptr(exit_), w, h,
+                                    C.byref(params), _ptr(out), C.byref(st) if stats else None)
+        self.check(rc, "hipdvr_render")
+        return out, (st.as_dict() if stats else None)
+
+    def render_device(self, vol, tf, d_entry, d_exit, w, h, params, d_out, stats=False):
+        st = Stats()
+        rc = self.lib.hipdvr_render_device(self.h, vol.h, tf.h, d_entry.ptr, d_exit.ptr, w, h,
+                                           C.byref(params), d_out.ptr,
+                                           C.byref(st) if stats else None)
+        self.check(rc, "hipdvr_render_device")
+        return st.as_dict() if stats else None
+
+    def entry_exit_device(self, cam, llf, urb, w, h, d_entry, d_exit, quantize=False):
+        llf = np.asarray(llf, np.float32)
+        urb = np.asarray(urb, np.float32)
+        rc = self.lib.hipdvr_entry_exit_device(self.h, C.byref(cam), _ptr(llf), _ptr(urb), w, h,
+                                               int(quantize), d_entry.ptr, d_exit.ptr)
+        self.check(rc, "hipdvr_entry_exit_device")
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.hipdvr_memcpy_h2d(self.ctx.h, self.ptr, _ptr(arr), arr.nbytes),
+                       "hipdvr_memcpy_h2d")
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.hipdvr_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes),
+                       "hipdvr_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.hipdvr_buffer_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Volume:
+    """Device volume (hipdvr_volume_create); voxels is a (Z, Y, X) array."""
+
+    def __init__(self, ctx, voxels, rwm, spacing, offset, brick_log2):
+        voxels = np.ascontiguousarray(voxels)
+        if voxels.dtype not in (np.uint8, np.uint16) or voxels.ndim != 3:
+            raise ValueError("voxels must be a 3D uint8 or uint16 array")
+        self.ctx = ctx
+        d = VolumeDesc()
+        nz, ny, nx = voxels.shape
+        d.dims[:] = [nx, ny, nz]
+        d.dtype = U8 if voxels.dtype == np.uint8 else U16
+        d.rwm_scale, d.rwm_offset = rwm
+        d.spacing[:] = list(spacing)
+        d.offset[:] = list(offset)
+        d.brick_log2 = brick_log2
+        self.desc = d
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.hipdvr_volume_create(ctx.h, C.byref(d), _ptr(voxels), C.byref(self.h)),
+                  "hipdvr_volume_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipdvr_volume_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class TransFunc:
+    def __init__(self, ctx, rgba8, domain):
+        rgba8 = np.ascontiguousarray(rgba8, np.uint8)
+        self.ctx = ctx
+        self.width = rgba8.shape[0]
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.hipdvr_tf_create(ctx.h, _ptr(rgba8), self.width, domain[0], domain[1],
+                                           C.byref(self.h)), "hipdvr_tf_create")
+
+    def set_preint(self, table):
+        table = np.ascontiguousarray(table, np.float32)
+        self.ctx.check(self.ctx.lib.hipdvr_tf_set_preint(self.h, _ptr(table), table.shape[0]),
+                       "hipdvr_tf_set_preint")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipdvr_tf_destroy(self.h)
+            self.h = C.c_void_p()
