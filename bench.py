@@ -48,7 +48,7 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     """Time the oracle's CPURaycaster restatement (no shading) on a band of
     rows at the image centre; scale by the plain sample count of a frame."""
     from oracle import oracle as O
-    cores = os.cpu_count() or 1
+    cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
     h = en.shape[0]
     rows = 8
     while True:

@@ -22,6 +22,12 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-3
 
 
+@pytest.fixture(params=["reference", "bricked"])
+def kernel(request):
+    from voreen_amd import hipdvr as H
+    return {"reference": H.KERNEL_REFERENCE, "bricked": H.KERNEL_BRICKED}[request.param]
+
+
 def say(msg):
     print(f"[case] {msg}", flush=True)
 
@@ -101,10 +107,10 @@ def test_entry_exit_device_matches_oracle(ctx, oracle):
         assert 0.05 < np.mean(ge[..., 3] > 0) <= 1.0
 
 
-def test_c0_full_frame(ctx, oracle):
-    say("c0 128^3 u8 256^2 step 1/256 linear")
+def test_c0_full_frame(ctx, oracle, kernel):
+    say(f"c0 128^3 u8 256^2 step 1/256 linear kernel {kernel}")
     vol, tf, en, ex, step = c0_inputs(oracle)
-    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, fp64=True)
+    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, kernel=kernel, fp64=True)
     d = report("c0", r, oracle)
     assert d.max() <= TOL
     assert r["stats"]["samples"] == r["n"]
@@ -112,17 +118,18 @@ def test_c0_full_frame(ctx, oracle):
     assert oracle.psnr(r["gpu"], r["fp64"]) > 50.0
 
 
-def test_c0_orbit_views_nearest_and_linear(ctx, oracle):
+def test_c0_orbit_views_nearest_and_linear(ctx, oracle, kernel):
     for view in (5, 13, 29):
         for filt in (0, 1):
-            say(f"c0 view {view}/36 filter {filt} 160^2")
+            say(f"c0 view {view}/36 filter {filt} 160^2 kernel {kernel}")
             vol, tf, en, ex, step = c0_inputs(oracle, size=160, view=view, nviews=36)
-            r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, filter_=filt)
+            r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, filter_=filt,
+                            kernel=kernel)
             assert report(f"c0 v{view} f{filt}", r, oracle).max() <= TOL
             assert r["stats"]["samples"] == r["n"]
 
 
-def test_nucleon_regression_workspace_setup(ctx, oracle):
+def test_nucleon_regression_workspace_setup(ctx, oracle, kernel):
     # modules/base/test/render/cpuraycaster.vws: nucleon, camera :96-103,
     # canvas 200x200, samplingRate 2, linear filter, TF domain [0, 249]
     # (real-world units, so rwm scale 255), 12-key TF of :228-270 (== nucleon.tfi).
@@ -138,32 +145,33 @@ def test_nucleon_regression_workspace_setup(ctx, oracle):
                                llf=llf, urb=urb, quantize=True)
     step = float(np.float32(1.0) / (np.float32(41) * np.float32(2.0)))
     for filt in (1, 0):
-        say(f"nucleon 41^3 u8 200^2 filter {filt}")
+        say(f"nucleon 41^3 u8 200^2 filter {filt} kernel {kernel}")
         r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(255.0, 0.0),
-                        filter_=filt, spacing=tuple(sp), offset=v["offset"])
+                        filter_=filt, spacing=tuple(sp), offset=v["offset"], kernel=kernel)
         assert report(f"nucleon f{filt}", r, oracle).max() <= TOL
         assert r["stats"]["samples"] == r["n"]
         assert np.mean(r["gpu"][..., 3] > 0) > 0.05
 
 
-def test_walnut_uint16_noncubic(ctx, oracle):
+def test_walnut_uint16_noncubic(ctx, oracle, kernel):
     from voreen_amd import workloads as W
     v = F.load_vvd("walnut")
     t = F.load_tfi("walnut")
     tf = oracle.tf_build(t["keys"], t["width"], t["gamma"], t["threshold"], t["alpha_mode"])
     llf, urb = W.world_box(v["dims"], v["spacing"])
     for view in (0, 9):
-        say(f"walnut 128x96x114 u16 view {view} 192x144")
+        say(f"walnut 128x96x114 u16 view {view} 192x144 kernel {kernel}")
         cam = W.orbit_camera(view, 24, radius=2.5)
         en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 192, 144, llf=llf, urb=urb)
         step = W.step_for(v["dims"], 2.0)
-        r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(65535.0, 0.0))
+        r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(65535.0, 0.0),
+                        kernel=kernel)
         assert report(f"walnut v{view}", r, oracle).max() <= TOL
         assert r["stats"]["samples"] == r["n"]
         assert np.mean(r["gpu"][..., 3] > 0) > 0.02
 
 
-def test_phong_on_shells(ctx, oracle):
+def test_phong_on_shells(ctx, oracle, kernel):
     from voreen_amd import workloads as W
     dims = (96, 96, 96)
     vol = W.make_volume("c1", dims=dims)
@@ -174,9 +182,9 @@ def test_phong_on_shells(ctx, oracle):
     en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 128, 128, llf=llf, urb=urb)
     sp = (1.0 / 96,) * 3
     off = tuple(float(llf[i] + 0.5 / 96) for i in range(3))
-    say("phong shells 96^3 u16 128^2")
+    say(f"phong shells 96^3 u16 128^2 kernel {kernel}")
     r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, W.step_for(dims, 2.0), shading=1,
-                    spacing=sp, offset=off, camera_pos=tuple(cam.eye), fp64=True)
+                    spacing=sp, offset=off, camera_pos=tuple(cam.eye), kernel=kernel, fp64=True)
     d = report("phong", r, oracle).max(axis=2)
     assert np.mean(d > TOL) < 0.002
     assert oracle.psnr(r["gpu"], r["fp64"]) > 45.0
@@ -194,7 +202,7 @@ def test_preintegrated_classification(ctx, oracle, integral):
     assert report(f"preint {integral}", r, oracle).max() <= TOL
 
 
-def test_edge_cases(ctx, oracle):
+def test_edge_cases(ctx, oracle, kernel):
     from voreen_amd import workloads as W
     rng = np.random.default_rng(5)
     keys = W.random_keys(99, 5)
@@ -209,7 +217,7 @@ def test_edge_cases(ctx, oracle):
     ex[5, :] = en[5, :]               # degenerate rays
     vol = rng.integers(0, 256, (9, 11, 13), dtype=np.uint8)
     say("edge: ragged 37x23 with background and degenerate rays")
-    r = render_both(ctx, oracle, vol, tf256, (0.0, 1.0), en, ex, 1 / 26)
+    r = render_both(ctx, oracle, vol, tf256, (0.0, 1.0), en, ex, 1 / 26, kernel=kernel)
     assert report("ragged", r, oracle).max() <= TOL
     assert np.all(r["gpu"][0:3] == 0.0)
     # tiny volumes and a 1x1 image
@@ -218,7 +226,8 @@ def test_edge_cases(ctx, oracle):
         e1 = np.array([[[0.1, 0.2, 0.3, 1.0]]], np.float32)
         x1 = np.array([[[0.9, 0.7, 0.6, 1.0]]], np.float32)
         say(f"edge: volume {shape} 1x1")
-        r = render_both(ctx, oracle, v, oracle.tf_build(keys, 4096), (0.0, 1.0), e1, x1, 1 / 14)
+        r = render_both(ctx, oracle, v, oracle.tf_build(keys, 4096), (0.0, 1.0), e1, x1, 1 / 14,
+                        kernel=kernel)
         assert report(f"tiny {shape}", r, oracle).max() <= TOL
         assert r["stats"]["samples"] == r["n"]
     # loop cap, TF width 2, degenerate domain
@@ -226,12 +235,12 @@ def test_edge_cases(ctx, oracle):
     x1 = np.array([[[1.0, 0.5, 0.5, 1.0]]], np.float32)
     zero_tf = np.zeros((256, 4), np.uint8)
     say("edge: loop cap")
-    r = render_both(ctx, oracle, vol, zero_tf, (0.0, 1.0), e1, x1, 1e-6)
+    r = render_both(ctx, oracle, vol, zero_tf, (0.0, 1.0), e1, x1, 1e-6, kernel=kernel)
     assert r["stats"]["samples"] == 255 * 255 == r["n"]
     say("edge: TF width 2, domain lo == hi")
     tf2 = np.array([[255, 0, 0, 30], [0, 255, 0, 200]], np.uint8)
     for dom in ((0.0, 1.0), (0.4, 0.4), (0.2, 0.3)):
-        r = render_both(ctx, oracle, vol, tf2, dom, en, ex, 1 / 40)
+        r = render_both(ctx, oracle, vol, tf2, dom, en, ex, 1 / 40, kernel=kernel)
         assert report(f"tf2 {dom}", r, oracle).max() <= TOL
 
 
@@ -261,3 +270,79 @@ def test_invalid_arguments_return_codes(ctx):
     finally:
         v.close()
         t.close()
+
+
+def test_bricked_equals_reference_and_skips(ctx, oracle):
+    """The bricked kernel must produce the reference kernel's image bit for bit
+    while reconstructing fewer samples; a TF change must refresh its brick
+    occupancy (same volume, second TF)."""
+    from voreen_amd import hipdvr as H, workloads as W
+    dims = (160, 144, 128)
+    vol = W.make_volume("c1", dims=dims)
+    keys, _ = W.make_tf("c1")
+    cam = W.orbit_camera(11, 36)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 200, 176, llf=llf, urb=urb)
+    step = W.step_for(dims, 2.0)
+    v = ctx.volume(vol)
+    try:
+        for seed in (0, 1):
+            tfk = keys if seed == 0 else W.random_keys(1234, 6)
+            tf = oracle.tf_build(tfk, 4096)
+            t = ctx.transfunc(tf)
+            outs = {}
+            for name, k in (("ref", H.KERNEL_REFERENCE), ("brick", H.KERNEL_BRICKED)):
+                for shading in (0, 1):
+                    say(f"bricked-vs-reference tf{seed} {name} shading {shading}")
+                    p = H.default_params(step, shading=shading, kernel=k, camera_pos=tuple(cam.eye))
+                    outs[(name, shading)] = ctx.render(v, t, en, ex, p)
+            t.close()
+            for shading in (0, 1):
+                a, sa = outs[("ref", shading)]
+                b, sb = outs[("brick", shading)]
+                assert np.array_equal(a, b), f"tf{seed} shading {shading}"
+                assert sa["samples"] == sb["samples"] and sa["shaded"] == sb["shaded"]
+                print(f"tf{seed} shading {shading}: fetched {sb['fetched']} of {sb['samples']}",
+                      flush=True)
+                if seed == 0:
+                    assert sb["fetched"] < 0.8 * sb["samples"]
+            cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+            assert np.abs(outs[("brick", 0)][0] - cpu).max() <= TOL
+    finally:
+        v.close()
+
+
+def test_bricked_all_transparent_fetches_nothing(ctx, oracle):
+    from voreen_amd import hipdvr as H, workloads as W
+    vol = W.make_volume("c0", dims=(64, 64, 64))
+    tf = np.zeros((256, 4), np.uint8)
+    tf[:, :3] = 200
+    en, ex = oracle.entry_exit((0.5, 0.6, 3.0), (0.5, 0.5, 0.5), (0, 1, 0), 30.0, 0.1, 64, 64)
+    say("bricked all-transparent TF")
+    v, t = ctx.volume(vol), ctx.transfunc(tf)
+    try:
+        img, st = ctx.render(v, t, en, ex, H.default_params(1 / 128, kernel=H.KERNEL_BRICKED))
+    finally:
+        v.close()
+        t.close()
+    assert st["fetched"] == 0 and st["samples"] > 0 and np.all(img == 0)
+
+
+@pytest.mark.parametrize("brick_log2", [2, 3, 4, 5])
+def test_bricked_brick_sizes(ctx, oracle, brick_log2):
+    from voreen_amd import hipdvr as H, workloads as W
+    vol = W.make_volume("c0", dims=(70, 50, 90))
+    keys, tf = W.make_tf("c0")
+    cam = W.orbit_camera(20, 36)
+    llf, urb = W.world_box((70, 50, 90))
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 120, 90, llf=llf, urb=urb)
+    step = W.step_for((70, 50, 90), 2.0)
+    say(f"bricked brick_log2 {brick_log2}")
+    v, t = ctx.volume(vol, brick_log2=brick_log2), ctx.transfunc(tf)
+    try:
+        img, st = ctx.render(v, t, en, ex, H.default_params(step, kernel=H.KERNEL_BRICKED))
+    finally:
+        v.close()
+        t.close()
+    cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+    assert np.abs(img - cpu).max() <= TOL and st["samples"] == n

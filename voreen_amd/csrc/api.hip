@@ -7,6 +7,7 @@
 // message is kept per context.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,9 +37,11 @@ struct hipdvr_ctx {
 
 struct hipdvr_tf {
     hipdvr_ctx* ctx = nullptr;
+    uint64_t serial = 0;             // unique per TF object (occupancy cache key)
     uint32_t width = 0;
     float domain_lo = 0.f, domain_hi = 1.f;
     uint32_t* d_rgba = nullptr;
+    uint32_t* d_alpha_prefix = nullptr;   // [width+1] count of non-zero alpha bytes below i
     std::vector<uint8_t> host_rgba;
     float4* d_preint = nullptr;
     uint32_t preint_n = 0;
@@ -68,6 +71,40 @@ int guarded(hipdvr_ctx* ctx, F&& f) {
 }
 
 void set_device(hipdvr_ctx* ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
+
+uint64_t next_serial() {
+    static std::atomic<uint64_t> s{1};
+    return s.fetch_add(1);
+}
+
+uint64_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// Per-brick occupancy of `vol` under the TF mapping of `tf` (and the volume's
+// real-world mapping); recomputed only when the pair changes.
+void ensure_occupancy(hipdvr_ctx* ctx, hipdvr_volume* vol, const hipdvr_tf* tf) {
+    const uint64_t key[4] = {tf->serial, (fbits(tf->domain_lo) << 32) | fbits(tf->domain_hi),
+                             (fbits(vol->desc.rwm_scale) << 32) | fbits(vol->desc.rwm_offset),
+                             tf->width};
+    if (std::memcmp(key, vol->occ_key, sizeof key) == 0) return;
+    OccArgs o;
+    o.minmax = vol->d_minmax;
+    o.occ = vol->d_occ;
+    o.count = vol->brick_count;
+    o.dtype = vol->desc.dtype;
+    o.rwm_scale = vol->desc.rwm_scale;
+    o.rwm_offset = vol->desc.rwm_offset;
+    o.dom_lo = tf->domain_lo;
+    o.dom_hi = tf->domain_hi;
+    o.tf_width = (int32_t)tf->width;
+    o.alpha_prefix = tf->d_alpha_prefix;
+    launch_occupancy(o, ctx->stream);
+    hip_check(hipGetLastError(), "occupancy kernel launch");
+    std::memcpy(vol->occ_key, key, sizeof key);
+}
 
 // Opacity correction per alpha byte: cpuraycaster.cpp:291 with the TF colour
 // byte/255.f of :330, in float with the float overload of pow.
@@ -102,7 +139,8 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     std::memset(&a, 0, sizeof(a));
     const VolumeDesc& v = vol->desc;
     a.vox = vol->d_linear;
-    a.bricks = vol->d_bricks;
+    a.bricks = vol->d_tiled;
+    a.brick_occ = vol->d_occ;
     a.brick_log2 = v.brick_log2;
     for (int i = 0; i < 3; ++i) {
         a.dims[i] = v.dims[i];
@@ -153,14 +191,21 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     fill_march_args(a, vol, tf, p, d_entry, d_exit, d_out, w, h, ctx->d_counters);
     const bool phong = p->shading == HIPDVR_SHADE_PHONG;
     int kernel = p->kernel;
-    if (kernel == HIPDVR_KERNEL_AUTO) kernel = HIPDVR_KERNEL_REFERENCE;
+    // the bricked kernel skips samples, which the pre-integrated mode cannot
+    // (its lookup pairs every sample with the previous one)
+    if (kernel == HIPDVR_KERNEL_AUTO)
+        kernel = p->classification == HIPDVR_CLASS_TF ? HIPDVR_KERNEL_BRICKED : HIPDVR_KERNEL_REFERENCE;
+    if (kernel == HIPDVR_KERNEL_BRICKED && p->classification != HIPDVR_CLASS_TF)
+        fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel supports transfer-function classification only");
+    if (kernel == HIPDVR_KERNEL_BRICKED) ensure_occupancy(ctx, const_cast<hipdvr_volume*>(vol), tf);
     if (count) hip_check(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     const bool timed = ctx->tcount < ctx->tcap;
     if (timed) hip_check(hipEventRecord(ctx->tev[2 * ctx->tcount], ctx->stream), "hipEventRecord");
     if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else {
-        fail(HIPDVR_ERR_UNSUPPORTED, "bricked kernel not available in this build");
+        const bool wide = vol->tiled_elems > 0xffffffffull;
+        launch_march_fast(a, vol->desc.dtype, p->filter, phong, count, wide, ctx->stream);
     }
     hip_check(hipGetLastError(), "march kernel launch");
     if (timed) {
@@ -328,12 +373,19 @@ int hipdvr_tf_create(hipdvr_ctx* ctx, const uint8_t* rgba8, uint32_t width, floa
         tf->width = width;
         tf->domain_lo = domain_lo;
         tf->domain_hi = domain_hi;
+        tf->serial = next_serial();
         tf->host_rgba.assign(rgba8, rgba8 + 4 * (size_t)width);
+        std::vector<uint32_t> prefix(width + 1, 0);
+        for (uint32_t i = 0; i < width; ++i) prefix[i + 1] = prefix[i] + (rgba8[4 * i + 3] != 0);
         hipError_t e = hipMalloc((void**)&tf->d_rgba, 4 * (size_t)width);
         if (e == hipSuccess)
             e = hipMemcpy(tf->d_rgba, rgba8, 4 * (size_t)width, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void**)&tf->d_alpha_prefix, 4 * (size_t)(width + 1));
+        if (e == hipSuccess)
+            e = hipMemcpy(tf->d_alpha_prefix, prefix.data(), 4 * (size_t)(width + 1), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (tf->d_rgba) (void)hipFree(tf->d_rgba);
+            if (tf->d_alpha_prefix) (void)hipFree(tf->d_alpha_prefix);
             delete tf;
             hip_check(e, "TF upload");
         }
@@ -345,6 +397,7 @@ int hipdvr_tf_destroy(hipdvr_tf* tf) {
     if (!tf) return HIPDVR_OK;
     (void)hipSetDevice(tf->ctx->device);
     if (tf->d_rgba) (void)hipFree(tf->d_rgba);
+    if (tf->d_alpha_prefix) (void)hipFree(tf->d_alpha_prefix);
     if (tf->d_preint) (void)hipFree(tf->d_preint);
     delete tf;
     return HIPDVR_OK;

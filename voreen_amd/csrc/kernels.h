@@ -21,6 +21,8 @@ struct RayGen {
 
 void launch_march_ref(const MarchArgs& a, int dtype, int filter, int classif, bool phong,
                       bool count, hipStream_t st);
+void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bool count, bool wide,
+                       hipStream_t st);
 // Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // Streaming copy of n 16-byte vectors (bandwidth calibration).

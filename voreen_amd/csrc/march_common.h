@@ -21,8 +21,8 @@ constexpr int kDefaultLoopCap = 255 * 255;  // cpuraycaster.cpp:239
 // Arguments of one march launch (plain data; passed by value as kernarg).
 struct MarchArgs {
     const void* vox;            // linear voxels (reference kernel)
-    const void* bricks;         // bricked voxels with +1 apron (bricked kernel)
-    const uint32_t* brick_occ;  // per-brick occupancy bits for this TF (bricked kernel)
+    const void* bricks;         // 4^3-block tiled voxels (bricked kernel)
+    const uint8_t* brick_occ;   // per-brick occupancy (1 = may hold alpha > 0) for this TF
     uint32_t dims[3];
     uint32_t nbricks[3];
     uint32_t brick_log2;

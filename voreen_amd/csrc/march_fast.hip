@@ -1,0 +1,239 @@
+// march_fast.hip -- ray march on the 4^3-block tiled volume with exact
+// empty-space skipping.
+//
+// Same sample positions, reconstruction, classification and compositing as
+// march_ref.hip (and as CPURaycaster::directRendering,
+// modules/base/processors/render/cpuraycaster.cpp:210-326), so the image is
+// bit-identical.  Differences are only in where the voxels come from and which
+// samples are reconstructed:
+//   * voxels are read from the 4x4x4-block tiled copy (separable addressing,
+//     one 128-byte line per uint16 block);
+//   * a sample whose footprint lies in a brick the occupancy table marks empty
+//     maps to a zero-alpha TF entry, so it is not reconstructed: the ray only
+//     advances t (t += step, as the reference) and re-tests its brick;
+//   * workgroups are remapped so each XCD marches a contiguous band of tiles
+//     (its L2 holds the bricks that band reads).
+#include "kernels.h"
+#include "march_common.h"
+
+namespace hipdvr {
+
+template <typename T, bool WIDE>
+struct TiledFetch {
+    typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type off_t;
+    const T* p;
+    uint32_t sy, sz;   // elements per block row (64*NBX) and per block slab (64*NBX*NBY)
+
+    __device__ __forceinline__ off_t ox(uint32_t x) const { return (off_t)(((x >> 2) << 6) | (x & 3)); }
+    __device__ __forceinline__ off_t oy(uint32_t y) const { return (off_t)(y >> 2) * sy + ((y & 3) << 2); }
+    __device__ __forceinline__ off_t oz(uint32_t z) const { return (off_t)(z >> 2) * sz + ((z & 3) << 4); }
+
+    __device__ __forceinline__ void fetch8(const Corners& c, float v[8]) const {
+        const off_t x0 = ox(c.lo[0]), x1 = ox(c.lo[0] + c.d[0]);
+        const off_t y0 = oy(c.lo[1]), y1 = oy(c.lo[1] + c.d[1]);
+        const off_t z0 = oz(c.lo[2]), z1 = oz(c.lo[2] + c.d[2]);
+        const uint32_t r0 = p[x0 + y0 + z0];
+        const uint32_t r1 = p[x1 + y0 + z0];
+        const uint32_t r2 = p[x1 + y1 + z0];
+        const uint32_t r3 = p[x0 + y1 + z0];
+        const uint32_t r4 = p[x0 + y0 + z1];
+        const uint32_t r5 = p[x1 + y0 + z1];
+        const uint32_t r6 = p[x1 + y1 + z1];
+        const uint32_t r7 = p[x0 + y1 + z1];
+        v[0] = VoxNorm<T>::apply(r0);
+        v[1] = VoxNorm<T>::apply(r1);
+        v[2] = VoxNorm<T>::apply(r2);
+        v[3] = VoxNorm<T>::apply(r3);
+        v[4] = VoxNorm<T>::apply(r4);
+        v[5] = VoxNorm<T>::apply(r5);
+        v[6] = VoxNorm<T>::apply(r6);
+        v[7] = VoxNorm<T>::apply(r7);
+    }
+    __device__ __forceinline__ float nearest(const float vp[3]) const {
+        const uint32_t x = (uint32_t)iround_tgt(vp[0]);
+        const uint32_t y = (uint32_t)iround_tgt(vp[1]);
+        const uint32_t z = (uint32_t)iround_tgt(vp[2]);
+        return VoxNorm<T>::apply(p[ox(x) + oy(y) + oz(z)]);
+    }
+};
+
+template <typename T, bool WIDE, int FILTER>
+__device__ __forceinline__ float reconstruct_vp(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
+                                                const float vp[3]) {
+    if (FILTER == 0) return f.nearest(vp);
+    Corners c;
+    corners(vp, a.dims, c);
+    float v[8];
+    f.fetch8(c, v);
+    return trilerp(v, c.p);
+}
+
+template <typename T, bool WIDE, int FILTER>
+__device__ __forceinline__ float reconstruct_s(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
+                                               const float s[3]) {
+    float vp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+    return reconstruct_vp<T, WIDE, FILTER>(f, a, vp);
+}
+
+template <typename T, bool WIDE, int FILTER, bool PHONG, bool COUNT>
+__global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, const uint32_t tiles_x,
+                                                         const uint32_t tiles_total,
+                                                         const uint32_t tiles_per_xcd) {
+    // XCD-aware tile order: hardware places workgroup b on XCD b % 8, so XCD k
+    // gets tiles [k*tiles_per_xcd, (k+1)*tiles_per_xcd) in row-major order.
+    const uint32_t b = blockIdx.x;
+    const uint32_t tile = (b & 7u) * tiles_per_xcd + (b >> 3);
+    if (tile >= tiles_total) return;   // whole workgroup leaves together
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
+    s_alpha[tid] = a.alpha_corr[tid];
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool inside = (px < a.w) && (py < a.h);
+
+    TiledFetch<T, WIDE> f;
+    f.p = (const T*)a.bricks;
+    f.sy = (uint32_t)((a.dims[0] + 3) >> 2) << 6;
+    f.sz = f.sy * ((a.dims[1] + 3) >> 2);
+    const uint32_t bl = a.brick_log2;
+    const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
+
+    unsigned long long n_samples = 0, n_fetched = 0, n_shaded = 0, n_rays = 0;
+    if (inside) {
+        const size_t pix = (size_t)py * a.w + px;
+        const float4 e = a.entry[pix];
+        const float4 l = a.exit[pix];
+        float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!is_background(e, l)) {
+            n_rays = 1;
+            const float first[3] = {e.x, e.y, e.z};
+            const float last[3] = {l.x, l.y, l.z};
+            float dir[3];
+            const float tend = ray_setup(first, last, a.step, dir);
+            const int cap = a.loop_cap;
+            float t = 0.0f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
+            uint32_t cur_brick = 0xffffffffu;
+            bool cur_occ = false;
+            bool finished = false;
+            for (int loop = 0; !finished && loop < cap; ++loop) {
+                ++n_samples;
+                const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1],
+                                    first[2] + t * dir[2]};
+                float vp[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+                const uint32_t brick = (((uint32_t)vp[2] >> bl) * nbxy) +
+                                       (((uint32_t)vp[1] >> bl) * nbx) + ((uint32_t)vp[0] >> bl);
+                if (brick != cur_brick) {
+                    cur_brick = brick;
+                    cur_occ = a.brick_occ[brick] != 0;
+                }
+                if (cur_occ) {
+                    ++n_fetched;
+                    const float inten = reconstruct_vp<T, WIDE, FILTER>(f, a, vp);
+                    const uint32_t c8 = s_tf[tf_index(inten, a)];
+                    const float ca = s_alpha[c8 >> 24];
+                    if (ca > 0.0f) {
+                        float cr = byte_norm(c8 & 0xffu);
+                        float cg = byte_norm((c8 >> 8) & 0xffu);
+                        float cb = byte_norm((c8 >> 16) & 0xffu);
+                        if (PHONG) {
+                            float gr[3];
+                            if (t == 0.0f) {
+                                clip_border_gradient(a, px, py, s, dir, gr);
+                            } else {
+                                float vals[6];
+#pragma unroll
+                                for (int k = 0; k < 6; ++k) {
+                                    const int axis = k % 3;
+                                    const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
+                                    float q[3];
+                                    q[axis] = s[axis] + off;
+                                    q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
+                                    q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
+                                    vals[k] = reconstruct_s<T, WIDE, FILTER>(f, a, q);
+                                }
+#pragma unroll
+                                for (int i = 0; i < 3; ++i)
+                                    gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+                            }
+                            float pos[3], c[3] = {cr, cg, cb};
+#pragma unroll
+                            for (int i = 0; i < 3; ++i)
+                                pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
+                            phong(a, gr, pos, c);
+                            cr = c[0];
+                            cg = c[1];
+                            cb = c[2];
+                            ++n_shaded;
+                        }
+                        const float wgt = (1.0f - A) * ca;
+                        r = r + wgt * cr;
+                        g = g + wgt * cg;
+                        bb = bb + wgt * cb;
+                        A = A + (1.0f - A) * ca;
+                    }
+                    if (A >= kErtOpacity) {
+                        A = 1.0f;
+                        finished = true;
+                    }
+                }
+                t += a.step;
+                finished = finished || (t > tend);
+            }
+            res = make_float4(r, g, bb, A);
+        }
+        a.out[pix] = res;
+    }
+    if (COUNT) {
+        wave_add(a.counters + 0, n_rays);
+        wave_add(a.counters + 1, n_samples);
+        wave_add(a.counters + 2, n_fetched);
+        wave_add(a.counters + 3, n_shaded);
+    }
+}
+
+template <typename T, bool WIDE, int FILTER, bool PHONG>
+static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
+    const uint32_t tx = (a.w + 15) / 16, ty = (a.h + 15) / 16;
+    const uint32_t total = tx * ty;
+    const uint32_t per_xcd = (total + 7) / 8;
+    const dim3 grid(per_xcd * 8);
+    if (count)
+        hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, true>), grid, dim3(256), 0, st,
+                           a, tx, total, per_xcd);
+    else
+        hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, false>), grid, dim3(256), 0, st,
+                           a, tx, total, per_xcd);
+}
+
+template <typename T, bool WIDE>
+static void launch_w(const MarchArgs& a, int filter, bool phong, bool count, hipStream_t st) {
+    if (filter == 0) {
+        if (phong) launch_t<T, WIDE, 0, true>(a, count, st);
+        else launch_t<T, WIDE, 0, false>(a, count, st);
+    } else {
+        if (phong) launch_t<T, WIDE, 1, true>(a, count, st);
+        else launch_t<T, WIDE, 1, false>(a, count, st);
+    }
+}
+
+void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bool count,
+                       bool wide, hipStream_t st) {
+    if (dtype == 0) {
+        if (wide) launch_w<uint8_t, true>(a, filter, phong, count, st);
+        else launch_w<uint8_t, false>(a, filter, phong, count, st);
+    } else {
+        if (wide) launch_w<uint16_t, true>(a, filter, phong, count, st);
+        else launch_w<uint16_t, false>(a, filter, phong, count, st);
+    }
+}
+
+}  // namespace hipdvr

@@ -1,15 +1,24 @@
 // volume.h -- device representation of a volume (internal).
 //
-// HBM layout:
+// HBM layout of one volume:
 //   d_linear  x-fastest copy, z*X*Y + y*X + x (the VolumeRAM layout,
-//             include/voreen/core/datastructures/volume/volumeatomic.h:599-608)
-//   d_bricks  B^3 bricks (B = 2^brick_log2) stored with a +1 apron on the
-//             upper side of every axis, (B+1)^3 voxels per brick, bricks in
-//             Morton (Z-order) of their brick coordinates.  Every trilinear
-//             footprint whose lower corner lies in a brick is inside that
-//             brick's apron-extended copy.
-//   d_minmax  per brick (min, max) over the apron-extended voxels, as raw
-//             voxel values (uint32 pairs).
+//             include/voreen/core/datastructures/volume/volumeatomic.h:599-608);
+//             read by the reference-layout kernel.
+//   d_tiled   the same voxels in 4x4x4 blocks: dims padded up to multiples of 4,
+//             blocks in x-fastest block order, voxels x-fastest inside a block.
+//             One block of uint16 is 128 bytes = one cache line, so the 8
+//             corners of a trilinear footprint touch 1..8 lines whatever the
+//             ray direction.  The element offset is separable per axis:
+//               off(x,y,z) = ox(x) + oy(y) + oz(z)
+//               ox(x) = (x>>2)*64 + (x&3)
+//               oy(y) = (y>>2)*64*NBX + (y&3)*4
+//               oz(z) = (z>>2)*64*NBX*NBY + (z&3)*16
+//             Padding voxels replicate the nearest edge voxel (never read:
+//             every coordinate is clamped to dims-1 before the fetch).
+//   d_minmax  per brick of B^3 voxels (B = 2^brick_log2), the (min, max) raw
+//             voxel value over the brick extended by one voxel on the upper
+//             side of every axis (clamped to the volume): every trilinear or
+//             nearest footprint whose lower corner lies in the brick is inside.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,11 +42,14 @@ struct hipdvr_volume {
     hipdvr_ctx* ctx = nullptr;
     hipdvr::VolumeDesc desc{};
     void* d_linear = nullptr;
-    void* d_bricks = nullptr;
+    void* d_tiled = nullptr;
     uint2* d_minmax = nullptr;
+    uint8_t* d_occ = nullptr;        // per-brick occupancy for the last TF mapping
+    uint64_t occ_key[4] = {0, 0, 0, 0};  // TF identity/domain/rwm the occupancy belongs to
+    uint32_t nblocks[3] = {0, 0, 0}; // 4^3 blocks per axis
     uint32_t nbricks[3] = {0, 0, 0};
     uint64_t brick_count = 0;
-    uint64_t brick_stride = 0;   // voxels per stored brick ((B+1)^3, padded)
+    uint64_t tiled_elems = 0;
 };
 
 namespace hipdvr {
@@ -45,5 +57,19 @@ namespace hipdvr {
 hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volume_desc* desc,
                              const void* voxels, bool on_device);
 void volume_destroy(hipdvr_volume* vol);
+
+// Per-brick occupancy for a TF mapping: a brick is empty when no TF entry
+// within one index of the range its [min, max] (widened by 1e-5 in
+// normalized intensity) maps to has a non-zero alpha byte.
+struct OccArgs {
+    const uint2* minmax;
+    uint8_t* occ;
+    uint64_t count;
+    int32_t dtype;
+    float rwm_scale, rwm_offset, dom_lo, dom_hi;
+    int32_t tf_width;
+    const uint32_t* alpha_prefix;   // [w+1]: number of non-zero alpha bytes in [0, i)
+};
+void launch_occupancy(const OccArgs& a, hipStream_t st);
 
 }  // namespace hipdvr

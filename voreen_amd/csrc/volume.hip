@@ -1,12 +1,129 @@
-// volume.hip -- volume upload: linear copy, bricked copy, brick min/max.
+// volume.hip -- volume upload: linear copy, 4^3-block tiled copy, brick
+// min/max, and the per-TF brick occupancy used for exact empty-space skipping.
 #include <hip/hip_runtime.h>
 
 #include <string>
 
 #include "errors.h"
+#include "march_common.h"
 #include "volume.h"
 
 namespace hipdvr {
+
+namespace {
+
+// One lane per tiled element: gather from the linear copy (clamped), write
+// the tiled copy contiguously.
+template <typename T>
+__global__ __launch_bounds__(256) void tile_kernel(const T* __restrict__ src, T* __restrict__ dst,
+                                                   uint32_t X, uint32_t Y, uint32_t Z, uint32_t nbx,
+                                                   uint32_t nby, uint64_t total) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t in = (uint32_t)(i & 63);
+    const uint64_t blk = i >> 6;
+    const uint32_t bx = (uint32_t)(blk % nbx);
+    const uint64_t r = blk / nbx;
+    const uint32_t by = (uint32_t)(r % nby);
+    const uint32_t bz = (uint32_t)(r / nby);
+    uint32_t x = bx * 4 + (in & 3), y = by * 4 + ((in >> 2) & 3), z = bz * 4 + (in >> 4);
+    x = x < X ? x : X - 1;
+    y = y < Y ? y : Y - 1;
+    z = z < Z ? z : Z - 1;
+    dst[i] = src[((uint64_t)z * Y + y) * X + x];
+}
+
+// One workgroup per brick: min/max over [b*B, b*B + B] per axis, clamped.
+template <typename T>
+__global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ src, uint2* __restrict__ mm,
+                                                     uint32_t X, uint32_t Y, uint32_t Z, uint32_t bl,
+                                                     uint32_t nbx, uint32_t nby) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const uint32_t B1 = (1u << bl) + 1;
+    const uint32_t n = B1 * B1 * B1;
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        uint32_t x = (bx << bl) + k % B1;
+        uint32_t y = (by << bl) + (k / B1) % B1;
+        uint32_t z = (bz << bl) + k / (B1 * B1);
+        x = x < X ? x : X - 1;
+        y = y < Y ? y : Y - 1;
+        z = z < Z ? z : Z - 1;
+        const uint32_t v = src[((uint64_t)z * Y + y) * X + x];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    __shared__ uint32_t s_lo[4], s_hi[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[w] = lo;
+        s_hi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            lo = s_lo[i] < lo ? s_lo[i] : lo;
+            hi = s_hi[i] > hi ? s_hi[i] : hi;
+        }
+        mm[b] = make_uint2(lo, hi);
+    }
+}
+
+template <typename T>
+void build_layouts(hipdvr_volume* v, hipStream_t st) {
+    const uint32_t X = v->desc.dims[0], Y = v->desc.dims[1], Z = v->desc.dims[2];
+    const uint64_t total = v->tiled_elems;
+    hipLaunchKernelGGL(tile_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       (const T*)v->d_linear, (T*)v->d_tiled, X, Y, Z, v->nblocks[0], v->nblocks[1],
+                       total);
+    hip_check(hipGetLastError(), "tile kernel launch");
+    hipLaunchKernelGGL(minmax_kernel<T>, dim3((unsigned)v->brick_count), dim3(256), 0, st,
+                       (const T*)v->d_linear, v->d_minmax, X, Y, Z, v->desc.brick_log2,
+                       v->nbricks[0], v->nbricks[1]);
+    hip_check(hipGetLastError(), "min/max kernel launch");
+}
+
+__device__ __forceinline__ float norm_raw(uint32_t v, int dtype) {
+    return dtype == 0 ? VoxNorm<uint8_t>::apply(v) : VoxNorm<uint16_t>::apply(v);
+}
+
+__device__ __forceinline__ int map_index(float n, const OccArgs& a) {
+    const float rw = (n * a.rwm_scale) + a.rwm_offset;
+    const float x = rw_to_norm(rw, a.dom_lo, a.dom_hi);
+    const int w1 = a.tf_width - 1;
+    const int idx = iround_tgt(x * (float)w1);
+    return idx < 0 ? 0 : (idx > w1 ? w1 : idx);
+}
+
+__global__ __launch_bounds__(256) void occupancy_kernel(const OccArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const uint2 mm = a.minmax[i];
+    const float n0 = norm_raw(mm.x, a.dtype) - 1e-5f;
+    const float n1 = norm_raw(mm.y, a.dtype) + 1e-5f;
+    int i0 = map_index(n0, a), i1 = map_index(n1, a);
+    if (i0 > i1) {
+        const int t = i0;
+        i0 = i1;
+        i1 = t;
+    }
+    i0 = i0 > 0 ? i0 - 1 : 0;
+    i1 = i1 < a.tf_width - 1 ? i1 + 1 : a.tf_width - 1;
+    a.occ[i] = (a.alpha_prefix[i1 + 1] - a.alpha_prefix[i0]) != 0u ? 1 : 0;
+}
+
+}  // namespace
+
+void launch_occupancy(const OccArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(occupancy_kernel, dim3((unsigned)((a.count + 255) / 256)), dim3(256), 0, st, a);
+}
 
 hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volume_desc* d,
                              const void* voxels, bool on_device) {
@@ -15,15 +132,20 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
             fail(HIPDVR_ERR_INVALID, "volume dimensions must be in [1, 2^20]");
     if (d->dtype != HIPDVR_U8 && d->dtype != HIPDVR_U16)
         fail(HIPDVR_ERR_INVALID, "dtype must be HIPDVR_U8 or HIPDVR_U16");
-    uint32_t bl = d->brick_log2 ? d->brick_log2 : 4;
-    if (bl < 3 || bl > 5) fail(HIPDVR_ERR_INVALID, "brick_log2 must be 3, 4 or 5");
+    uint32_t bl = d->brick_log2 ? d->brick_log2 : 3;
+    if (bl < 2 || bl > 5) fail(HIPDVR_ERR_INVALID, "brick_log2 must be in [2, 5]");
     hipdvr_volume* v = new hipdvr_volume();
     v->ctx = ctx;
+    v->tiled_elems = 64;
+    v->brick_count = 1;
     for (int i = 0; i < 3; ++i) {
         v->desc.dims[i] = d->dims[i];
         v->desc.spacing[i] = d->spacing[i];
         v->desc.offset[i] = d->offset[i];
+        v->nblocks[i] = (d->dims[i] + 3) / 4;
         v->nbricks[i] = (d->dims[i] + (1u << bl) - 1) >> bl;
+        v->tiled_elems *= v->nblocks[i];
+        v->brick_count *= v->nbricks[i];
     }
     v->desc.dtype = d->dtype;
     v->desc.rwm_scale = d->rwm_scale;
@@ -33,9 +155,14 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
     const size_t bytes = (size_t)d->dims[0] * d->dims[1] * d->dims[2] * vsz;
     try {
         hip_check(hipMalloc(&v->d_linear, bytes), "hipMalloc(volume)");
+        hip_check(hipMalloc(&v->d_tiled, v->tiled_elems * vsz), "hipMalloc(tiled volume)");
+        hip_check(hipMalloc((void**)&v->d_minmax, v->brick_count * sizeof(uint2)), "hipMalloc(minmax)");
+        hip_check(hipMalloc((void**)&v->d_occ, v->brick_count), "hipMalloc(occupancy)");
         hip_check(hipMemcpyAsync(v->d_linear, voxels, bytes,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
-               "volume upload");
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
+                  "volume upload");
+        if (d->dtype == HIPDVR_U8) build_layouts<uint8_t>(v, st);
+        else build_layouts<uint16_t>(v, st);
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     } catch (...) {
         volume_destroy(v);
@@ -47,8 +174,9 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
 void volume_destroy(hipdvr_volume* v) {
     if (!v) return;
     if (v->d_linear) (void)hipFree(v->d_linear);
-    if (v->d_bricks) (void)hipFree(v->d_bricks);
+    if (v->d_tiled) (void)hipFree(v->d_tiled);
     if (v->d_minmax) (void)hipFree(v->d_minmax);
+    if (v->d_occ) (void)hipFree(v->d_occ);
     delete v;
 }
 
