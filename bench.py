@@ -117,6 +117,8 @@ def main():
     fetched = sum(s["fetched"] for s in per_view)
     shaded = sum(s["shaded"] for s in per_view)
     rays = sum(s["rays"] for s in per_view)
+    lane_slots = sum(s["lane_slots"] for s in per_view)
+    shade_slots = sum(s["shade_slots"] for s in per_view)
     setup_s = time.perf_counter() - t_setup
 
     for _ in range(args.warmup):
@@ -168,6 +170,8 @@ def main():
         "shaded_per_frame": shaded // nviews,
         "rays_per_frame": rays // nviews,
         "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 4),
+        "simd_efficiency": round(samples / max(lane_slots, 1), 4),
+        "shade_efficiency": round(shaded / max(shade_slots, 1), 4),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),

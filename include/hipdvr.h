@@ -136,6 +136,9 @@ typedef struct hipdvr_stats {
     uint64_t samples;       /* ray-march loop iterations (the reference's steps) */
     uint64_t fetched;       /* samples whose volume value was reconstructed */
     uint64_t shaded;        /* samples shaded (Phong) */
+    uint64_t lane_slots;    /* 64 x loop iterations of each wavefront (SIMD-efficiency
+                               diagnostic: samples / lane_slots) */
+    uint64_t shade_slots;   /* 64 x iterations in which any lane of the wavefront shaded */
     float kernel_ms;        /* HIP-event time of the march kernel on the ctx stream */
 } hipdvr_stats;
 

@@ -1,0 +1,186 @@
+"""HIPRaycaster, the Voreen processor, driven through the Voreen shim.
+
+CPU tests check the drop-in contract: the reference's port and property ids,
+option keys and defaults (src/core/processors/volumeraycaster.cpp:42-157,
+modules/base/processors/render/cpuraycaster.cpp:44-76), isReady
+(cpuraycaster.cpp:82-84), the sampling step (cpuraycaster.cpp:104), and that
+a failure is logged and leaves a cleared output (no exception escapes).  GPU
+tests render through process() and compare with the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import hipraycaster_driver as D  # noqa: E402
+
+
+@pytest.fixture
+def rc():
+    r = D.Raycaster()
+    yield r
+    r.close()
+
+
+def test_ports_and_properties_match_the_reference(rc):
+    d = rc.describe()
+    assert d["module"] == "HIPRaycaster" and d["registered"] == "HIPRaycaster"
+    assert d["class"] == "HIPRaycaster" and d["category"] == "Raycasting"
+    ports = {p["id"]: p["in"] for p in d["ports"]}
+    assert ports == {"volumehandle.volumehandle": True, "image.entrypoints": True,
+                     "image.exitpoints": True, "image.output": False,
+                     "volumehandle.gradientvolumehandle": True}
+    props = d["properties"]
+    for pid in ("samplingRate", "interactionCoarseness", "interactionQuality", "tFuncType",
+                "tFunc1DKeys", "tFunc2DPrimitives", "textureFilterMode_", "classification",
+                "preIntegrationTableSize", "camera", "gradient", "shading", "compositing",
+                "lightPosition", "lightAmbient", "lightDiffuse", "lightSpecular",
+                "materialShininess"):
+        assert pid in props, pid
+    assert props["samplingRate"]["value"] == "2"
+    assert props["textureFilterMode_"]["options"] == ["nearest", "linear"]
+    assert props["textureFilterMode_"]["value"] == "linear"
+    assert props["classification"]["options"] == [
+        "transfer-function", "pre-integrated-gpu", "pre-integrated-cpu",
+        "pre-integrated-approximation"]
+    assert props["classification"]["value"] == "transfer-function"
+    assert props["compositing"]["options"] == ["dvr", "mip", "mop", "mida", "iso", "fhp", "fhn"]
+    assert props["shading"]["value"] == "none"
+    assert "phong" in props["shading"]["options"]
+    assert props["preIntegrationTableSize"]["options"] == ["deriveFromBitDepth", "256", "512", "1024"]
+    assert props["tFuncType"]["options"] == ["1DKeys", "2DPrimitives"]
+    assert props["lightPosition"]["value"] == "2.3 1.5 1.5 1"
+    assert props["materialShininess"]["value"] == "60"
+    hip = [k for k in props if k.startswith("hip")]
+    assert sorted(hip) == ["hipBrickSize", "hipDevice", "hipKernel"]
+
+
+def test_is_ready_needs_volume_and_both_inports(rc):
+    assert not rc.is_ready()
+    rc.set_volume(np.zeros((4, 4, 4), np.uint8))
+    assert not rc.is_ready()
+    rc.connect(True, False)
+    assert not rc.is_ready()
+    rc.connect(True, True)
+    assert rc.is_ready()
+
+
+def test_sampling_step_is_the_reference_rule(rc):
+    rc.set_volume(np.zeros((96, 114, 128), np.uint16))   # max dim 128
+    assert rc.step() == np.float32(1.0) / (np.float32(128) * np.float32(2.0))
+    assert rc.set("samplingRate", 4.0) == 0
+    assert rc.step() == np.float32(1.0) / (np.float32(128) * np.float32(4.0))
+
+
+def test_unsupported_options_log_and_clear(rc):
+    rc.set_volume(np.full((8, 8, 8), 100, np.uint8))
+    en = np.zeros((4, 4, 4), np.float32)
+    en[..., :3] = 0.25
+    en[..., 3] = 1.0
+    ex = en.copy()
+    ex[..., :3] = 0.75
+    rc.set_images(en, ex)
+    rc.set_tf([(0.0, (255, 255, 255, 255))])
+    assert rc.set("compositing", "mip") == 0
+    out, nerr = rc.process()
+    assert nerr == 1 and "compositing" in rc.last_error() and np.all(out == 0)
+    rc.set("compositing", "dvr")
+    rc.set("shading", "toon")
+    out, nerr = rc.process()
+    assert nerr == 1 and "shading" in rc.last_error() and np.all(out == 0)
+    assert rc.set("shading", "no-such-mode") == 2
+
+
+def test_process_without_device_logs_an_error():
+    """On a machine without a GPU, process() must log and clear the output."""
+    from voreen_amd import hipdvr as H
+    import ctypes as C
+    h = C.c_void_p()
+    rc0 = H.load().hipdvr_create(C.byref(h), 0)
+    if rc0 == H.OK:
+        H.load().hipdvr_destroy(h)
+        pytest.skip("a GPU is present")
+    r = D.Raycaster()
+    try:
+        r.set_volume(np.full((8, 8, 8), 100, np.uint8))
+        en = np.full((2, 2, 4), 0.3, np.float32)
+        r.set_images(en, en)
+        r.set_tf([(0.0, (255, 255, 255, 255))])
+        out, nerr = r.process()
+        assert nerr == 1 and "hipdvr_create failed" in r.last_error() and np.all(out == 0)
+    finally:
+        r.close()
+
+
+def test_pre_integrated_gpu_falls_back_to_cpu(rc):
+    """cpuraycaster.cpp:123-129: 'pre-integrated-gpu' selects '-cpu' and invalidates."""
+    rc.set_volume(np.full((8, 8, 8), 100, np.uint8))
+    en = np.full((2, 2, 4), 0.3, np.float32)
+    rc.set_images(en, en)
+    rc.set("classification", "pre-integrated-gpu")
+    out, nerr = rc.process()
+    assert nerr == 1
+    assert rc.describe()["properties"]["classification"]["value"] == "pre-integrated-cpu"
+
+
+# ---------------------------------------------------------------------------
+# GPU: process() renders what the oracle renders
+# ---------------------------------------------------------------------------
+
+def _c0_case(oracle, size=128):
+    from voreen_amd import workloads as W
+    vol = W.make_volume("c0", dims=(96, 80, 112))
+    keys, _ = W.make_tf("c0")
+    cam = W.orbit_camera(6, 36)
+    llf, urb = W.world_box((96, 80, 112))
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, size, size, llf=llf, urb=urb)
+    return vol, keys, en, ex, cam
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["auto", "reference", "bricked"])
+def test_process_matches_oracle(rc, oracle, kernel):
+    vol, keys, en, ex, cam = _c0_case(oracle)
+    print(f"[case] HIPRaycaster.process kernel {kernel}", flush=True)
+    rc.set_volume(vol)
+    rc.set_tf(keys)
+    rc.set_images(en, ex)
+    rc.set("hipKernel", kernel)
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    tf = oracle.tf_build(keys, 256)          # uint8 volume -> 256-entry table
+    step = rc.step()
+    ref, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+    assert np.abs(out - ref).max() <= 1e-3
+    assert rc.stats()["samples"] == n
+    # second frame reuses the uploads (no re-upload needed) and is identical
+    out2, nerr = rc.process()
+    assert nerr == 0 and np.array_equal(out, out2)
+
+
+@pytest.mark.gpu
+def test_process_phong_and_preintegration(rc, oracle):
+    vol, keys, en, ex, cam = _c0_case(oracle)
+    rc.set_volume(vol)
+    rc.set_tf(keys)
+    rc.set_images(en, ex)
+    tf = oracle.tf_build(keys, 256)
+    step = rc.step()
+    print("[case] HIPRaycaster.process phong", flush=True)
+    rc.set("shading", "phong")
+    rc.set("camera", " ".join(str(v) for v in cam.eye))
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    ref, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, shading=1,
+                           light=dict(camera=tuple(cam.eye)))
+    assert np.mean(np.abs(out - ref).max(axis=2) > 1e-3) < 0.002
+    print("[case] HIPRaycaster.process pre-integrated-cpu", flush=True)
+    rc.set("shading", "none")
+    rc.set("classification", "pre-integrated-cpu")
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    table = oracle.preint_build(keys, 256, step)
+    ref, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, classification=1, preint=table)
+    assert np.abs(out - ref).max() <= 1e-3

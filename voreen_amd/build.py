@@ -50,6 +50,28 @@ def _compile(src):
     return obj
 
 
+MODULE = os.path.join(HERE, "hipraycaster")
+MODULE_OUT = os.path.join(MODULE, "libhipraycaster_shim.so")
+
+
+def build_module(verbose=True):
+    """HIPRaycaster processor + module compiled against the Voreen shim, with
+    the test driver (host code only, g++), linked to libhipdvr.so."""
+    srcs = [os.path.join(MODULE, p) for p in ("shim/shim.cpp", "hipraycastermodule.cpp",
+                                               "processors/hipraycaster.cpp", "test/driver.cpp")]
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra",
+           "-Wno-unused-parameter", "-I", os.path.join(MODULE, "shim"),
+           "-I", os.path.join(HERE, "..", "include"), "-I", MODULE, *srcs,
+           "-L", HERE, "-lhipdvr", "-Wl,-rpath,$ORIGIN/..", "-o", MODULE_OUT + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"module build failed:\n{r.stderr}")
+    os.replace(MODULE_OUT + ".tmp", MODULE_OUT)
+    if verbose:
+        print(f"built {MODULE_OUT}")
+    return MODULE_OUT
+
+
 def build(verbose=True):
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = sources()
@@ -63,6 +85,7 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT} from {len(srcs)} sources")
+    build_module(verbose)
     return OUT
 
 

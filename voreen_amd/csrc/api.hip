@@ -28,7 +28,7 @@ struct hipdvr_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    unsigned long long* d_counters = nullptr;   // 4 x u64
+    unsigned long long* d_counters = nullptr;   // kNumCounters x u64
     std::string last_error;
     // per-launch timing (hipdvr_timing_begin/read)
     std::vector<hipEvent_t> tev;                 // 2 per slot
@@ -186,7 +186,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     validate_params(p, tf);
     set_device(ctx);
     const bool count = stats != nullptr;
-    if (count) launch_zero_u64(ctx->d_counters, 4, ctx->stream);
+    if (count) launch_zero_u64(ctx->d_counters, kNumCounters, ctx->stream);
     MarchArgs a;
     fill_march_args(a, vol, tf, p, d_entry, d_exit, d_out, w, h, ctx->d_counters);
     const bool phong = p->shading == HIPDVR_SHADE_PHONG;
@@ -214,7 +214,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     }
     if (count) {
         hip_check(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
-        unsigned long long c[4];
+        unsigned long long c[kNumCounters];
         hip_check(hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream),
                   "hipMemcpyAsync(counters)");
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -224,6 +224,8 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         stats->samples = c[1];
         stats->fetched = c[2];
         stats->shaded = c[3];
+        stats->lane_slots = c[4];
+        stats->shade_slots = c[5];
         stats->kernel_ms = ms;
     }
 }
@@ -289,7 +291,7 @@ int hipdvr_create(hipdvr_ctx** out, int device) {
         ctx->stream = ctx->own_stream;
         hip_check(hipEventCreate(&ctx->ev0), "hipEventCreate");
         hip_check(hipEventCreate(&ctx->ev1), "hipEventCreate");
-        hip_check(hipMalloc((void**)&ctx->d_counters, 4 * sizeof(unsigned long long)), "hipMalloc(counters)");
+        hip_check(hipMalloc((void**)&ctx->d_counters, kNumCounters * sizeof(unsigned long long)), "hipMalloc(counters)");
     });
     if (rc != HIPDVR_OK) {
         std::fprintf(stderr, "hipdvr_create: %s\n", ctx->last_error.c_str());

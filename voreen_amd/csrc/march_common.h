@@ -17,6 +17,7 @@ namespace hipdvr {
 constexpr float kErtOpacity = 0.95f;        // mod_raysetup.frag:36, cpuraycaster.cpp:310
 constexpr int kMaxTfWidth = 4096;           // transfunc1dkeysproperty.cpp:95
 constexpr int kDefaultLoopCap = 255 * 255;  // cpuraycaster.cpp:239
+constexpr int kNumCounters = 6;
 
 // Arguments of one march launch (plain data; passed by value as kernarg).
 struct MarchArgs {
@@ -47,7 +48,7 @@ struct MarchArgs {
     const float4* exit;
     float4* out;
     int32_t w, h;
-    unsigned long long* counters;  // [rays, samples, fetched, shaded]
+    unsigned long long* counters;  // [rays, samples, fetched, shaded, lane_slots, shade_slots]
     float alpha_corr[256];      // 1 - pow(1 - a/255, step*200) per alpha byte
 };
 
@@ -237,6 +238,16 @@ __device__ __forceinline__ float ray_setup(const float first[3], const float las
 __device__ __forceinline__ bool is_background(float4 f, float4 l) {
     return f.x == 0.f && f.y == 0.f && f.z == 0.f && f.w == 0.f && l.x == 0.f && l.y == 0.f &&
            l.z == 0.f && l.w == 0.f;
+}
+
+// 64-lane maximum of a per-lane counter, times 64, added once per wavefront.
+__device__ __forceinline__ void wave_add_max64(unsigned long long* dst, unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, 64ull * v);
 }
 
 // 64-lane sum of a per-lane counter; lane 0 adds it to the global counter.

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
     const uint32_t bl = a.brick_log2;
     const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
 
-    unsigned long long n_samples = 0, n_fetched = 0, n_shaded = 0, n_rays = 0;
+    unsigned long long n_slots = 0, n_shade_slots = 0, n_samples = 0, n_fetched = 0, n_shaded = 0, n_rays = 0;
     if (inside) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix];
@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
             bool finished = false;
             for (int loop = 0; !finished && loop < cap; ++loop) {
                 ++n_samples;
+                bool shade_now = false;
                 const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1],
                                     first[2] + t * dir[2]};
                 float vp[3];
@@ -141,6 +142,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
                     const uint32_t c8 = s_tf[tf_index(inten, a)];
                     const float ca = s_alpha[c8 >> 24];
                     if (ca > 0.0f) {
+                        shade_now = PHONG;
                         float cr = byte_norm(c8 & 0xffu);
                         float cg = byte_norm((c8 >> 8) & 0xffu);
                         float cb = byte_norm((c8 >> 16) & 0xffu);
@@ -185,6 +187,10 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
                         finished = true;
                     }
                 }
+                if (COUNT) {
+                    ++n_slots;
+                    n_shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
+                }
                 t += a.step;
                 finished = finished || (t > tend);
             }
@@ -197,6 +203,8 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
         wave_add(a.counters + 1, n_samples);
         wave_add(a.counters + 2, n_fetched);
         wave_add(a.counters + 3, n_shaded);
+        wave_add_max64(a.counters + 4, n_slots);
+        wave_add_max64(a.counters + 5, n_shade_slots);
     }
 }
 

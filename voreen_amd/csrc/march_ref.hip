@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
     f.X = a.dims[0];
     f.XY = (uint64_t)a.dims[0] * a.dims[1];
 
-    unsigned long long n_samples = 0, n_shaded = 0, n_rays = 0;
+    unsigned long long n_slots = 0, n_shade_slots = 0, n_samples = 0, n_shaded = 0, n_rays = 0;
     if (inside) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix];
@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
             bool finished = false;
             for (int loop = 0; !finished && loop < cap; ++loop) {
                 ++n_samples;
+                bool shade_now = false;
                 const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1],
                                     first[2] + t * dir[2]};
                 const float inten = reconstruct<T, FILTER>(f, a, s);
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
                     cb = byte_norm((c8 >> 16) & 0xffu);
                 }
                 if (ca > 0.0f) {
+                    shade_now = PHONG;
                     if (PHONG) {
                         float gr[3];
                         if (t == 0.0f) {
@@ -152,6 +154,10 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
                     A = 1.0f;
                     finished = true;
                 }
+                if (COUNT) {
+                    ++n_slots;
+                    n_shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
+                }
                 t += a.step;
                 finished = finished || (t > tend);
             }
@@ -164,6 +170,8 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
         wave_add(a.counters + 1, n_samples);
         wave_add(a.counters + 2, n_samples);
         wave_add(a.counters + 3, n_shaded);
+        wave_add_max64(a.counters + 4, n_slots);
+        wave_add_max64(a.counters + 5, n_shade_slots);
     }
 }
 

@@ -58,7 +58,8 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("fetched", C.c_uint64),
-                ("shaded", C.c_uint64), ("kernel_ms", C.c_float)]
+                ("shaded", C.c_uint64), ("lane_slots", C.c_uint64), ("shade_slots", C.c_uint64),
+                ("kernel_ms", C.c_float)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

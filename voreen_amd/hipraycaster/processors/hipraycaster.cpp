@@ -1,0 +1,343 @@
+#include "hipraycaster.h"
+
+#include "voreen/core/datastructures/transfunc/1d/1dkeys/transfunc1dkeys.h"
+#include "voreen/core/datastructures/transfunc/1d/preintegrationtable.h"
+#include "voreen/core/datastructures/volume/volumeram.h"
+#include "tgt/texture.h"
+
+#include <cstring>
+
+namespace voreen {
+
+const std::string HIPRaycaster::loggerCat_("voreen.hipraycaster.HIPRaycaster");
+
+HIPRaycaster::HIPRaycaster()
+    : VolumeRaycaster()
+    , gradientVolumePort_(Port::INPORT, "volumehandle.gradientvolumehandle", "Gradient Volume Input")
+    , tFuncType_("tFuncType", "TF Type")
+    , tFunc1DKeys_("tFunc1DKeys", "1D Color Map")
+    , tFunc2DPrimitives_("tFunc2DPrimitives", "2D Color Map")
+    , texFilterMode_("textureFilterMode_", "Texture Filtering")
+    , preIntegrationTableSize_("preIntegrationTableSize", "Width of pre-integration table")
+    , camera_("camera", "Camera", tgt::Camera(tgt::vec3(0.f, 0.f, 3.5f), tgt::vec3(0.f, 0.f, 0.f),
+                                              tgt::vec3(0.f, 1.f, 0.f)), true)
+    , hipDevice_("hipDevice", "HIP device", 0, 0, 63, Processor::INVALID_PROGRAM)
+    , hipBrickSize_("hipBrickSize", "Skipping brick size", Processor::INVALID_PROGRAM)
+    , hipKernel_("hipKernel", "March kernel")
+    , ctx_(0)
+    , dvrVolume_(0)
+    , dvrTF_(0)
+    , uploadedVolume_(0)
+    , uploadedDomain_(0.f, 0.f)
+    , uploadedBrickLog2_(0)
+    , uploadedPreintStep_(-1.f)
+{
+    // ports and properties of CPURaycaster (cpuraycaster.cpp:44-76)
+    addPort(gradientVolumePort_);
+
+    addProperty(tFuncType_);
+    addProperty(tFunc1DKeys_);
+    tFuncType_.addTransFuncProperty(&tFunc1DKeys_);
+    addProperty(tFunc2DPrimitives_);
+    tFuncType_.addTransFuncProperty(&tFunc2DPrimitives_);
+
+    texFilterMode_.addOption("nearest", "Nearest", GL_NEAREST);
+    texFilterMode_.addOption("linear", "Linear", GL_LINEAR);
+    texFilterMode_.selectByKey("linear");
+    addProperty(texFilterMode_);
+
+    addProperty(classificationMode_);
+
+    preIntegrationTableSize_.addOption("deriveFromBitDepth", "Derive from bit depth", 0);
+    preIntegrationTableSize_.addOption("256", "256", 256);
+    preIntegrationTableSize_.addOption("512", "512", 512);
+    preIntegrationTableSize_.addOption("1024", "1024", 1024);
+    preIntegrationTableSize_.select("deriveFromBitDepth");
+    addProperty(preIntegrationTableSize_);
+
+    // camera, shading and light as SingleVolumeRaycaster adds them
+    // (modules/base/processors/render/singlevolumeraycaster.cpp:107-130); the
+    // default shading is "none" so the default image is CPURaycaster's
+    addProperty(camera_);
+    addProperty(gradientMode_);
+    addProperty(shadeMode_);
+    shadeMode_.select("none");
+    addProperty(compositingMode1_);
+    addProperty(lightPosition_);
+    addProperty(lightAmbient_);
+    addProperty(lightDiffuse_);
+    addProperty(lightSpecular_);
+    addProperty(materialShininess_);
+
+    // HIP-specific
+    addProperty(hipDevice_);
+    hipBrickSize_.addOption("8", "8^3", 3);
+    hipBrickSize_.addOption("16", "16^3", 4);
+    hipBrickSize_.addOption("32", "32^3", 5);
+    hipBrickSize_.selectByKey("8");
+    addProperty(hipBrickSize_);
+    hipKernel_.addOption("auto", "Fastest exact kernel");
+    hipKernel_.addOption("reference", "Reference layout, no skipping");
+    hipKernel_.addOption("bricked", "Bricked, empty-space skipping");
+    hipKernel_.selectByKey("auto");
+    addProperty(hipKernel_);
+
+    std::memset(&lastStats_, 0, sizeof(lastStats_));
+}
+
+HIPRaycaster::~HIPRaycaster() {
+    releaseDevice();
+}
+
+Processor* HIPRaycaster::create() const {
+    return new HIPRaycaster();
+}
+
+bool HIPRaycaster::isReady() const {
+    return volumeInport_.hasData() && entryPort_.isConnected() && exitPort_.isConnected();
+}
+
+void HIPRaycaster::initialize() {
+    VolumeRaycaster::initialize();
+}
+
+void HIPRaycaster::deinitialize() {
+    releaseDevice();
+    VolumeRaycaster::deinitialize();
+}
+
+void HIPRaycaster::releaseDevice() {
+    if (dvrTF_) hipdvr_tf_destroy(dvrTF_);
+    if (dvrVolume_) hipdvr_volume_destroy(dvrVolume_);
+    if (ctx_) hipdvr_destroy(ctx_);
+    dvrTF_ = 0;
+    dvrVolume_ = 0;
+    ctx_ = 0;
+    uploadedVolume_ = 0;
+    uploadedTF_.clear();
+    uploadedPreintKey_.clear();
+}
+
+bool HIPRaycaster::fail(const std::string& msg) {
+    LERROR(msg);
+    return false;
+}
+
+bool HIPRaycaster::ensureContext() {
+    if (ctx_) return true;
+    int rc = hipdvr_create(&ctx_, hipDevice_.get());
+    if (rc != HIPDVR_OK) {
+        ctx_ = 0;
+        return fail("hipdvr_create failed on device " + std::to_string(hipDevice_.get()) +
+                    " (status " + std::to_string(rc) + ")");
+    }
+    return true;
+}
+
+// Upload on a new volume or a new brick size (INVALID_PROGRAM).
+bool HIPRaycaster::ensureVolume(const VolumeBase* vb, const VolumeRAM* vram) {
+    const int brickLog2 = hipBrickSize_.getValue();
+    if (dvrVolume_ && uploadedVolume_ == vb && uploadedBrickLog2_ == brickLog2 &&
+        !volumeInport_.hasChanged())
+        return true;
+    if (dvrVolume_) {
+        hipdvr_volume_destroy(dvrVolume_);
+        dvrVolume_ = 0;
+        uploadedVolume_ = 0;
+    }
+    const std::string type = vram->getBaseType();
+    hipdvr_volume_desc d;
+    std::memset(&d, 0, sizeof(d));
+    if (type == "uint8") d.dtype = HIPDVR_U8;
+    else if (type == "uint16") d.dtype = HIPDVR_U16;
+    else return fail("HIPRaycaster supports uint8 and uint16 volumes, got " + type);
+    if (vram->getNumChannels() != 1) return fail("HIPRaycaster supports single-channel volumes");
+    const tgt::svec3 dims = vram->getDimensions();
+    for (int i = 0; i < 3; ++i) d.dims[i] = static_cast<uint32_t>(dims[i]);
+    const RealWorldMapping rwm = vb->getRealWorldMapping();
+    d.rwm_scale = rwm.getScale();
+    d.rwm_offset = rwm.getOffset();
+    const tgt::vec3 sp = vb->getSpacing(), off = vb->getOffset();
+    for (int i = 0; i < 3; ++i) {
+        d.spacing[i] = sp[i];
+        d.offset[i] = off[i];
+    }
+    d.brick_log2 = static_cast<uint32_t>(brickLog2);
+    const int rc = hipdvr_volume_create(ctx_, &d, vram->getData(), &dvrVolume_);
+    if (rc != HIPDVR_OK) {
+        dvrVolume_ = 0;
+        return fail(std::string("volume upload failed: ") + hipdvr_last_error(ctx_));
+    }
+    uploadedVolume_ = vb;
+    uploadedBrickLog2_ = brickLog2;
+    return true;
+}
+
+// Upload the RGBA8 lookup table of tFunc1DKeys_ when it changed.
+bool HIPRaycaster::ensureTransFunc(tgt::Texture* tfTexture, const tgt::vec2& domain) {
+    const int width = tfTexture->getWidth();
+    const GLubyte* data = tfTexture->getCpuTextureData();
+    if (!data || width < 2) return fail("transfer function has no texture data");
+    const size_t bytes = 4 * static_cast<size_t>(width);
+    if (dvrTF_ && uploadedTF_.size() == bytes && std::memcmp(uploadedTF_.data(), data, bytes) == 0 &&
+        uploadedDomain_ == domain)
+        return true;
+    if (dvrTF_) {
+        hipdvr_tf_destroy(dvrTF_);
+        dvrTF_ = 0;
+        uploadedPreintKey_.clear();
+    }
+    const int rc = hipdvr_tf_create(ctx_, data, static_cast<uint32_t>(width), domain.x, domain.y, &dvrTF_);
+    if (rc != HIPDVR_OK) {
+        dvrTF_ = 0;
+        uploadedTF_.clear();
+        return fail(std::string("transfer function upload failed: ") + hipdvr_last_error(ctx_));
+    }
+    uploadedTF_.assign(data, data + bytes);
+    uploadedDomain_ = domain;
+    return true;
+}
+
+void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step) {
+    std::memset(&p, 0, sizeof(p));
+    p.step = step;
+    p.filter = texFilterMode_.getValue() == GL_NEAREST ? HIPDVR_FILTER_NEAREST : HIPDVR_FILTER_LINEAR;
+    p.classification = HIPDVR_CLASS_TF;
+    p.compositing = HIPDVR_COMPOSITE_DVR;
+    p.shading = shadeMode_.getKey() == "phong" ? HIPDVR_SHADE_PHONG : HIPDVR_SHADE_NONE;
+    // light and camera to the volume's physical space (src/core/utils/glsl.cpp:61-72)
+    const tgt::mat4 worldToPhysical = vb->getWorldToPhysicalMatrix();
+    const tgt::vec3 lp = (worldToPhysical * lightPosition_.get()).xyz();
+    const tgt::vec3 cp = (worldToPhysical * tgt::vec4(camera_.get().getPositionWithOffsets(), 1.f)).xyz();
+    for (int i = 0; i < 3; ++i) {
+        p.light_pos[i] = lp[i];
+        p.light_ambient[i] = lightAmbient_.get()[i];
+        p.light_diffuse[i] = lightDiffuse_.get()[i];
+        p.light_specular[i] = lightSpecular_.get()[i];
+        p.camera_pos[i] = cp[i];
+    }
+    p.shininess = materialShininess_.get();
+    p.loop_cap = 0;
+    const std::string k = hipKernel_.getKey();
+    p.kernel = k == "reference" ? HIPDVR_KERNEL_REFERENCE
+             : k == "bricked"   ? HIPDVR_KERNEL_BRICKED
+                                : HIPDVR_KERNEL_AUTO;
+}
+
+// Output exactly as CPURaycaster::process (cpuraycaster.cpp:157-159,197-207).
+void HIPRaycaster::writeOutput(const std::vector<float>& rgba, int w, int h) {
+    outport1_.activateTarget();
+    outport1_.clearTarget();
+    if (!rgba.empty()) {
+        tgt::Texture outputTex(tgt::ivec3(w, h, 1), GL_RGBA, GL_RGBA, GL_FLOAT, tgt::Texture::LINEAR,
+                               tgt::Texture::REPEAT, (GLubyte*)(const_cast<float*>(rgba.data())), false);
+        outputTex.uploadTexture();
+        glClear(GL_COLOR_BUFFER_BIT | GL_DEPTH_BUFFER_BIT);
+        renderQuad();
+    }
+    outport1_.deactivateTarget();
+}
+
+void HIPRaycaster::process() {
+    const VolumeBase* vb = volumeInport_.getData();
+    const VolumeRAM* vram = vb ? vb->getRepresentation<VolumeRAM>() : 0;
+    const tgt::ivec2 size = entryPort_.getSize();
+    if (!vram) {
+        fail("no input volume");
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+    tFunc1DKeys_.setVolume(vb);
+    tFunc2DPrimitives_.setVolume(vb);
+
+    if (tFuncType_.getValue() != TFT_1DKEYS) {
+        fail("HIPRaycaster supports 1D key transfer functions only");
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+    if (classificationMode_.getKey() == "pre-integrated-gpu") {
+        // same fallback as cpuraycaster.cpp:123-129
+        LERROR("HIPRaycaster computes the pre-integration table on the CPU. Falling back to CPU.");
+        classificationMode_.select("pre-integrated-cpu");
+        invalidate();
+        return;
+    }
+    const std::string comp = compositingMode1_.getKey();
+    if (comp != "dvr") {
+        fail("compositing mode '" + comp + "' is not implemented by HIPRaycaster (dvr only)");
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+    const std::string shade = shadeMode_.getKey();
+    if (shade != "none" && shade != "phong") {
+        fail("shading mode '" + shade + "' is not implemented by HIPRaycaster (none, phong)");
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+    if (shade == "phong" && gradientMode_.getKey() != "central-differences")
+        LWARNING("HIPRaycaster computes gradients by central differences");
+
+    if (!ensureContext() || !ensureVolume(vb, vram)) {
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+
+    // sampling step, volumeraycaster.cpp:312-322 (CPURaycaster: cpuraycaster.cpp:104)
+    const float step = getSamplingStepSize(vb);
+
+    tgt::Texture* tfTexture = tFunc1DKeys_.get()->getTexture();
+    tfTexture->downloadTexture();
+    if (!ensureTransFunc(tfTexture, tFunc1DKeys_.get()->getDomain())) {
+        writeOutput(std::vector<float>(), size.x, size.y);
+        return;
+    }
+
+    hipdvr_params params;
+    fillParams(params, vb, step);
+
+    const std::string cls = classificationMode_.getKey();
+    if (cls == "pre-integrated-cpu" || cls == "pre-integrated-approximation") {
+        const bool approx = cls == "pre-integrated-approximation";
+        const std::string key = cls + ":" + std::to_string(preIntegrationTableSize_.getValue());
+        if (uploadedPreintKey_ != key || uploadedPreintStep_ != step) {
+            // the reference's own table (preintegrationtable.cpp:67-204), uploaded once per TF/step
+            const PreIntegrationTable* table = tFunc1DKeys_.get()->getPreIntegrationTable(
+                step, static_cast<size_t>(preIntegrationTableSize_.getValue()), approx);
+            if (!table || !table->getTable()) {
+                fail("no pre-integration table");
+                writeOutput(std::vector<float>(), size.x, size.y);
+                return;
+            }
+            const int rc = hipdvr_tf_set_preint(dvrTF_, reinterpret_cast<const float*>(table->getTable()),
+                                                static_cast<uint32_t>(table->getDimension()));
+            if (rc != HIPDVR_OK) {
+                fail(std::string("pre-integration table upload failed: ") + hipdvr_last_error(ctx_));
+                writeOutput(std::vector<float>(), size.x, size.y);
+                return;
+            }
+            uploadedPreintKey_ = key;
+            uploadedPreintStep_ = step;
+        }
+        params.classification = HIPDVR_CLASS_PREINT;
+    }
+
+    // entry/exit points, as CPURaycaster downloads them (cpuraycaster.cpp:166-169)
+    tgt::vec4* entryBuffer = reinterpret_cast<tgt::vec4*>(
+        entryPort_.getColorTexture()->downloadTextureToBuffer(GL_RGBA, GL_FLOAT));
+    tgt::vec4* exitBuffer = reinterpret_cast<tgt::vec4*>(
+        exitPort_.getColorTexture()->downloadTextureToBuffer(GL_RGBA, GL_FLOAT));
+
+    std::vector<float> output(4 * static_cast<size_t>(size.x) * size.y, 0.f);
+    const int rc = hipdvr_render(ctx_, dvrVolume_, dvrTF_, reinterpret_cast<const float*>(entryBuffer),
+                                 reinterpret_cast<const float*>(exitBuffer), size.x, size.y, &params,
+                                 output.data(), &lastStats_);
+    delete[] entryBuffer;
+    delete[] exitBuffer;
+    if (rc != HIPDVR_OK) {
+        fail(std::string("hipdvr_render failed: ") + hipdvr_last_error(ctx_));
+        output.clear();
+    }
+    writeOutput(output, size.x, size.y);
+}
+
+}  // namespace voreen

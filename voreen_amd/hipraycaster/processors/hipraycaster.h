@@ -1,0 +1,107 @@
+#ifndef VRN_HIPRAYCASTER_H
+#define VRN_HIPRAYCASTER_H
+
+#include "voreen/core/processors/volumeraycaster.h"
+#include "voreen/core/properties/transfunc/transfunctypeproperty.h"
+#include "voreen/core/properties/transfunc/1d/1dkeys/transfunc1dkeysproperty.h"
+#include "voreen/core/properties/transfunc/2d/2dprimitives/transfunc2dprimitivesproperty.h"
+#include "voreen/core/properties/optionproperty.h"
+#include "voreen/core/properties/intproperty.h"
+#include "voreen/core/properties/floatproperty.h"
+#include "voreen/core/properties/boolproperty.h"
+#include "voreen/core/properties/cameraproperty.h"
+#include "voreen/core/ports/volumeport.h"
+
+#include "hipdvr.h"
+
+#include <string>
+#include <vector>
+
+namespace voreen {
+
+/**
+ * Direct volume rendering on an AMD Instinct MI355X.
+ *
+ * Drop-in for the ray-march path of CPURaycaster
+ * (modules/base/processors/render/cpuraycaster.h:48-93): the same ports
+ * (volumehandle.volumehandle, image.entrypoints, image.exitpoints,
+ * image.output, volumehandle.gradientvolumehandle) and the same properties
+ * (samplingRate, classification, compositing, shading, gradient, light and
+ * material from VolumeRaycaster/VolumeRenderer; tFuncType, tFunc1DKeys,
+ * tFunc2DPrimitives, textureFilterMode_, preIntegrationTableSize from
+ * CPURaycaster; camera as in SingleVolumeRaycaster).  The march runs in the
+ * HIP kernels of libhipdvr.so through its C-ABI (hipdvr.h).  Properties added
+ * here have ids starting with "hip".
+ *
+ * Invalidation: a new volume (INVALID_PROGRAM on the volume inport,
+ * src/core/processors/volumeraycaster.cpp:44) re-uploads the volume; a new TF
+ * re-uploads the table; everything else only re-renders.
+ */
+class HIPRaycaster : public VolumeRaycaster {
+public:
+    HIPRaycaster();
+    virtual ~HIPRaycaster();
+
+    virtual Processor* create() const;
+
+    virtual std::string getClassName() const { return "HIPRaycaster"; }
+    virtual std::string getCategory() const  { return "Raycasting"; }
+    virtual CodeState getCodeState() const   { return CODE_STATE_TESTING; }
+
+    /// Same rule as CPURaycaster::isReady (cpuraycaster.cpp:82-84).
+    virtual bool isReady() const;
+
+    /// Statistics of the last frame (rays, samples, reconstructed samples,
+    /// shaded samples, kernel time).
+    const hipdvr_stats& getLastStats() const { return lastStats_; }
+
+protected:
+    virtual void setDescriptions() {
+        setDescription("Direct volume rendering ray march on an AMD MI355X GPU (HIP). Results match "
+                       "CPURaycaster sample for sample; Phong shading follows the GLSL raycaster.");
+    }
+
+    virtual void process();
+    virtual void initialize();
+    virtual void deinitialize();
+
+private:
+    bool ensureContext();
+    bool ensureVolume(const VolumeBase* vb, const VolumeRAM* vram);
+    bool ensureTransFunc(tgt::Texture* tfTexture, const tgt::vec2& domain);
+    void releaseDevice();
+    void fillParams(hipdvr_params& p, const VolumeBase* vb, float step);
+    void writeOutput(const std::vector<float>& rgba, int w, int h);
+    bool fail(const std::string& msg);
+
+    VolumePort gradientVolumePort_;
+
+    TransFuncTypeProperty tFuncType_;
+    TransFunc1DKeysProperty tFunc1DKeys_;
+    TransFunc2DPrimitivesProperty tFunc2DPrimitives_;
+    IntOptionProperty texFilterMode_;
+    IntOptionProperty preIntegrationTableSize_;
+    CameraProperty camera_;
+
+    // HIP-specific
+    IntProperty hipDevice_;
+    IntOptionProperty hipBrickSize_;
+    StringOptionProperty hipKernel_;
+
+    hipdvr_ctx* ctx_;
+    hipdvr_volume* dvrVolume_;
+    hipdvr_tf* dvrTF_;
+    const VolumeBase* uploadedVolume_;
+    std::vector<uint8_t> uploadedTF_;
+    tgt::vec2 uploadedDomain_;
+    int uploadedBrickLog2_;
+    float uploadedPreintStep_;
+    std::string uploadedPreintKey_;
+    hipdvr_stats lastStats_;
+
+    static const std::string loggerCat_;
+};
+
+}  // namespace voreen
+
+#endif

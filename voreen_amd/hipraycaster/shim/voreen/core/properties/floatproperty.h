@@ -1,0 +1,2 @@
+#pragma once
+#include "voreen/core/shim_base.h"
