@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--config", default="c1", choices=["c0", "c1"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked"])
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
+    ap.add_argument("--shading", default="default", choices=["default", "none", "phong"],
+                    help="override the workload's shading (changes the workload name)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -80,7 +82,9 @@ def main():
         sys.exit(2)
     from voreen_amd import hipdvr as H, workloads as W
 
-    cfg = W.CONFIGS[args.config]
+    cfg = dict(W.CONFIGS[args.config])
+    if args.shading != "default":
+        cfg["shading"] = H.SHADE_PHONG if args.shading == "phong" else H.SHADE_NONE
     dims = cfg["dims"]
     wimg, himg = cfg["image"]
     nviews = args.views or cfg["views"]

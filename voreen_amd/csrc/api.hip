@@ -162,6 +162,7 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.rwm_offset = v.rwm_offset;
     a.dom_lo = tf->domain_lo;
     a.dom_hi = tf->domain_hi;
+    a.dom_unit = (tf->domain_hi - tf->domain_lo) == 1.0f ? 1 : 0;
     a.tf_width = (int32_t)tf->width;
     a.tf = tf->d_rgba;
     a.preint = tf->d_preint;
@@ -204,7 +205,9 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else {
-        const bool wide = vol->tiled_elems > 0xffffffffull;
+        // buffer descriptors address at most 2^32 - 1 bytes
+        const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
+        const bool wide = vol->tiled_elems * vsz > 0xffffff00ull;
         launch_march_fast(a, vol->desc.dtype, p->filter, phong, count, wide, ctx->stream);
     }
     hip_check(hipGetLastError(), "march kernel launch");

@@ -31,6 +31,7 @@ struct MarchArgs {
     float fdm1[3];              // (float)dims - 1
     float rwm_scale, rwm_offset;
     float dom_lo, dom_hi;
+    int32_t dom_unit;           // dom_hi - dom_lo == 1: the division is exact, x = rw - lo
     int32_t tf_width;
     const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
     const float4* preint;       // N*N, [fe*N + fs]
@@ -131,10 +132,17 @@ __device__ __forceinline__ float rw_to_norm(float rw, float lo, float hi) {
     return (rw - lo) / (hi - lo);
 }
 
+// rw_to_norm for a domain of width exactly 1: (rw - lo) / 1 == rw - lo.
+__device__ __forceinline__ float rw_to_norm_unit(float rw, float lo, float hi) {
+    if (rw < lo) return 0.f;
+    if (rw > hi) return 1.f;
+    return rw - lo;
+}
+
 // Intensity -> TF table index (cpuraycaster.cpp:254,266 and :328-332).
 __device__ __forceinline__ int tf_index(float inten, const MarchArgs& a) {
     float rw = (inten * a.rwm_scale) + a.rwm_offset;
-    float x = rw_to_norm(rw, a.dom_lo, a.dom_hi);
+    float x = a.dom_unit ? rw_to_norm_unit(rw, a.dom_lo, a.dom_hi) : rw_to_norm(rw, a.dom_lo, a.dom_hi);
     int w1 = a.tf_width - 1;
     int idx = iround_tgt(x * (float)w1);
     return idx < 0 ? 0 : (idx > w1 ? w1 : idx);

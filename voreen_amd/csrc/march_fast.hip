@@ -3,43 +3,60 @@
 //
 // Same sample positions, reconstruction, classification and compositing as
 // march_ref.hip (and as CPURaycaster::directRendering,
-// modules/base/processors/render/cpuraycaster.cpp:210-326), so the image is
-// bit-identical.  Differences are only in where the voxels come from and which
-// samples are reconstructed:
+// modules/base/processors/render/cpuraycaster.cpp:210-326), so the unshaded
+// image is bit-identical.  Differences are only in where the voxels come
+// from and which samples are reconstructed:
 //   * voxels are read from the 4x4x4-block tiled copy (separable addressing,
-//     one 128-byte line per uint16 block);
+//     one 128-byte line per uint16 block) with 32-bit-offset buffer loads;
 //   * a sample whose footprint lies in a brick the occupancy table marks empty
-//     maps to a zero-alpha TF entry, so it is not reconstructed: the ray only
-//     advances t (t += step, as the reference) and re-tests its brick;
+//     maps to a zero-alpha TF entry, so it is not reconstructed; inside such a
+//     brick the ray fast-forwards with t += step (the reference's float
+//     sequence) up to an analytic bound of the brick exit;
 //   * workgroups are remapped so each XCD marches a contiguous band of tiles
-//     (its L2 holds the bricks that band reads).
+//     (its L2 holds the bricks that band reads);
+//   * Phong's central differences (mod_gradients.frag:202-214: six trilinear
+//     lookups at +-1 voxel) are evaluated from per-axis plane bilinears that
+//     share the sample's weights: the same clamped stencil positions and
+//     interpolation, about a fifth of the arithmetic, equal to the six-lookup
+//     form up to float rounding.
 #include "kernels.h"
 #include "march_common.h"
 
 namespace hipdvr {
 
+// Voxel source with separable block addressing.  WIDE volumes (more than
+// 2^31 elements) use 64-bit global loads; the rest use buffer loads with a
+// 32-bit byte offset from one descriptor.
 template <typename T, bool WIDE>
 struct TiledFetch {
     typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type off_t;
     const T* p;
+    __amdgpu_buffer_rsrc_t rsrc;
     uint32_t sy, sz;   // elements per block row (64*NBX) and per block slab (64*NBX*NBY)
 
     __device__ __forceinline__ off_t ox(uint32_t x) const { return (off_t)(((x >> 2) << 6) | (x & 3)); }
     __device__ __forceinline__ off_t oy(uint32_t y) const { return (off_t)(y >> 2) * sy + ((y & 3) << 2); }
     __device__ __forceinline__ off_t oz(uint32_t z) const { return (off_t)(z >> 2) * sz + ((z & 3) << 4); }
 
+    __device__ __forceinline__ uint32_t load(off_t e) const {
+        if (WIDE) return p[e];
+        if (sizeof(T) == 2)
+            return __builtin_amdgcn_raw_buffer_load_b16(rsrc, (uint32_t)e * 2u, 0, 0);
+        return __builtin_amdgcn_raw_buffer_load_b8(rsrc, (uint32_t)e, 0, 0);
+    }
+
     __device__ __forceinline__ void fetch8(const Corners& c, float v[8]) const {
         const off_t x0 = ox(c.lo[0]), x1 = ox(c.lo[0] + c.d[0]);
         const off_t y0 = oy(c.lo[1]), y1 = oy(c.lo[1] + c.d[1]);
         const off_t z0 = oz(c.lo[2]), z1 = oz(c.lo[2] + c.d[2]);
-        const uint32_t r0 = p[x0 + y0 + z0];
-        const uint32_t r1 = p[x1 + y0 + z0];
-        const uint32_t r2 = p[x1 + y1 + z0];
-        const uint32_t r3 = p[x0 + y1 + z0];
-        const uint32_t r4 = p[x0 + y0 + z1];
-        const uint32_t r5 = p[x1 + y0 + z1];
-        const uint32_t r6 = p[x1 + y1 + z1];
-        const uint32_t r7 = p[x0 + y1 + z1];
+        const uint32_t r0 = load(x0 + y0 + z0);
+        const uint32_t r1 = load(x1 + y0 + z0);
+        const uint32_t r2 = load(x1 + y1 + z0);
+        const uint32_t r3 = load(x0 + y1 + z0);
+        const uint32_t r4 = load(x0 + y0 + z1);
+        const uint32_t r5 = load(x1 + y0 + z1);
+        const uint32_t r6 = load(x1 + y1 + z1);
+        const uint32_t r7 = load(x0 + y1 + z1);
         v[0] = VoxNorm<T>::apply(r0);
         v[1] = VoxNorm<T>::apply(r1);
         v[2] = VoxNorm<T>::apply(r2);
@@ -53,28 +70,99 @@ struct TiledFetch {
         const uint32_t x = (uint32_t)iround_tgt(vp[0]);
         const uint32_t y = (uint32_t)iround_tgt(vp[1]);
         const uint32_t z = (uint32_t)iround_tgt(vp[2]);
-        return VoxNorm<T>::apply(p[ox(x) + oy(y) + oz(z)]);
+        return VoxNorm<T>::apply(load(ox(x) + oy(y) + oz(z)));
     }
 };
 
-template <typename T, bool WIDE, int FILTER>
-__device__ __forceinline__ float reconstruct_vp(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
-                                                const float vp[3]) {
-    if (FILTER == 0) return f.nearest(vp);
-    Corners c;
-    corners(vp, a.dims, c);
-    float v[8];
-    f.fetch8(c, v);
-    return trilerp(v, c.p);
+// Stencil coordinate along one axis: clamp(u +- 1) split into the lower plane,
+// the upper plane (clamped) and the fraction.
+struct Plane2 {
+    uint32_t lo, hi;
+    float p;
+};
+
+__device__ __forceinline__ Plane2 stencil_plane(float u, float fdm1, uint32_t m) {
+    const float c = sminf(smaxf(u, 0.0f), fdm1);
+    Plane2 r;
+    r.p = c - floorf(c);
+    r.lo = (uint32_t)c;
+    r.lo = r.lo < m ? r.lo : m;
+    r.hi = r.lo + 1 < m ? r.lo + 1 : m;
+    return r;
 }
 
-template <typename T, bool WIDE, int FILTER>
-__device__ __forceinline__ float reconstruct_s(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
-                                               const float s[3]) {
-    float vp[3];
+// Central-difference gradient of the trilinear interpolant at the voxel
+// position (u unclamped, c = corners of the clamped position): for axis i the
+// two stencil samples clamp(u_i -+ 1) are interpolated along i between plane
+// bilinears over the other two axes, which use the sample's own (clamped)
+// coordinates and weights.  Raw voxel values; scaled once at the end.
+template <typename T, bool WIDE>
+__device__ __forceinline__ void gradient_planes(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
+                                                const float u[3], const Corners& c, float g[3]) {
+    typedef typename TiledFetch<T, WIDE>::off_t off_t;
+    off_t o[3][2];
+    o[0][0] = f.ox(c.lo[0]);
+    o[0][1] = f.ox(c.lo[0] + c.d[0]);
+    o[1][0] = f.oy(c.lo[1]);
+    o[1][1] = f.oy(c.lo[1] + c.d[1]);
+    o[2][0] = f.oz(c.lo[2]);
+    o[2][1] = f.oz(c.lo[2] + c.d[2]);
+    float w[3][2];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
-    return reconstruct_vp<T, WIDE, FILTER>(f, a, vp);
+    for (int i = 0; i < 3; ++i) {
+        w[i][0] = 1.f - c.p[i];
+        w[i][1] = c.p[i];
+    }
+    const float norm = sizeof(T) == 2 ? (1.0f / 65535.0f) : (1.0f / 255.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        const uint32_t m = a.dims[i] - 1;
+        const Plane2 lo = stencil_plane(u[i] - 1.0f, a.fdm1[i], m);
+        const Plane2 hi = stencil_plane(u[i] + 1.0f, a.fdm1[i], m);
+        const uint32_t planes[4] = {lo.lo, lo.hi, hi.lo, hi.hi};
+        float B[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const off_t oi = i == 0 ? f.ox(planes[q]) : (i == 1 ? f.oy(planes[q]) : f.oz(planes[q]));
+            float s = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk)
+                    s = s + (float)f.load(oi + o[j][jj] + o[k][kk]) * (w[j][jj] * w[k][kk]);
+            B[q] = s;
+        }
+        const float vm = (1.f - lo.p) * B[0] + lo.p * B[1];
+        const float vpl = (1.f - hi.p) * B[2] + hi.p * B[3];
+        g[i] = (((vm - vpl) * norm) * a.grad_scale[i]) * a.rwm_scale;
+    }
+}
+
+// Ray parameter below which every sample lies in the (empty) brick holding
+// the voxel position vp: the smallest exit parameter over the three axes of
+// the brick's voxel slab shrunk by 0.05 voxel (far above the float error of
+// the sample position), in unclamped voxel space where the first and last
+// brick of an axis extend to infinity (positions are clamped to the volume).
+__device__ __forceinline__ float empty_brick_limit(const MarchArgs& a, const float first[3],
+                                                   const float dir[3], const float vp[3],
+                                                   uint32_t bl) {
+    float tl = INFINITY;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t b = (uint32_t)vp[i] >> bl;
+        const float B = (float)(1u << bl);
+        if (dir[i] > 0.0f && b + 1 < a.nbricks[i]) {
+            const float vb = (float)(b + 1) * B - 0.05f;           // voxel coordinate bound
+            const float sb = (vb + 0.5f) / a.fdims[i];               // texture coordinate
+            tl = fminf(tl, (sb - first[i]) / dir[i]);
+        } else if (dir[i] < 0.0f && b > 0) {
+            const float vb = (float)b * B + 0.05f;
+            const float sb = (vb + 0.5f) / a.fdims[i];
+            tl = fminf(tl, (sb - first[i]) / dir[i]);
+        }
+    }
+    return tl;
 }
 
 template <typename T, bool WIDE, int FILTER, bool PHONG, bool COUNT>
@@ -102,10 +190,15 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
     f.p = (const T*)a.bricks;
     f.sy = (uint32_t)((a.dims[0] + 3) >> 2) << 6;
     f.sz = f.sy * ((a.dims[1] + 3) >> 2);
+    if (!WIDE) {
+        const uint32_t bytes = (uint32_t)(f.sz * ((a.dims[2] + 3) >> 2) * sizeof(T));
+        f.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.bricks, (short)0, (int)bytes, 0x00020000);
+    }
     const uint32_t bl = a.brick_log2;
     const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
 
-    unsigned long long n_slots = 0, n_shade_slots = 0, n_samples = 0, n_fetched = 0, n_shaded = 0, n_rays = 0;
+    unsigned long long n_slots = 0, n_shade_slots = 0, n_samples = 0, n_fetched = 0, n_shaded = 0,
+                       n_rays = 0;
     if (inside) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix];
@@ -121,24 +214,51 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
             float t = 0.0f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
             uint32_t cur_brick = 0xffffffffu;
             bool cur_occ = false;
+            float t_lim = 0.0f;
             bool finished = false;
             for (int loop = 0; !finished && loop < cap; ++loop) {
                 ++n_samples;
                 bool shade_now = false;
                 const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1],
                                     first[2] + t * dir[2]};
-                float vp[3];
+                float u[3], vp[3];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+                for (int i = 0; i < 3; ++i) {
+                    u[i] = (a.fdims[i] * s[i]) + (-0.5f);
+                    vp[i] = sminf(smaxf(u[i], 0.0f), a.fdm1[i]);
+                }
                 const uint32_t brick = (((uint32_t)vp[2] >> bl) * nbxy) +
                                        (((uint32_t)vp[1] >> bl) * nbx) + ((uint32_t)vp[0] >> bl);
                 if (brick != cur_brick) {
                     cur_brick = brick;
                     cur_occ = a.brick_occ[brick] != 0;
+                    if (!cur_occ) t_lim = empty_brick_limit(a, first, dir, vp, bl);
                 }
-                if (cur_occ) {
+                if (!cur_occ) {
+                    // Fast-forward: every sample at t' < t_lim lies in this empty
+                    // brick, so its only effect in the reference loop is the step
+                    // count and t += step (same float sequence, same cap, same
+                    // termination test).
+                    float tn = t + a.step;
+                    while (tn < t_lim && !(tn > tend) && loop + 1 < cap) {
+                        t = tn;
+                        ++loop;
+                        ++n_samples;
+                        if (COUNT) ++n_slots;
+                        tn = t + a.step;
+                    }
+                } else {
                     ++n_fetched;
-                    const float inten = reconstruct_vp<T, WIDE, FILTER>(f, a, vp);
+                    Corners c;
+                    float inten;
+                    if (FILTER == 0) {
+                        inten = f.nearest(vp);
+                    } else {
+                        corners(vp, a.dims, c);
+                        float v[8];
+                        f.fetch8(c, v);
+                        inten = trilerp(v, c.p);
+                    }
                     const uint32_t c8 = s_tf[tf_index(inten, a)];
                     const float ca = s_alpha[c8 >> 24];
                     if (ca > 0.0f) {
@@ -151,29 +271,17 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
                             if (t == 0.0f) {
                                 clip_border_gradient(a, px, py, s, dir, gr);
                             } else {
-                                float vals[6];
-#pragma unroll
-                                for (int k = 0; k < 6; ++k) {
-                                    const int axis = k % 3;
-                                    const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
-                                    float q[3];
-                                    q[axis] = s[axis] + off;
-                                    q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
-                                    q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
-                                    vals[k] = reconstruct_s<T, WIDE, FILTER>(f, a, q);
-                                }
-#pragma unroll
-                                for (int i = 0; i < 3; ++i)
-                                    gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+                                if (FILTER == 0) corners(vp, a.dims, c);
+                                gradient_planes<T, WIDE>(f, a, u, c, gr);
                             }
-                            float pos[3], c[3] = {cr, cg, cb};
+                            float pos[3], col[3] = {cr, cg, cb};
 #pragma unroll
                             for (int i = 0; i < 3; ++i)
                                 pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                            phong(a, gr, pos, c);
-                            cr = c[0];
-                            cg = c[1];
-                            cb = c[2];
+                            phong(a, gr, pos, col);
+                            cr = col[0];
+                            cg = col[1];
+                            cb = col[2];
                             ++n_shaded;
                         }
                         const float wgt = (1.0f - A) * ca;
