@@ -1,0 +1,71 @@
+/*
+ * hipdvr_learned.h -- learned stepping for the DVR ray march.
+ *
+ * No counterpart in the reference: Voreen's raycasters march at a fixed step
+ * (cpuraycaster.cpp:104,315) and skip nothing.  This is the learned addition
+ * beside the drop-in path.  A small network looks at per-brick statistics of
+ * the volume and at the transfer function, and proposes for each 16^3 brick
+ * and ray-direction octant:
+ *   - whether the brick can be skipped;
+ *   - a step multiplier (1, 2, 4 or 8 times the plain step).
+ *
+ * The network has 16 inputs, two hidden layers of 64 (GELU) and 2 outputs.
+ * Its weights are BF16 with FP32 accumulation (MFMA). It is trained on the GPU
+ * by the library's own forward/backward/Adam kernels, on targets measured
+ * from plain-marcher traces.
+ *
+ * The plain marcher stays the judge. For each learned frame, 1/64 of the rays
+ * (every 8th pixel in x and y) are re-marched plainly and compared. If the
+ * PSNR of that sparse set falls below the floor (45 dB by default), the frame
+ * is rendered plainly instead.
+ *
+ * Features of one (brick, octant) row, normalized intensities in [0, 1]:
+ *   0 min  1 max  2 mean  3 std of the brick's voxels (16^3 + 1-voxel apron)
+ *   4 mean corrected TF alpha over the index range [idx(min), idx(max)]
+ *   5 corrected TF alpha at idx(mean)
+ *   6 fraction of that range with non-zero alpha
+ *   7 opacity of 32 samples at the mean alpha: 1 - (1 - f4)^32
+ *   8..10 ray direction octant signs (+1 / -1)
+ *   11..15 zero
+ * Targets (3 floats): skip label in {0,1}, log2 multiplier label in [0,3],
+ * multiplier-loss weight.
+ */
+#ifndef HIPDVR_LEARNED_H
+#define HIPDVR_LEARNED_H
+
+#include <stdint.h>
+
+#include "hipdvr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPDVR_MLP_INPUTS 16
+#define HIPDVR_MLP_OUTPUTS 2
+#define HIPDVR_MLP_TARGETS 3
+/* W1[16][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2] */
+#define HIPDVR_MLP_PARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2)
+
+typedef struct hipdvr_mlp hipdvr_mlp;
+
+/* network with seeded Xavier-uniform weights and zero biases */
+int hipdvr_mlp_create(hipdvr_ctx* ctx, uint64_t seed, hipdvr_mlp** out);
+int hipdvr_mlp_destroy(hipdvr_mlp* mlp);
+/* FP32 master parameters (HIPDVR_MLP_PARAMS floats), host arrays */
+int hipdvr_mlp_get_params(hipdvr_mlp* mlp, float* out);
+int hipdvr_mlp_set_params(hipdvr_mlp* mlp, const float* in);
+/* device arrays: x [rows][16] -> y [rows][2] */
+int hipdvr_mlp_forward(hipdvr_mlp* mlp, const float* d_x, uint32_t rows, float* d_y);
+/* gradient of the mean loss over the rows (host output: HIPDVR_MLP_PARAMS
+ * floats) and the loss itself; device inputs x [rows][16], t [rows][3] */
+int hipdvr_mlp_gradient(hipdvr_mlp* mlp, const float* d_x, const float* d_t, uint32_t rows,
+                        float* grad_out, float* loss_out);
+/* one Adam step on a minibatch (device arrays); loss_out may be NULL */
+int hipdvr_mlp_train_step(hipdvr_mlp* mlp, const float* d_x, const float* d_t, uint32_t rows,
+                          float lr, float* loss_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -121,6 +121,17 @@ int64_t dvr_fp64_render(const dvr_volume* vol, const dvr_tf* tf, const dvr_param
 /* single trilinear reconstruction, volumeram.cpp:78-97 (for unit tests) */
 float dvr_sample_linear(const dvr_volume* vol, float vx, float vy, float vz);
 
+/* ---- stepping network oracle (FP64), TEST INFRASTRUCTURE ONLY ----
+ * 16 -> 64 (GELU) -> 64 (GELU) -> 2; params as the library's flat FP32
+ * layout W1[16][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2].  With
+ * quantize != 0 the weight matrices are first rounded to BF16 (nearest even),
+ * as the GPU kernels use them; activations stay FP64.  Loss per row:
+ * BCE-with-logits(y0, t0) + t2*(y1 - t1)^2, mean over rows. */
+#define DVR_MLP_PARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2)
+void dvr_mlp_forward(const float* params, int32_t quantize, const float* x, int32_t rows, double* y);
+double dvr_mlp_loss_grad(const double* params, int32_t quantize, const float* x, const float* t,
+                         int32_t rows, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

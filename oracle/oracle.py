@@ -85,6 +85,11 @@ def _load():
         fn.restype = C.c_int64
     lib.dvr_sample_linear.argtypes = [P(Volume), C.c_float, C.c_float, C.c_float]
     lib.dvr_sample_linear.restype = C.c_float
+    lib.dvr_mlp_forward.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.dvr_mlp_forward.restype = None
+    lib.dvr_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                      C.c_void_p]
+    lib.dvr_mlp_loss_grad.restype = C.c_double
     return lib
 
 
@@ -226,3 +231,25 @@ def psnr(a, b):
     """PSNR = 10*log10(1/MSE) over all RGBA channels of images in [0, 1]."""
     mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
     return float("inf") if mse == 0.0 else 10.0 * np.log10(1.0 / mse)
+
+
+MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2
+
+
+def mlp_forward(params, x, quantize=True):
+    """FP64 forward of the stepping network; params FP32 flat layout."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((x.shape[0], 2), np.float64)
+    lib().dvr_mlp_forward(_ptr(params), int(quantize), _ptr(x), x.shape[0], _ptr(y))
+    return y
+
+
+def mlp_loss_grad(params, x, t, quantize=True):
+    """FP64 mean loss and its gradient w.r.t. the (FP64) parameters."""
+    params = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = lib().dvr_mlp_loss_grad(_ptr(params), int(quantize), _ptr(x), _ptr(t), x.shape[0], _ptr(g))
+    return loss, g

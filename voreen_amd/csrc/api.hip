@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,23 +18,12 @@
 #include <vector>
 
 #include "../../include/hipdvr.h"
+#include "ctx.h"
 #include "errors.h"
 #include "kernels.h"
 #include "volume.h"
 
 using namespace hipdvr;
-
-struct hipdvr_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    unsigned long long* d_counters = nullptr;   // kNumCounters x u64
-    std::string last_error;
-    // per-launch timing (hipdvr_timing_begin/read)
-    std::vector<hipEvent_t> tev;                 // 2 per slot
-    uint32_t tcap = 0, tcount = 0;
-};
 
 struct hipdvr_tf {
     hipdvr_ctx* ctx = nullptr;
@@ -51,23 +41,7 @@ namespace {
 
 template <typename F>
 int guarded(hipdvr_ctx* ctx, F&& f) {
-    try {
-        f();
-        if (ctx) ctx->last_error.clear();
-        return HIPDVR_OK;
-    } catch (const Error& e) {
-        if (ctx) ctx->last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (ctx) ctx->last_error = "host allocation failed";
-        return HIPDVR_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        if (ctx) ctx->last_error = e.what();
-        return HIPDVR_ERR_HIP;
-    } catch (...) {
-        if (ctx) ctx->last_error = "unknown error";
-        return HIPDVR_ERR_HIP;
-    }
+    return ctx_guarded(ctx, std::forward<F>(f));
 }
 
 void set_device(hipdvr_ctx* ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
@@ -200,8 +174,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel supports transfer-function classification only");
     if (kernel == HIPDVR_KERNEL_BRICKED) ensure_occupancy(ctx, const_cast<hipdvr_volume*>(vol), tf);
     if (count) hip_check(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
-    const bool timed = ctx->tcount < ctx->tcap;
-    if (timed) hip_check(hipEventRecord(ctx->tev[2 * ctx->tcount], ctx->stream), "hipEventRecord");
+    const bool timed = ctx_timing_start(ctx);
     if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else {
@@ -211,10 +184,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         launch_march_fast(a, vol->desc.dtype, p->filter, phong, count, wide, ctx->stream);
     }
     hip_check(hipGetLastError(), "march kernel launch");
-    if (timed) {
-        hip_check(hipEventRecord(ctx->tev[2 * ctx->tcount + 1], ctx->stream), "hipEventRecord");
-        ++ctx->tcount;
-    }
+    if (timed) ctx_timing_stop(ctx);
     if (count) {
         hip_check(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
         unsigned long long c[kNumCounters];

@@ -200,6 +200,38 @@ __device__ __forceinline__ void phong(const MarchArgs& a, const float g[3], cons
     }
 }
 
+// The same Phong terms with hardware reciprocal, reciprocal square root and
+// exp2/log2 (about 1 ulp each): for kernels whose shading is checked against
+// the FP64 oracle by tolerance.  A zero gradient gives N = 0; N.H = 0 gives a
+// zero specular term.
+__device__ __forceinline__ void phong_fast(const MarchArgs& a, const float g[3], const float pos[3],
+                                           float c[3]) {
+    const float gl = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    const float rn = gl > 0.f ? __builtin_amdgcn_rsqf(gl) : 0.f;
+    float L[3], V[3], H[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { L[i] = a.light_pos[i] - pos[i]; V[i] = a.cam_pos[i] - pos[i]; }
+    const float rv = __builtin_amdgcn_rsqf(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    const float rl = __builtin_amdgcn_rsqf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        V[i] *= rv;
+        L[i] *= rl;
+        H[i] = V[i] + L[i];
+    }
+    const float rh = __builtin_amdgcn_rsqf(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+    const float ndl = fmaxf((g[0] * L[0] + g[1] * L[1] + g[2] * L[2]) * rn, 0.f);
+    const float ndh = fmaxf((g[0] * H[0] + g[1] * H[1] + g[2] * H[2]) * (rn * rh), 0.f);
+    const float sp = ndh > 0.f ? __builtin_amdgcn_exp2f(a.shininess * __builtin_amdgcn_logf(ndh)) : 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float amb = c[i] * a.light_amb[i];
+        const float dif = (c[i] * a.light_dif[i]) * ndl;
+        const float spe = a.light_spe[i] * sp;
+        c[i] = (amb + dif) + spe;
+    }
+}
+
 // fixClipBorderGradient, mod_gradients.frag:39-46 (neighbours clamp to edge).
 __device__ __forceinline__ void clip_border_gradient(const MarchArgs& a, int x, int y,
                                                      const float s[3], const float dir[3],

@@ -248,51 +248,88 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
                         tn = t + a.step;
                     }
                 } else {
-                    ++n_fetched;
-                    Corners c;
-                    float inten;
-                    if (FILTER == 0) {
-                        inten = f.nearest(vp);
-                    } else {
-                        corners(vp, a.dims, c);
-                        float v[8];
-                        f.fetch8(c, v);
-                        inten = trilerp(v, c.p);
-                    }
-                    const uint32_t c8 = s_tf[tf_index(inten, a)];
-                    const float ca = s_alpha[c8 >> 24];
-                    if (ca > 0.0f) {
-                        shade_now = PHONG;
-                        float cr = byte_norm(c8 & 0xffu);
-                        float cg = byte_norm((c8 >> 8) & 0xffu);
-                        float cb = byte_norm((c8 >> 16) & 0xffu);
-                        if (PHONG) {
-                            float gr[3];
-                            if (t == 0.0f) {
-                                clip_border_gradient(a, px, py, s, dir, gr);
-                            } else {
-                                if (FILTER == 0) corners(vp, a.dims, c);
-                                gradient_planes<T, WIDE>(f, a, u, c, gr);
-                            }
-                            float pos[3], col[3] = {cr, cg, cb};
+                    // Two samples per pass: the gathers of the next sample (same
+                    // brick, within the ray and the cap) are issued before this
+                    // sample is composited, so their latency overlaps.  If this
+                    // sample terminates the ray the next one is dropped unseen.
+                    const float t1 = t + a.step;
+                    float s1[3], u1[3], vp1[3];
+                    bool pair = !(t1 > tend) && (loop + 1 < cap);
+                    if (pair) {
 #pragma unroll
-                            for (int i = 0; i < 3; ++i)
-                                pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                            phong(a, gr, pos, col);
-                            cr = col[0];
-                            cg = col[1];
-                            cb = col[2];
-                            ++n_shaded;
+                        for (int i = 0; i < 3; ++i) {
+                            s1[i] = first[i] + t1 * dir[i];
+                            u1[i] = (a.fdims[i] * s1[i]) + (-0.5f);
+                            vp1[i] = sminf(smaxf(u1[i], 0.0f), a.fdm1[i]);
                         }
-                        const float wgt = (1.0f - A) * ca;
-                        r = r + wgt * cr;
-                        g = g + wgt * cg;
-                        bb = bb + wgt * cb;
-                        A = A + (1.0f - A) * ca;
+                        const uint32_t brick1 = (((uint32_t)vp1[2] >> bl) * nbxy) +
+                                                (((uint32_t)vp1[1] >> bl) * nbx) + ((uint32_t)vp1[0] >> bl);
+                        pair = brick1 == cur_brick;
                     }
-                    if (A >= kErtOpacity) {
-                        A = 1.0f;
-                        finished = true;
+                    Corners c0, c1;
+                    float v0[8], v1[8];
+                    if (FILTER == 0) {
+                        v0[0] = f.nearest(vp);
+                        if (pair) v1[0] = f.nearest(vp1);
+                    } else {
+                        corners(vp, a.dims, c0);
+                        f.fetch8(c0, v0);
+                        if (pair) {
+                            corners(vp1, a.dims, c1);
+                            f.fetch8(c1, v1);
+                        }
+                    }
+                    auto composite = [&](const float* vv, Corners& cc, const float* ss, const float* uu,
+                                         const float* pp, float tt) {
+                        ++n_fetched;
+                        const float inten = FILTER == 0 ? vv[0] : trilerp(vv, cc.p);
+                        const uint32_t c8 = s_tf[tf_index(inten, a)];
+                        const float ca = s_alpha[c8 >> 24];
+                        if (ca > 0.0f) {
+                            shade_now = PHONG;
+                            float cr = byte_norm(c8 & 0xffu);
+                            float cg = byte_norm((c8 >> 8) & 0xffu);
+                            float cb = byte_norm((c8 >> 16) & 0xffu);
+                            if (PHONG) {
+                                float gr[3];
+                                if (tt == 0.0f) {
+                                    clip_border_gradient(a, px, py, ss, dir, gr);
+                                } else {
+                                    if (FILTER == 0) corners(pp, a.dims, cc);
+                                    gradient_planes<T, WIDE>(f, a, uu, cc, gr);
+                                }
+                                float pos[3], col[3] = {cr, cg, cb};
+#pragma unroll
+                                for (int i = 0; i < 3; ++i)
+                                    pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
+                                phong_fast(a, gr, pos, col);
+                                cr = col[0];
+                                cg = col[1];
+                                cb = col[2];
+                                ++n_shaded;
+                            }
+                            const float wgt = (1.0f - A) * ca;
+                            r = r + wgt * cr;
+                            g = g + wgt * cg;
+                            bb = bb + wgt * cb;
+                            A = A + (1.0f - A) * ca;
+                        }
+                        if (A >= kErtOpacity) {
+                            A = 1.0f;
+                            finished = true;
+                        }
+                    };
+                    composite(v0, c0, s, u, vp, t);
+                    if (pair && !finished) {
+                        if (COUNT) {
+                            ++n_slots;
+                            n_shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
+                            shade_now = false;
+                        }
+                        ++loop;
+                        ++n_samples;
+                        t = t1;
+                        composite(v1, c1, s1, u1, vp1, t1);
                     }
                 }
                 if (COUNT) {

@@ -32,7 +32,12 @@ EXPORTS = (
     "hipdvr_buffer_alloc", "hipdvr_buffer_free", "hipdvr_memcpy_h2d", "hipdvr_memcpy_d2h",
     "hipdvr_synth_blobs", "hipdvr_synth_shells",
     "hipdvr_timing_begin", "hipdvr_timing_read", "hipdvr_copy_device",
+    "hipdvr_mlp_create", "hipdvr_mlp_destroy", "hipdvr_mlp_get_params", "hipdvr_mlp_set_params",
+    "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
 )
+
+MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
+MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2
 
 
 class HipDvrError(RuntimeError):
@@ -116,6 +121,13 @@ def load():
         "hipdvr_timing_begin": ([vp, u32], C.c_int),
         "hipdvr_timing_read": ([vp, C.POINTER(C.c_float), u32, C.POINTER(u32)], C.c_int),
         "hipdvr_copy_device": ([vp, vp, vp, C.c_size_t], C.c_int),
+        "hipdvr_mlp_create": ([vp, C.c_uint64, P(vp)], C.c_int),
+        "hipdvr_mlp_destroy": ([vp], C.c_int),
+        "hipdvr_mlp_get_params": ([vp, vp], C.c_int),
+        "hipdvr_mlp_set_params": ([vp, vp], C.c_int),
+        "hipdvr_mlp_forward": ([vp, vp, u32, vp], C.c_int),
+        "hipdvr_mlp_gradient": ([vp, vp, vp, u32, vp, P(C.c_float)], C.c_int),
+        "hipdvr_mlp_train_step": ([vp, vp, vp, u32, C.c_float, P(C.c_float)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -361,3 +373,62 @@ class TransFunc:
         if self.h:
             self.ctx.lib.hipdvr_tf_destroy(self.h)
             self.h = C.c_void_p()
+
+
+class Mlp:
+    """The stepping network (include/hipdvr_learned.h)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.hipdvr_mlp_create(ctx.h, seed, C.byref(self.h)), "hipdvr_mlp_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipdvr_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def params(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        self.ctx.check(self.ctx.lib.hipdvr_mlp_get_params(self.h, _ptr(out)), "hipdvr_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        self.ctx.check(self.ctx.lib.hipdvr_mlp_set_params(self.h, _ptr(p)), "hipdvr_mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        rows = x.shape[0]
+        dx, dy = self.ctx.alloc(max(x.nbytes, 4)), self.ctx.alloc(max(rows * 8, 4))
+        try:
+            dx.upload(x)
+            self.ctx.check(self.ctx.lib.hipdvr_mlp_forward(self.h, dx.ptr, rows, dy.ptr), "hipdvr_mlp_forward")
+            return dy.download((rows, MLP_OUTPUTS), np.float32)
+        finally:
+            dx.free()
+            dy.free()
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        dx, dt = self.ctx.alloc(x.nbytes), self.ctx.alloc(t.nbytes)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        try:
+            dx.upload(x)
+            dt.upload(t)
+            self.ctx.check(self.ctx.lib.hipdvr_mlp_gradient(self.h, dx.ptr, dt.ptr, x.shape[0], _ptr(g),
+                                                            C.byref(loss)), "hipdvr_mlp_gradient")
+            return g, loss.value
+        finally:
+            dx.free()
+            dt.free()
+
+    def train_step_device(self, dx, dt, rows, lr, want_loss=False):
+        loss = C.c_float()
+        self.ctx.check(self.ctx.lib.hipdvr_mlp_train_step(self.h, dx.ptr, dt.ptr, rows, lr,
+                                                          C.byref(loss) if want_loss else None),
+                       "hipdvr_mlp_train_step")
+        return loss.value if want_loss else None
