@@ -65,6 +65,49 @@ int hipdvr_mlp_gradient(hipdvr_mlp* mlp, const float* d_x, const float* d_t, uin
 int hipdvr_mlp_train_step(hipdvr_mlp* mlp, const float* d_x, const float* d_t, uint32_t rows,
                           float lr, float* loss_out);
 
+/* ---- learned rendering ---- */
+typedef struct hipdvr_learned hipdvr_learned;
+
+typedef struct hipdvr_learned_opts {
+    float psnr_floor;        /* dB; below it the frame is rendered plainly (default 45) */
+    int32_t remarch_stride;  /* sparse judge: every n-th pixel in x and y (default 8 = 1/64) */
+    float skip_logit;        /* skip a brick when the skip logit exceeds this (default 0) */
+    int32_t max_log2_step;   /* largest step multiplier, as log2 (0..3, default 3) */
+} hipdvr_learned_opts;
+
+typedef struct hipdvr_learned_stats {
+    uint64_t samples;        /* samples the learned marcher reconstructed */
+    uint64_t shaded;
+    float sparse_psnr;       /* PSNR of the learned image at the sparse pixels vs plain */
+    int32_t fell_back;       /* 1 if the frame was rendered plainly */
+} hipdvr_learned_stats;
+
+/* Learned state of one volume: 16^3 brick statistics, features, the
+ * (brick, octant) stepping table, trace accumulators. */
+int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_learned** out);
+int hipdvr_learned_destroy(hipdvr_learned* l);
+/* Features for this TF and step; with mlp != NULL the table comes from the
+ * network, with mlp == NULL it is the no-network baseline (multiplier 1, skip
+ * only bricks whose value range maps to zero alpha). */
+int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* prm,
+                           hipdvr_mlp* mlp, const hipdvr_learned_opts* opts);
+/* Training data: clear the trace accumulators, trace one view (device
+ * entry/exit) into them, and append the visited rows (features of the last
+ * prepare + targets) to device arrays x [capacity][16], t [capacity][3]
+ * starting at *count (updated). */
+int hipdvr_learned_trace_reset(hipdvr_learned* l);
+int hipdvr_learned_trace(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* prm,
+                         const float* d_entry, const float* d_exit, uint32_t w, uint32_t h);
+int hipdvr_learned_collect(hipdvr_learned* l, float tau_skip, float tau_step, float* d_x, float* d_t,
+                           uint32_t capacity, uint32_t* count);
+/* Render with learned stepping; the sparse plain re-march judges it and the
+ * frame falls back to the plain marcher below the PSNR floor.  Asynchronous
+ * unless stats != NULL. */
+int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
+                                 const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
+                                 const hipdvr_learned_opts* opts, float* d_out,
+                                 hipdvr_learned_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,119 @@
+"""Learned stepping end to end on the GPU: the no-network baseline matches
+the plain marcher, a network trained on plain traces (the library's own
+kernels) keeps >= 45 dB while reconstructing fewer samples than the baseline,
+and bad weights trip the sparse judge so the frame falls back to the plain
+image exactly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def psnr(a, b):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * np.log10(1 / mse)
+
+
+@pytest.fixture(scope="module")
+def scene(ctx):
+    from voreen_amd import hipdvr as H, workloads as W
+    dims = (128, 128, 128)
+    vol = W.make_volume("c1", dims=dims)
+    llf, urb = W.world_box(dims)
+    sp = tuple(float((urb[i] - llf[i]) / dims[i]) for i in range(3))
+    off = tuple(float(llf[i] + 0.5 * sp[i]) for i in range(3))
+    v = ctx.volume(vol, spacing=sp, offset=off)
+    keys, _ = W.make_tf("c1")
+    t = ctx.transfunc(H.tf_from_keys(keys, 4096))
+    w = h = 256
+    cam = W.orbit_camera(5, 36)
+    de, dx, do, dl = (ctx.alloc(w * h * 16) for _ in range(4))
+    ctx.entry_exit_device(cam, llf, urb, w, h, de, dx)
+    step = W.step_for(dims, 2.0)
+    yield dict(v=v, t=t, de=de, dx=dx, do=do, dl=dl, w=w, h=h, step=step, cam=cam, dims=dims)
+    for b in (de, dx, do, dl):
+        b.free()
+    v.close()
+    t.close()
+
+
+def _plain(ctx, s, shading):
+    from voreen_amd import hipdvr as H
+    p = H.default_params(s["step"], shading=shading, camera_pos=tuple(s["cam"].eye))
+    st = ctx.render_device(s["v"], s["t"], s["de"], s["dx"], s["w"], s["h"], p, s["do"], stats=True)
+    return s["do"].download((s["h"], s["w"], 4), np.float32), st, p
+
+
+def test_learned_baseline_matches_plain(ctx, scene):
+    from voreen_amd import hipdvr as H
+    print("[case] learned baseline (no network)", flush=True)
+    plain, st, p = _plain(ctx, scene, 0)
+    l = H.Learned(ctx, scene["v"])
+    try:
+        l.prepare(scene["t"], p, None)
+        ls = l.render_device(scene["t"], scene["de"], scene["dx"], scene["w"], scene["h"], p, scene["dl"],
+                             stats=True)
+        img = scene["dl"].download((scene["h"], scene["w"], 4), np.float32)
+    finally:
+        l.close()
+    q = psnr(img, plain)
+    print(f"baseline: psnr {q:.1f} dB sparse {ls['sparse_psnr']:.1f} samples {ls['samples']} "
+          f"(plain fetched {st['fetched']}) fell_back {ls['fell_back']}", flush=True)
+    assert q > 60.0 and not ls["fell_back"]
+
+
+def test_trained_network_gains_over_baseline(ctx, scene):
+    from voreen_amd import hipdvr as H, stepper
+    print("[case] train stepping network (24 TFs x 8 views) and render", flush=True)
+    mlp, info = stepper.train(ctx, scene["v"], scene["dims"], n_tfs=24, n_views=8, trace_size=128,
+                              epochs=6)
+    print(f"training: {info}", flush=True)
+    try:
+        assert info["rows"] > 1000 and info["loss_last"] < info["loss_first"]
+        results = {}
+        for shading in (0, 1):
+            plain, st, p = _plain(ctx, scene, shading)
+            l = H.Learned(ctx, scene["v"])
+            try:
+                for name, net in (("baseline", None), ("learned", mlp)):
+                    l.prepare(scene["t"], p, net)
+                    ls = l.render_device(scene["t"], scene["de"], scene["dx"], scene["w"], scene["h"], p,
+                                         scene["dl"], stats=True)
+                    img = scene["dl"].download((scene["h"], scene["w"], 4), np.float32)
+                    results[(shading, name)] = (psnr(img, plain), ls)
+                    print(f"shading {shading} {name}: psnr {psnr(img, plain):.1f} dB, "
+                          f"samples {ls['samples']}, sparse {ls['sparse_psnr']:.1f}, "
+                          f"fell_back {ls['fell_back']}", flush=True)
+            finally:
+                l.close()
+        for shading in (0, 1):
+            qb, lb = results[(shading, "baseline")]
+            ql, ll = results[(shading, "learned")]
+            if not ll["fell_back"]:
+                assert ql >= 45.0
+                assert ll["samples"] < lb["samples"]
+        assert any(not results[(s, "learned")][1]["fell_back"] for s in (0, 1))
+    finally:
+        mlp.close()
+
+
+def test_bad_weights_fall_back_to_plain(ctx, scene):
+    from voreen_amd import hipdvr as H
+    print("[case] learned fallback with a skip-everything network", flush=True)
+    plain, st, p = _plain(ctx, scene, 0)
+    m = H.Mlp(ctx, seed=3)
+    l = H.Learned(ctx, scene["v"])
+    try:
+        w = m.params()
+        w[-2] = 50.0      # b3[0]: skip logit for every brick
+        m.set_params(w)
+        l.prepare(scene["t"], p, m)
+        ls = l.render_device(scene["t"], scene["de"], scene["dx"], scene["w"], scene["h"], p, scene["dl"],
+                             stats=True)
+        img = scene["dl"].download((scene["h"], scene["w"], 4), np.float32)
+    finally:
+        l.close()
+        m.close()
+    print(f"fallback: sparse psnr {ls['sparse_psnr']:.1f} fell_back {ls['fell_back']}", flush=True)
+    assert ls["fell_back"] == 1 and ls["sparse_psnr"] < 45.0
+    assert np.array_equal(img, plain)

@@ -21,21 +21,10 @@
 #include "ctx.h"
 #include "errors.h"
 #include "kernels.h"
+#include "objects.h"
 #include "volume.h"
 
 using namespace hipdvr;
-
-struct hipdvr_tf {
-    hipdvr_ctx* ctx = nullptr;
-    uint64_t serial = 0;             // unique per TF object (occupancy cache key)
-    uint32_t width = 0;
-    float domain_lo = 0.f, domain_hi = 1.f;
-    uint32_t* d_rgba = nullptr;
-    uint32_t* d_alpha_prefix = nullptr;   // [width+1] count of non-zero alpha bytes below i
-    std::vector<uint8_t> host_rgba;
-    float4* d_preint = nullptr;
-    uint32_t preint_n = 0;
-};
 
 namespace {
 
@@ -56,6 +45,10 @@ uint64_t fbits(float f) {
     std::memcpy(&u, &f, 4);
     return u;
 }
+
+}  // namespace
+
+namespace hipdvr {
 
 // Per-brick occupancy of `vol` under the TF mapping of `tf` (and the volume's
 // real-world mapping); recomputed only when the pair changes.
@@ -148,9 +141,18 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.out = (float4*)d_out;
     a.w = (int32_t)w;
     a.h = (int32_t)h;
+    a.grid_w = (int32_t)w;
+    a.grid_h = (int32_t)h;
+    a.stride = 1;
+    a.pix_offset = 0;
+    a.cond_flag = nullptr;
     a.counters = counters;
     fill_alpha_table(p->step, a.alpha_corr);
 }
+
+}  // namespace hipdvr
+
+namespace {
 
 void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_tf* tf,
                         const float* d_entry, const float* d_exit, uint32_t w, uint32_t h,

@@ -1,0 +1,52 @@
+// learned.h -- learned stepping internals: brick size, kernel arguments,
+// launchers (see learned_march.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "march_common.h"
+
+namespace hipdvr {
+
+constexpr uint32_t kLBrickLog2 = 4;
+constexpr uint32_t kLBrick = 1u << kLBrickLog2;   // 16^3 bricks
+constexpr int kLFeatures = 16;
+constexpr int kLMaxMult = 8;                      // largest step multiplier
+
+struct FeatureArgs {
+    const float4* stats;        // per brick: min, max, mean, var (normalized)
+    uint32_t rows;              // bricks * 8
+    float rwm_scale, rwm_offset, dom_lo, dom_hi;
+    int32_t tf_width;
+    const float* alpha;         // [w] opacity-corrected alpha per TF index (base step)
+    const float* alpha_prefix;  // [w+1] prefix sums of alpha
+    const uint32_t* nz_prefix;  // [w+1] prefix counts of non-zero alpha bytes
+    float* x;                   // [rows][16] features out
+    uint8_t* occ16;             // [bricks] exact occupancy at 16^3 (written for octant 0)
+};
+
+struct LearnedArgs {
+    MarchArgs m;                // volume, TF, images, Phong, step (base)
+    int32_t u16;
+    uint32_t nb16[3];
+    const uint8_t* table;       // [bricks*8] bit 7 skip, bits 0..1 log2 multiplier
+    const uint8_t* occ16;       // [bricks] exact occupancy
+    const float* alpha_m;       // [8][256] opacity correction for step * 1 .. 8 (device)
+};
+
+void launch_brick_stats(const void* linear, int dtype, const uint32_t dims[3], const uint32_t nb[3],
+                        float4* out, hipStream_t st);
+void launch_features(const FeatureArgs& a, hipStream_t st);
+void launch_table(const float* y, uint32_t rows, uint8_t* table, float skip_logit, int max_log2,
+                  hipStream_t st);
+void launch_fill_table(uint8_t* table, uint32_t rows, uint8_t v, hipStream_t st);
+void launch_trace(const LearnedArgs& la, float* err, unsigned int* visits, hipStream_t st);
+void launch_targets(const float* err, const unsigned int* visits, const float* feat, uint32_t rows,
+                    float tau_skip, float tau_step, float* out_x, float* out_t, unsigned int* count,
+                    uint32_t capacity, hipStream_t st);
+void launch_learned(const LearnedArgs& la, bool phong, bool count, hipStream_t st);
+void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
+                  int offs, float* sse, int* flag, float mse_max, hipStream_t st);
+
+}  // namespace hipdvr

@@ -10,7 +10,10 @@
 #include "../../include/hipdvr_learned.h"
 #include "ctx.h"
 #include "errors.h"
+#include "kernels.h"
+#include "learned.h"
 #include "mlp.h"
+#include "objects.h"
 
 using namespace hipdvr;
 
@@ -186,3 +189,327 @@ namespace hipdvr {
 const MlpDevParams& mlp_dev_params(const hipdvr_mlp* m) { return m->dev; }
 hipdvr_ctx* mlp_ctx(const hipdvr_mlp* m) { return m->ctx; }
 }  // namespace hipdvr
+
+// ---------------------------------------------------------------------------
+// learned rendering
+// ---------------------------------------------------------------------------
+struct hipdvr_learned {
+    hipdvr_ctx* ctx = nullptr;
+    const hipdvr_volume* vol = nullptr;
+    uint32_t nb[3] = {0, 0, 0};
+    uint32_t bricks = 0, rows = 0;
+    float4* d_stats = nullptr;
+    uint8_t* d_occ16 = nullptr;
+    float* d_feat = nullptr;         // [rows][16]
+    float* d_y = nullptr;            // [rows][2]
+    uint8_t* d_table = nullptr;      // [rows]
+    float* d_err = nullptr;          // [rows][4]
+    unsigned int* d_visits = nullptr;
+    float* d_alpha = nullptr;        // [4096] corrected alpha (base step)
+    float* d_alpha_prefix = nullptr; // [4097]
+    uint32_t* d_nz_prefix = nullptr; // [4097]
+    float* d_alpha_m = nullptr;      // [8][256] corrected alpha for step * 1 .. 8
+    std::vector<float> h_alpha_m;    // host copy (source of the last upload)
+    float alpha_m_step = -1.f;       // step the tables were made for
+    float4* d_sparse = nullptr;      // sparse plain image
+    size_t sparse_cap = 0;
+    float* d_sse = nullptr;
+    int* d_flag = nullptr;
+    unsigned int* d_count = nullptr;
+};
+
+namespace {
+
+void free_learned(hipdvr_learned* l) {
+    if (!l) return;
+    (void)hipFree(l->d_stats);
+    (void)hipFree(l->d_occ16);
+    (void)hipFree(l->d_feat);
+    (void)hipFree(l->d_y);
+    (void)hipFree(l->d_table);
+    (void)hipFree(l->d_err);
+    (void)hipFree(l->d_visits);
+    (void)hipFree(l->d_alpha);
+    (void)hipFree(l->d_alpha_prefix);
+    (void)hipFree(l->d_nz_prefix);
+    (void)hipFree(l->d_alpha_m);
+    (void)hipFree(l->d_sparse);
+    (void)hipFree(l->d_sse);
+    (void)hipFree(l->d_flag);
+    (void)hipFree(l->d_count);
+    delete l;
+}
+
+hipdvr_learned_opts default_opts(const hipdvr_learned_opts* o) {
+    hipdvr_learned_opts d;
+    d.psnr_floor = 45.f;
+    d.remarch_stride = 8;
+    d.skip_logit = 0.f;
+    d.max_log2_step = 3;
+    if (o) {
+        if (o->psnr_floor > 0.f) d.psnr_floor = o->psnr_floor;
+        if (o->remarch_stride > 0) d.remarch_stride = o->remarch_stride;
+        d.skip_logit = o->skip_logit;
+        d.max_log2_step = o->max_log2_step < 0 ? 0 : (o->max_log2_step > 3 ? 3 : o->max_log2_step);
+    }
+    return d;
+}
+
+// Corrected alpha per TF index at the base step, its prefix sums and the
+// prefix counts of non-zero alpha bytes (feature inputs).
+void upload_tf_tables(hipdvr_learned* l, const hipdvr_tf* tf, float step, hipStream_t st) {
+    float corr[256];
+    fill_alpha_table(step, corr);
+    const uint32_t w = tf->width;
+    std::vector<float> alpha(w), pre(w + 1, 0.f);
+    std::vector<uint32_t> nz(w + 1, 0);
+    for (uint32_t i = 0; i < w; ++i) {
+        const uint8_t ab = tf->host_rgba[4 * i + 3];
+        alpha[i] = corr[ab];
+        pre[i + 1] = pre[i] + alpha[i];
+        nz[i + 1] = nz[i] + (ab != 0);
+    }
+    hip_check(hipMemcpyAsync(l->d_alpha, alpha.data(), w * 4, hipMemcpyHostToDevice, st), "upload");
+    hip_check(hipMemcpyAsync(l->d_alpha_prefix, pre.data(), (w + 1) * 4, hipMemcpyHostToDevice, st), "upload");
+    hip_check(hipMemcpyAsync(l->d_nz_prefix, nz.data(), (w + 1) * 4, hipMemcpyHostToDevice, st), "upload");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// Opacity-correction tables for step multiples 1..8, uploaded when the step
+// changes.
+void ensure_alpha_tables(hipdvr_learned* l, float step, hipStream_t st) {
+    if (l->alpha_m_step == step) return;
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");   // previous upload done with the host copy
+    l->h_alpha_m.assign((size_t)kLMaxMult * 256, 0.f);
+    for (int k = 0; k < kLMaxMult; ++k) fill_alpha_table(step * (float)(k + 1), &l->h_alpha_m[(size_t)k * 256]);
+    hip_check(hipMemcpyAsync(l->d_alpha_m, l->h_alpha_m.data(), l->h_alpha_m.size() * sizeof(float),
+                             hipMemcpyHostToDevice, st), "upload");
+    l->alpha_m_step = step;
+}
+
+void fill_learned_args(LearnedArgs& la, hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* p,
+                       const float* d_entry, const float* d_exit, float* d_out, uint32_t w, uint32_t h) {
+    ensure_alpha_tables(l, p->step, ctx_stream(l->ctx));
+    std::memset(&la, 0, sizeof(la));
+    fill_march_args(la.m, l->vol, tf, p, d_entry, d_exit, d_out, w, h, l->ctx->d_counters);
+    la.u16 = l->vol->desc.dtype == HIPDVR_U16 ? 1 : 0;
+    for (int i = 0; i < 3; ++i) la.nb16[i] = l->nb[i];
+    la.table = l->d_table;
+    la.occ16 = l->d_occ16;
+    la.alpha_m = l->d_alpha_m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_learned** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(ctx, [&] {
+        if (!vol || !out) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (vol->ctx != ctx) fail(HIPDVR_ERR_INVALID, "volume from another context");
+        const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
+        if (vol->tiled_elems * vsz > 0xffffff00ull)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping supports volumes below 4 GiB");
+        ctx_set_device(ctx);
+        hipdvr_learned* l = new hipdvr_learned();
+        l->ctx = ctx;
+        l->vol = vol;
+        try {
+            l->bricks = 1;
+            for (int i = 0; i < 3; ++i) {
+                l->nb[i] = (vol->desc.dims[i] + kLBrick - 1) / kLBrick;
+                l->bricks *= l->nb[i];
+            }
+            l->rows = l->bricks * 8;
+            hip_check(hipMalloc((void**)&l->d_stats, l->bricks * sizeof(float4)), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_occ16, l->bricks), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_feat, (size_t)l->rows * kLFeatures * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_y, (size_t)l->rows * 2 * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_table, l->rows), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_err, (size_t)l->rows * 4 * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_visits, (size_t)l->rows * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_alpha, 4096 * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_alpha_prefix, 4097 * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_nz_prefix, 4097 * 4), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_alpha_m, (size_t)kLMaxMult * 256 * sizeof(float)), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_sse, 64), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_flag, 64), "hipMalloc");
+            hip_check(hipMalloc((void**)&l->d_count, 64), "hipMalloc");
+            hipStream_t st = ctx_stream(ctx);
+            launch_brick_stats(vol->d_linear, vol->desc.dtype, vol->desc.dims, l->nb, l->d_stats, st);
+            launch_fill_table(l->d_table, l->rows, 0, st);
+            hip_check(hipGetLastError(), "brick stats launch");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        } catch (...) {
+            free_learned(l);
+            throw;
+        }
+        *out = l;
+    });
+}
+
+int hipdvr_learned_destroy(hipdvr_learned* l) {
+    if (!l) return HIPDVR_OK;
+    (void)hipStreamSynchronize(ctx_stream(l->ctx));
+    free_learned(l);
+    return HIPDVR_OK;
+}
+
+int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* prm, hipdvr_mlp* mlp,
+                           const hipdvr_learned_opts* o) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        if (!tf || !prm) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (tf->ctx != l->ctx || (mlp && mlp->ctx != l->ctx)) fail(HIPDVR_ERR_INVALID, "handle from another context");
+        validate_params(prm, tf);
+        const hipdvr_learned_opts opts = default_opts(o);
+        ctx_set_device(l->ctx);
+        hipStream_t st = ctx_stream(l->ctx);
+        upload_tf_tables(l, tf, prm->step, st);
+        FeatureArgs fa;
+        fa.stats = l->d_stats;
+        fa.rows = l->rows;
+        fa.rwm_scale = l->vol->desc.rwm_scale;
+        fa.rwm_offset = l->vol->desc.rwm_offset;
+        fa.dom_lo = tf->domain_lo;
+        fa.dom_hi = tf->domain_hi;
+        fa.tf_width = (int32_t)tf->width;
+        fa.alpha = l->d_alpha;
+        fa.alpha_prefix = l->d_alpha_prefix;
+        fa.nz_prefix = l->d_nz_prefix;
+        fa.x = l->d_feat;
+        fa.occ16 = l->d_occ16;
+        launch_features(fa, st);
+        if (mlp) {
+            launch_mlp_forward(mlp->dev, l->d_feat, l->rows, l->d_y, st);
+            launch_table(l->d_y, l->rows, l->d_table, opts.skip_logit, opts.max_log2_step, st);
+        } else {
+            launch_fill_table(l->d_table, l->rows, 0, st);
+        }
+        hip_check(hipGetLastError(), "learned prepare launch");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+int hipdvr_learned_trace_reset(hipdvr_learned* l) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        ctx_set_device(l->ctx);
+        hipStream_t st = ctx_stream(l->ctx);
+        hip_check(hipMemsetAsync(l->d_err, 0, (size_t)l->rows * 16, st), "memset");
+        hip_check(hipMemsetAsync(l->d_visits, 0, (size_t)l->rows * 4, st), "memset");
+    });
+}
+
+int hipdvr_learned_trace(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* prm, const float* d_entry,
+                         const float* d_exit, uint32_t w, uint32_t h) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        if (!tf || !prm || !d_entry || !d_exit || !w || !h) fail(HIPDVR_ERR_INVALID, "bad arguments");
+        validate_params(prm, tf);
+        if (prm->filter != HIPDVR_FILTER_LINEAR || prm->classification != HIPDVR_CLASS_TF)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
+        ctx_set_device(l->ctx);
+        LearnedArgs la;
+        fill_learned_args(la, l, tf, prm, d_entry, d_exit, nullptr, w, h);
+        launch_trace(la, l->d_err, l->d_visits, ctx_stream(l->ctx));
+        hip_check(hipGetLastError(), "trace launch");
+    });
+}
+
+int hipdvr_learned_collect(hipdvr_learned* l, float tau_skip, float tau_step, float* d_x, float* d_t,
+                           uint32_t capacity, uint32_t* count) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        if (!d_x || !d_t || !count) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        ctx_set_device(l->ctx);
+        hipStream_t st = ctx_stream(l->ctx);
+        const unsigned int start = *count;
+        hip_check(hipMemcpyAsync(l->d_count, &start, 4, hipMemcpyHostToDevice, st), "upload");
+        launch_targets(l->d_err, l->d_visits, l->d_feat, l->rows, tau_skip, tau_step, d_x, d_t, l->d_count,
+                       capacity, st);
+        hip_check(hipGetLastError(), "targets launch");
+        unsigned int n = 0;
+        hip_check(hipMemcpyAsync(&n, l->d_count, 4, hipMemcpyDeviceToHost, st), "download");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *count = n < capacity ? n : capacity;
+    });
+}
+
+int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
+                                 const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
+                                 const hipdvr_learned_opts* o, float* d_out, hipdvr_learned_stats* stats) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        if (!tf || !prm || !d_entry || !d_exit || !d_out || !w || !h) fail(HIPDVR_ERR_INVALID, "bad arguments");
+        validate_params(prm, tf);
+        if (prm->filter != HIPDVR_FILTER_LINEAR || prm->classification != HIPDVR_CLASS_TF)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
+        const hipdvr_learned_opts opts = default_opts(o);
+        ctx_set_device(l->ctx);
+        hipStream_t st = ctx_stream(l->ctx);
+        hipdvr_volume* vol = const_cast<hipdvr_volume*>(l->vol);
+        ensure_occupancy(l->ctx, vol, tf);
+        // sparse judge pixels: (gx*stride + offs, gy*stride + offs), inside the image
+        const uint32_t stride = (uint32_t)opts.remarch_stride;
+        uint32_t offs = stride / 2;
+        if (offs > w - 1) offs = w - 1;
+        if (offs > h - 1) offs = h - 1;
+        const int swc = (int)((w - 1 - offs) / stride) + 1;
+        const int shc = (int)((h - 1 - offs) / stride) + 1;
+        const size_t need = (size_t)swc * shc * sizeof(float4);
+        if (need > l->sparse_cap) {
+            if (l->d_sparse) (void)hipFree(l->d_sparse);
+            l->d_sparse = nullptr;
+            l->sparse_cap = 0;
+            hip_check(hipMalloc((void**)&l->d_sparse, need), "hipMalloc(sparse)");
+            l->sparse_cap = need;
+        }
+        const bool phong = prm->shading == HIPDVR_SHADE_PHONG;
+        const bool count = stats != nullptr;
+        if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
+        const bool timed = ctx_timing_start(l->ctx);
+        // 1. learned frame
+        LearnedArgs la;
+        fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
+        launch_learned(la, phong, count, st);
+        // 2. sparse plain re-march (bricked exact kernel on every stride-th pixel)
+        MarchArgs sp;
+        fill_march_args(sp, vol, tf, prm, d_entry, d_exit, (float*)l->d_sparse, w, h, nullptr);
+        sp.grid_w = swc;
+        sp.grid_h = shc;
+        sp.stride = (int32_t)stride;
+        sp.pix_offset = (int32_t)offs;
+        const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
+        const bool wide = vol->tiled_elems * vsz > 0xffffff00ull;
+        launch_march_fast(sp, vol->desc.dtype, prm->filter, phong, false, wide, st);
+        // 3. judge: flag = PSNR below the floor
+        const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);
+        launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)offs,
+                     l->d_sse, l->d_flag, mse_max, st);
+        // 4. fallback: the plain frame, launched always, runs only if flagged
+        MarchArgs fb;
+        fill_march_args(fb, vol, tf, prm, d_entry, d_exit, d_out, w, h, nullptr);
+        fb.cond_flag = l->d_flag;
+        launch_march_fast(fb, vol->desc.dtype, prm->filter, phong, false, wide, st);
+        hip_check(hipGetLastError(), "learned render launch");
+        if (timed) ctx_timing_stop(l->ctx);
+        if (count) {
+            unsigned long long c[kNumCounters];
+            float sse = 0.f;
+            int flag = 0;
+            hip_check(hipMemcpyAsync(c, l->ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st), "download");
+            hip_check(hipMemcpyAsync(&sse, l->d_sse, 4, hipMemcpyDeviceToHost, st), "download");
+            hip_check(hipMemcpyAsync(&flag, l->d_flag, 4, hipMemcpyDeviceToHost, st), "download");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            const double mse = (double)sse / ((double)swc * shc * 4);
+            stats->samples = c[1];
+            stats->shaded = c[3];
+            stats->sparse_psnr = mse > 0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.f;
+            stats->fell_back = flag;
+        }
+    });
+}
+
+}  // extern "C"

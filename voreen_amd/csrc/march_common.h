@@ -48,7 +48,11 @@ struct MarchArgs {
     const float4* entry;
     const float4* exit;
     float4* out;
-    int32_t w, h;
+    int32_t w, h;               // image size (entry/exit indexing)
+    // launch grid in pixels: grid pixel (gx, gy) marches image pixel
+    // (gx*stride + pix_offset, gy*stride + pix_offset), writes out[gy*grid_w + gx]
+    int32_t grid_w, grid_h, stride, pix_offset;
+    const int* cond_flag;       // if set, the launch does nothing unless *cond_flag != 0
     unsigned long long* counters;  // [rays, samples, fetched, shaded, lane_slots, shade_slots]
     float alpha_corr[256];      // 1 - pow(1 - a/255, step*200) per alpha byte
 };

@@ -139,6 +139,77 @@ __device__ __forceinline__ void gradient_planes(const TiledFetch<T, WIDE>& f, co
     }
 }
 
+// Interior form of gradient_planes: when every axis has u_i in [1, dim_i - 3)
+// and a proper fraction (the center footprint spans lo..lo+1), u_i +- 1 is
+// exact in float, the stencil planes are lo-1, lo, lo+1, lo+2 with the
+// sample's own fraction, and the two inner planes are the sample's corner
+// values v[] (already normalized); only the outer planes are loaded.
+template <typename T, bool WIDE>
+__device__ __forceinline__ bool gradient_interior(const TiledFetch<T, WIDE>& f, const MarchArgs& a,
+                                                  const float u[3], const Corners& c, const float v[8],
+                                                  float g[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (!(u[i] >= 1.0f) || !(u[i] + 3.0f < a.fdims[i]) || c.d[i] == 0) return false;
+    typedef typename TiledFetch<T, WIDE>::off_t off_t;
+    const float norm = sizeof(T) == 2 ? (1.0f / 65535.0f) : (1.0f / 255.0f);
+    const float px = c.p[0], py = c.p[1], pz = c.p[2];
+    const float qx = 1.f - px, qy = 1.f - py, qz = 1.f - pz;
+    // corner order: 0 (x0y0z0) 1 (x1y0z0) 2 (x1y1z0) 3 (x0y1z0) 4 (x0y0z1) 5 (x1y0z1) 6 (x1y1z1) 7 (x0y1z1)
+    const off_t ox0 = f.ox(c.lo[0]), ox1 = f.ox(c.lo[0] + 1);
+    const off_t oy0 = f.oy(c.lo[1]), oy1 = f.oy(c.lo[1] + 1);
+    const off_t oz0 = f.oz(c.lo[2]), oz1 = f.oz(c.lo[2] + 1);
+    // x: planes lo-1 and lo+2, bilinear over (y, z)
+    {
+        const off_t xm = f.ox(c.lo[0] - 1), xp = f.ox(c.lo[0] + 2);
+        const float wyz[4] = {qy * qz, py * qz, qy * pz, py * pz};
+        const off_t oyz[4] = {oy0 + oz0, oy1 + oz0, oy0 + oz1, oy1 + oz1};
+        float bm = 0.f, bp = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bm = bm + (float)f.load(xm + oyz[k]) * wyz[k];
+            bp = bp + (float)f.load(xp + oyz[k]) * wyz[k];
+        }
+        const float b0 = v[0] * wyz[0] + v[3] * wyz[1] + v[4] * wyz[2] + v[7] * wyz[3];
+        const float b1 = v[1] * wyz[0] + v[2] * wyz[1] + v[5] * wyz[2] + v[6] * wyz[3];
+        const float vm = qx * (bm * norm) + px * b0, vpl = qx * b1 + px * (bp * norm);
+        g[0] = ((vm - vpl) * a.grad_scale[0]) * a.rwm_scale;
+    }
+    // y: planes lo-1 and lo+2, bilinear over (x, z)
+    {
+        const off_t ym = f.oy(c.lo[1] - 1), yp = f.oy(c.lo[1] + 2);
+        const float wxz[4] = {qx * qz, px * qz, qx * pz, px * pz};
+        const off_t oxz[4] = {ox0 + oz0, ox1 + oz0, ox0 + oz1, ox1 + oz1};
+        float bm = 0.f, bp = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bm = bm + (float)f.load(ym + oxz[k]) * wxz[k];
+            bp = bp + (float)f.load(yp + oxz[k]) * wxz[k];
+        }
+        const float b0 = v[0] * wxz[0] + v[1] * wxz[1] + v[4] * wxz[2] + v[5] * wxz[3];
+        const float b1 = v[3] * wxz[0] + v[2] * wxz[1] + v[7] * wxz[2] + v[6] * wxz[3];
+        const float vm = qy * (bm * norm) + py * b0, vpl = qy * b1 + py * (bp * norm);
+        g[1] = ((vm - vpl) * a.grad_scale[1]) * a.rwm_scale;
+    }
+    // z: planes lo-1 and lo+2, bilinear over (x, y)
+    {
+        const off_t zm = f.oz(c.lo[2] - 1), zp = f.oz(c.lo[2] + 2);
+        const float wxy[4] = {qx * qy, px * qy, qx * py, px * py};
+        const off_t oxy[4] = {ox0 + oy0, ox1 + oy0, ox0 + oy1, ox1 + oy1};
+        float bm = 0.f, bp = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bm = bm + (float)f.load(zm + oxy[k]) * wxy[k];
+            bp = bp + (float)f.load(zp + oxy[k]) * wxy[k];
+        }
+        const float b0 = v[0] * wxy[0] + v[1] * wxy[1] + v[3] * wxy[2] + v[2] * wxy[3];
+        const float b1 = v[4] * wxy[0] + v[5] * wxy[1] + v[7] * wxy[2] + v[6] * wxy[3];
+        const float vm = qz * (bm * norm) + pz * b0, vpl = qz * b1 + pz * (bp * norm);
+        g[2] = ((vm - vpl) * a.grad_scale[2]) * a.rwm_scale;
+    }
+    return true;
+}
+
 // Ray parameter below which every sample lies in the (empty) brick holding
 // the voxel position vp: the smallest exit parameter over the three axes of
 // the brick's voxel slab shrunk by 0.05 voxel (far above the float error of
@@ -174,6 +245,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
     const uint32_t b = blockIdx.x;
     const uint32_t tile = (b & 7u) * tiles_per_xcd + (b >> 3);
     if (tile >= tiles_total) return;   // whole workgroup leaves together
+    if (a.cond_flag && *a.cond_flag == 0) return;
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[256];
     const int tid = threadIdx.x;
@@ -182,9 +254,10 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
-    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-    const bool inside = (px < a.w) && (py < a.h);
+    const int gx = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int gy = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
+    const bool inside = (gx < a.grid_w) && (gy < a.grid_h) && (px < a.w) && (py < a.h);
 
     TiledFetch<T, WIDE> f;
     f.p = (const T*)a.bricks;
@@ -294,8 +367,10 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
                                 float gr[3];
                                 if (tt == 0.0f) {
                                     clip_border_gradient(a, px, py, ss, dir, gr);
-                                } else {
-                                    if (FILTER == 0) corners(pp, a.dims, cc);
+                                } else if (FILTER == 0) {
+                                    corners(pp, a.dims, cc);
+                                    gradient_planes<T, WIDE>(f, a, uu, cc, gr);
+                                } else if (!gradient_interior<T, WIDE>(f, a, uu, cc, vv, gr)) {
                                     gradient_planes<T, WIDE>(f, a, uu, cc, gr);
                                 }
                                 float pos[3], col[3] = {cr, cg, cb};
@@ -341,7 +416,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
             }
             res = make_float4(r, g, bb, A);
         }
-        a.out[pix] = res;
+        a.out[(size_t)gy * a.grid_w + gx] = res;
     }
     if (COUNT) {
         wave_add(a.counters + 0, n_rays);
@@ -355,7 +430,7 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
 
 template <typename T, bool WIDE, int FILTER, bool PHONG>
 static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
-    const uint32_t tx = (a.w + 15) / 16, ty = (a.h + 15) / 16;
+    const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
     const uint32_t total = tx * ty;
     const uint32_t per_xcd = (total + 7) / 8;
     const dim3 grid(per_xcd * 8);

@@ -34,6 +34,9 @@ EXPORTS = (
     "hipdvr_timing_begin", "hipdvr_timing_read", "hipdvr_copy_device",
     "hipdvr_mlp_create", "hipdvr_mlp_destroy", "hipdvr_mlp_get_params", "hipdvr_mlp_set_params",
     "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
+    "hipdvr_learned_create", "hipdvr_learned_destroy", "hipdvr_learned_prepare",
+    "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
+    "hipdvr_render_learned_device",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -65,6 +68,19 @@ class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("fetched", C.c_uint64),
                 ("shaded", C.c_uint64), ("lane_slots", C.c_uint64), ("shade_slots", C.c_uint64),
                 ("kernel_ms", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LearnedOpts(C.Structure):
+    _fields_ = [("psnr_floor", C.c_float), ("remarch_stride", C.c_int32), ("skip_logit", C.c_float),
+                ("max_log2_step", C.c_int32)]
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("shaded", C.c_uint64), ("sparse_psnr", C.c_float),
+                ("fell_back", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -128,6 +144,14 @@ def load():
         "hipdvr_mlp_forward": ([vp, vp, u32, vp], C.c_int),
         "hipdvr_mlp_gradient": ([vp, vp, vp, u32, vp, P(C.c_float)], C.c_int),
         "hipdvr_mlp_train_step": ([vp, vp, vp, u32, C.c_float, P(C.c_float)], C.c_int),
+        "hipdvr_learned_create": ([vp, vp, P(vp)], C.c_int),
+        "hipdvr_learned_destroy": ([vp], C.c_int),
+        "hipdvr_learned_prepare": ([vp, vp, P(Params), vp, P(LearnedOpts)], C.c_int),
+        "hipdvr_learned_trace_reset": ([vp], C.c_int),
+        "hipdvr_learned_trace": ([vp, vp, P(Params), vp, vp, u32, u32], C.c_int),
+        "hipdvr_learned_collect": ([vp, C.c_float, C.c_float, vp, vp, u32, P(u32)], C.c_int),
+        "hipdvr_render_learned_device": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
+                                          P(LearnedStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -432,3 +456,51 @@ class Mlp:
                                                           C.byref(loss) if want_loss else None),
                        "hipdvr_mlp_train_step")
         return loss.value if want_loss else None
+
+
+def learned_opts(psnr_floor=45.0, remarch_stride=8, skip_logit=0.0, max_log2_step=3):
+    o = LearnedOpts()
+    o.psnr_floor, o.remarch_stride, o.skip_logit, o.max_log2_step = (psnr_floor, remarch_stride,
+                                                                      skip_logit, max_log2_step)
+    return o
+
+
+class Learned:
+    """Learned-stepping state of one volume (include/hipdvr_learned.h)."""
+
+    def __init__(self, ctx, vol):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.hipdvr_learned_create(ctx.h, vol.h, C.byref(self.h)), "hipdvr_learned_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipdvr_learned_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def prepare(self, tf, params, mlp=None, opts=None):
+        self.ctx.check(self.ctx.lib.hipdvr_learned_prepare(self.h, tf.h, C.byref(params),
+                                                           mlp.h if mlp else None,
+                                                           C.byref(opts) if opts else None),
+                       "hipdvr_learned_prepare")
+
+    def trace_reset(self):
+        self.ctx.check(self.ctx.lib.hipdvr_learned_trace_reset(self.h), "hipdvr_learned_trace_reset")
+
+    def trace(self, tf, params, d_entry, d_exit, w, h):
+        self.ctx.check(self.ctx.lib.hipdvr_learned_trace(self.h, tf.h, C.byref(params), d_entry.ptr,
+                                                         d_exit.ptr, w, h), "hipdvr_learned_trace")
+
+    def collect(self, tau_skip, tau_step, dx, dt, capacity, count):
+        n = C.c_uint32(count)
+        self.ctx.check(self.ctx.lib.hipdvr_learned_collect(self.h, tau_skip, tau_step, dx.ptr, dt.ptr,
+                                                           capacity, C.byref(n)), "hipdvr_learned_collect")
+        return n.value
+
+    def render_device(self, tf, d_entry, d_exit, w, h, params, d_out, opts=None, stats=False):
+        st = LearnedStats()
+        self.ctx.check(self.ctx.lib.hipdvr_render_learned_device(
+            self.h, tf.h, d_entry.ptr, d_exit.ptr, w, h, C.byref(params),
+            C.byref(opts) if opts else None, d_out.ptr, C.byref(st) if stats else None),
+            "hipdvr_render_learned_device")
+        return st.as_dict() if stats else None
