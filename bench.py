@@ -13,7 +13,8 @@ kernel (algorithmic bytes per launch / HIP-event launch time), and the CPU
 baseline (the FP32 CPURaycaster restatement, no shading, on the host's cores,
 on a bounded band of rows of view 0).
 
-    python bench.py [--steps K] [--warmup W] [--config c1|c0] [--kernel auto|reference|bricked]
+    python bench.py [--steps K] [--warmup W] [--config c1|c0] [--kernel auto|reference|bricked|cell]
+                    [--batch 2|4|8]
 """
 import argparse
 import json
@@ -35,7 +36,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c1", choices=["c0", "c1"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
+    ap.add_argument("--batch", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="cell kernel: samples fetched together per ray (0 = library default)")
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
     ap.add_argument("--shading", default="default", choices=["default", "none", "phong"],
                     help="override the workload's shading (changes the workload name)")
@@ -89,7 +92,7 @@ def main():
     wimg, himg = cfg["image"]
     nviews = args.views or cfg["views"]
     kernel = {"auto": H.KERNEL_AUTO, "reference": H.KERNEL_REFERENCE,
-              "bricked": H.KERNEL_BRICKED}[args.kernel]
+              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL}[args.kernel]
     t_setup = time.perf_counter()
     vol = W.make_volume(args.config)
     keys, tf = W.make_tf(args.config)
@@ -109,7 +112,7 @@ def main():
         de, dx, do = ctx.alloc(img_bytes), ctx.alloc(img_bytes), ctx.alloc(img_bytes)
         ctx.entry_exit_device(cam, llf, urb, wimg, himg, de, dx)
         bufs.append((de, dx, do))
-    params = [H.default_params(step, shading=cfg["shading"], kernel=kernel,
+    params = [H.default_params(step, shading=cfg["shading"], kernel=kernel, batch=args.batch,
                                camera_pos=tuple(cam.eye)) for cam in cams]
     ctx.synchronize()
 
@@ -166,6 +169,7 @@ def main():
                          f"{'u8' if vsz == 1 else 'u16'}, {wimg}x{himg}, step 1/{int(round(1 / step))}, "
                          f"{'phong' if cfg['shading'] else 'no shading'}, {nviews} orbit views per step"),
             "kernel": args.kernel,
+            "batch": args.batch,
             "voxel": "u8" if vsz == 1 else "u16",
         },
         "gsamples_per_s": round(samples * args.steps / elapsed / 1e9, 4),

@@ -96,7 +96,9 @@ enum { HIPDVR_ALPHA_ZERO = 0, HIPDVR_ALPHA_USE = 1, HIPDVR_ALPHA_ONE = 2 };
 enum {
     HIPDVR_KERNEL_AUTO = 0,      /* fastest exact kernel for the options */
     HIPDVR_KERNEL_REFERENCE = 1, /* linear layout, one lane per ray, no skipping */
-    HIPDVR_KERNEL_BRICKED = 2    /* bricked layout + exact empty-space skipping */
+    HIPDVR_KERNEL_BRICKED = 2,   /* bricked layout + exact empty-space skipping */
+    HIPDVR_KERNEL_CELL = 3       /* corner-replicated cell records + exact empty-space
+                                    skipping; linear filter, TF classification */
 };
 
 typedef struct hipdvr_ctx hipdvr_ctx;
@@ -129,6 +131,8 @@ typedef struct hipdvr_params {
     float camera_pos[3];    /* physical space */
     int32_t loop_cap;       /* 0 = 255*255 (cpuraycaster.cpp:239) */
     int32_t kernel;         /* HIPDVR_KERNEL_* */
+    int32_t batch;          /* cell kernel: samples fetched together per ray (2, 4, 8;
+                               0 = default) */
 } hipdvr_params;
 
 typedef struct hipdvr_stats {

@@ -22,10 +22,17 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-3
 
 
-@pytest.fixture(params=["reference", "bricked"])
+@pytest.fixture(params=["reference", "bricked", "cell"])
 def kernel(request):
     from voreen_amd import hipdvr as H
-    return {"reference": H.KERNEL_REFERENCE, "bricked": H.KERNEL_BRICKED}[request.param]
+    return {"reference": H.KERNEL_REFERENCE, "bricked": H.KERNEL_BRICKED,
+            "cell": H.KERNEL_CELL}[request.param]
+
+
+def filters_for(kernel):
+    """Filter modes a kernel serves (the cell records hold trilinear footprints)."""
+    from voreen_amd import hipdvr as H
+    return (1,) if kernel == H.KERNEL_CELL else (0, 1)
 
 
 def say(msg):
@@ -120,7 +127,7 @@ def test_c0_full_frame(ctx, oracle, kernel):
 
 def test_c0_orbit_views_nearest_and_linear(ctx, oracle, kernel):
     for view in (5, 13, 29):
-        for filt in (0, 1):
+        for filt in filters_for(kernel):
             say(f"c0 view {view}/36 filter {filt} 160^2 kernel {kernel}")
             vol, tf, en, ex, step = c0_inputs(oracle, size=160, view=view, nviews=36)
             r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, filter_=filt,
@@ -144,7 +151,7 @@ def test_nucleon_regression_workspace_setup(ctx, oracle, kernel):
                                (-0.74486113, 0.22378904, 0.62857014), 45.0, 0.1, 200, 200,
                                llf=llf, urb=urb, quantize=True)
     step = float(np.float32(1.0) / (np.float32(41) * np.float32(2.0)))
-    for filt in (1, 0):
+    for filt in sorted(filters_for(kernel), reverse=True):
         say(f"nucleon 41^3 u8 200^2 filter {filt} kernel {kernel}")
         r = render_both(ctx, oracle, v["data"], tf, t["domain"], en, ex, step, rwm=(255.0, 0.0),
                         filter_=filt, spacing=tuple(sp), offset=v["offset"], kernel=kernel)
@@ -273,9 +280,10 @@ def test_invalid_arguments_return_codes(ctx):
 
 
 def test_bricked_equals_reference_and_skips(ctx, oracle):
-    """The bricked kernel must produce the reference kernel's image bit for bit
-    while reconstructing fewer samples; a TF change must refresh its brick
-    occupancy (same volume, second TF)."""
+    """The bricked and cell kernels must produce the reference kernel's image
+    bit for bit (the cell kernel also when shaded) while reconstructing fewer
+    samples; a TF change must refresh the brick occupancy (same volume,
+    second TF)."""
     from voreen_amd import hipdvr as H, workloads as W
     dims = (160, 144, 128)
     vol = W.make_volume("c1", dims=dims)
@@ -291,14 +299,22 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             tf = oracle.tf_build(tfk, 4096)
             t = ctx.transfunc(tf)
             outs = {}
-            for name, k in (("ref", H.KERNEL_REFERENCE), ("brick", H.KERNEL_BRICKED)):
+            runs = [("ref", H.KERNEL_REFERENCE, 0), ("brick", H.KERNEL_BRICKED, 0)]
+            runs += [(f"cell{b}", H.KERNEL_CELL, b) for b in (2, 4, 8)]
+            for name, k, batch in runs:
                 for shading in (0, 1):
                     say(f"bricked-vs-reference tf{seed} {name} shading {shading}")
-                    p = H.default_params(step, shading=shading, kernel=k, camera_pos=tuple(cam.eye))
+                    p = H.default_params(step, shading=shading, kernel=k, camera_pos=tuple(cam.eye),
+                                         batch=batch)
                     outs[(name, shading)] = ctx.render(v, t, en, ex, p)
             t.close()
             for shading in (0, 1):
                 a, sa = outs[("ref", shading)]
+                for b in (2, 4, 8):
+                    c, sc = outs[(f"cell{b}", shading)]
+                    assert np.array_equal(a, c), f"cell batch {b} tf{seed} shading {shading}"
+                    assert sa["samples"] == sc["samples"] and sa["shaded"] == sc["shaded"]
+                    assert sc["fetched"] < sa["fetched"] or seed == 1
                 b, sb = outs[("brick", shading)]
                 if shading == 0:
                     assert np.array_equal(a, b), f"tf{seed} shading {shading}"

@@ -140,7 +140,7 @@ def _c0_case(oracle, size=128):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernel", ["auto", "reference", "bricked"])
+@pytest.mark.parametrize("kernel", ["auto", "reference", "bricked", "cell"])
 def test_process_matches_oracle(rc, oracle, kernel):
     vol, keys, en, ex, cam = _c0_case(oracle)
     print(f"[case] HIPRaycaster.process kernel {kernel}", flush=True)

@@ -96,8 +96,10 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_UNSUPPORTED, "only DVR compositing is implemented");
     if (p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
-    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_BRICKED)
+    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
+    if (p->batch != 0 && p->batch != 2 && p->batch != 4 && p->batch != 8)
+        fail(HIPDVR_ERR_INVALID, "batch must be 0, 2, 4 or 8");
 }
 
 void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf,
@@ -168,17 +170,31 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     fill_march_args(a, vol, tf, p, d_entry, d_exit, d_out, w, h, ctx->d_counters);
     const bool phong = p->shading == HIPDVR_SHADE_PHONG;
     int kernel = p->kernel;
-    // the bricked kernel skips samples, which the pre-integrated mode cannot
-    // (its lookup pairs every sample with the previous one)
-    if (kernel == HIPDVR_KERNEL_AUTO)
-        kernel = p->classification == HIPDVR_CLASS_TF ? HIPDVR_KERNEL_BRICKED : HIPDVR_KERNEL_REFERENCE;
-    if (kernel == HIPDVR_KERNEL_BRICKED && p->classification != HIPDVR_CLASS_TF)
-        fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel supports transfer-function classification only");
-    if (kernel == HIPDVR_KERNEL_BRICKED) ensure_occupancy(ctx, const_cast<hipdvr_volume*>(vol), tf);
+    // the skipping kernels drop samples, which the pre-integrated mode cannot
+    // (its lookup pairs every sample with the previous one); the cell records
+    // serve the linear filter
+    if (kernel == HIPDVR_KERNEL_AUTO) {
+        if (p->classification != HIPDVR_CLASS_TF) kernel = HIPDVR_KERNEL_REFERENCE;
+        else if (p->filter == HIPDVR_FILTER_LINEAR && cells_allowed(vol)) kernel = HIPDVR_KERNEL_CELL;
+        else kernel = HIPDVR_KERNEL_BRICKED;
+    }
+    if ((kernel == HIPDVR_KERNEL_BRICKED || kernel == HIPDVR_KERNEL_CELL) && p->classification != HIPDVR_CLASS_TF)
+        fail(HIPDVR_ERR_UNSUPPORTED, "the skipping kernels support transfer-function classification only");
+    if (kernel == HIPDVR_KERNEL_CELL && p->filter != HIPDVR_FILTER_LINEAR)
+        fail(HIPDVR_ERR_UNSUPPORTED, "the cell kernel supports the linear filter only");
+    hipdvr_volume* mvol = const_cast<hipdvr_volume*>(vol);
+    if (kernel == HIPDVR_KERNEL_BRICKED || kernel == HIPDVR_KERNEL_CELL) ensure_occupancy(ctx, mvol, tf);
+    if (kernel == HIPDVR_KERNEL_CELL) {
+        ensure_cells(mvol, ctx->stream);
+        a.cells = mvol->d_cells;
+    }
     if (count) hip_check(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     const bool timed = ctx_timing_start(ctx);
     if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
+    } else if (kernel == HIPDVR_KERNEL_CELL) {
+        launch_march_cell(a, vol->desc.dtype, phong, count, mvol->cells_wide, p->batch ? p->batch : kDefaultBatch,
+                          ctx->stream);
     } else {
         // buffer descriptors address at most 2^32 - 1 bytes
         const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;

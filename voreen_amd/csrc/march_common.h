@@ -23,6 +23,7 @@ constexpr int kNumCounters = 6;
 struct MarchArgs {
     const void* vox;            // linear voxels (reference kernel)
     const void* bricks;         // 4^3-block tiled voxels (bricked kernel)
+    const void* cells;          // corner-replicated cell records (cell kernel)
     const uint8_t* brick_occ;   // per-brick occupancy (1 = may hold alpha > 0) for this TF
     uint32_t dims[3];
     uint32_t nbricks[3];
@@ -282,6 +283,32 @@ __device__ __forceinline__ float ray_setup(const float first[3], const float las
 __device__ __forceinline__ bool is_background(float4 f, float4 l) {
     return f.x == 0.f && f.y == 0.f && f.z == 0.f && f.w == 0.f && l.x == 0.f && l.y == 0.f &&
            l.z == 0.f && l.w == 0.f;
+}
+
+// Ray parameter below which every sample lies in the (empty) brick holding
+// the voxel position vp: the smallest exit parameter over the three axes of
+// the brick's voxel slab shrunk by 0.05 voxel (far above the float error of
+// the sample position), in unclamped voxel space where the first and last
+// brick of an axis extend to infinity (positions are clamped to the volume).
+__device__ __forceinline__ float empty_brick_limit(const MarchArgs& a, const float first[3],
+                                                   const float dir[3], const float vp[3],
+                                                   uint32_t bl) {
+    float tl = INFINITY;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t b = (uint32_t)vp[i] >> bl;
+        const float B = (float)(1u << bl);
+        if (dir[i] > 0.0f && b + 1 < a.nbricks[i]) {
+            const float vb = (float)(b + 1) * B - 0.05f;           // voxel coordinate bound
+            const float sb = (vb + 0.5f) / a.fdims[i];               // texture coordinate
+            tl = fminf(tl, (sb - first[i]) / dir[i]);
+        } else if (dir[i] < 0.0f && b > 0) {
+            const float vb = (float)b * B + 0.05f;
+            const float sb = (vb + 0.5f) / a.fdims[i];
+            tl = fminf(tl, (sb - first[i]) / dir[i]);
+        }
+    }
+    return tl;
 }
 
 // 64-lane maximum of a per-lane counter, times 64, added once per wavefront.

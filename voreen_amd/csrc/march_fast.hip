@@ -210,32 +210,6 @@ __device__ __forceinline__ bool gradient_interior(const TiledFetch<T, WIDE>& f, 
     return true;
 }
 
-// Ray parameter below which every sample lies in the (empty) brick holding
-// the voxel position vp: the smallest exit parameter over the three axes of
-// the brick's voxel slab shrunk by 0.05 voxel (far above the float error of
-// the sample position), in unclamped voxel space where the first and last
-// brick of an axis extend to infinity (positions are clamped to the volume).
-__device__ __forceinline__ float empty_brick_limit(const MarchArgs& a, const float first[3],
-                                                   const float dir[3], const float vp[3],
-                                                   uint32_t bl) {
-    float tl = INFINITY;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const uint32_t b = (uint32_t)vp[i] >> bl;
-        const float B = (float)(1u << bl);
-        if (dir[i] > 0.0f && b + 1 < a.nbricks[i]) {
-            const float vb = (float)(b + 1) * B - 0.05f;           // voxel coordinate bound
-            const float sb = (vb + 0.5f) / a.fdims[i];               // texture coordinate
-            tl = fminf(tl, (sb - first[i]) / dir[i]);
-        } else if (dir[i] < 0.0f && b > 0) {
-            const float vb = (float)b * B + 0.05f;
-            const float sb = (vb + 0.5f) / a.fdims[i];
-            tl = fminf(tl, (sb - first[i]) / dir[i]);
-        }
-    }
-    return tl;
-}
-
 template <typename T, bool WIDE, int FILTER, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, const uint32_t tiles_x,
                                                          const uint32_t tiles_total,

@@ -15,6 +15,15 @@
 //               oz(z) = (z>>2)*64*NBX*NBY + (z&3)*16
 //             Padding voxels replicate the nearest edge voxel (never read:
 //             every coordinate is clamped to dims-1 before the fetch).
+//   d_cells   corner-replicated cell records, built on first use by the cell
+//             kernel: the record of cell (x, y, z) packs the 8 voxels
+//             (x..x+1, y..y+1, z..z+1), clamped to the volume, in trilinear
+//             order (16 bytes for uint16, 8 for uint8).  Cells are grouped
+//             2x2x2 per group (one 128-byte line for uint16), groups
+//             x-fastest over the dims rounded up to even:
+//               rec(x,y,z) = ((x>>1)*8 + (x&1)) + ((y>>1)*8*GX + (y&1)*2)
+//                            + ((z>>1)*8*GX*GY + (z&1)*4)
+//             8x the voxel bytes; bounded by kMaxCellBytes.
 //   d_minmax  per brick of B^3 voxels (B = 2^brick_log2), the (min, max) raw
 //             voxel value over the brick extended by one voxel on the upper
 //             side of every axis (clamped to the volume): every trilinear or
@@ -50,6 +59,9 @@ struct hipdvr_volume {
     uint32_t nbricks[3] = {0, 0, 0};
     uint64_t brick_count = 0;
     uint64_t tiled_elems = 0;
+    void* d_cells = nullptr;         // cell records (null until first use)
+    uint64_t cell_bytes = 0;
+    bool cells_wide = false;         // more than 2^32 - 256 bytes: 64-bit addressing
 };
 
 namespace hipdvr {
@@ -57,6 +69,13 @@ namespace hipdvr {
 hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volume_desc* desc,
                              const void* voxels, bool on_device);
 void volume_destroy(hipdvr_volume* vol);
+
+// Largest cell-record array the library allocates for one volume.
+constexpr uint64_t kMaxCellBytes = 96ull << 30;
+uint64_t cell_record_bytes(const hipdvr_volume* vol);
+bool cells_allowed(const hipdvr_volume* vol);
+// Build the cell records if they are not there yet (synchronous).
+void ensure_cells(hipdvr_volume* vol, hipStream_t st);
 
 // Per-brick occupancy for a TF mapping: a brick is empty when no TF entry
 // within one index of the range its [min, max] (widened by 1e-5 in

@@ -76,6 +76,45 @@ __global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ src, 
     }
 }
 
+// One lane per record: the 8 footprint voxels of cell (x, y, z), clamped.
+template <typename T>
+__global__ __launch_bounds__(256) void cells_kernel(const T* __restrict__ src, void* __restrict__ dst, uint32_t X,
+                                                    uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
+                                                    uint64_t total) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t in = (uint32_t)(i & 7);
+    const uint64_t grp = i >> 3;
+    const uint32_t x = (uint32_t)(grp % gx) * 2 + (in & 1);
+    const uint64_t r = grp / gx;
+    const uint32_t y = (uint32_t)(r % gy) * 2 + ((in >> 1) & 1);
+    const uint32_t z = (uint32_t)(r / gy) * 2 + (in >> 2);
+    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (x < X && y < Y && z < Z) {
+        const uint32_t x1 = x + 1 < X ? x + 1 : X - 1;
+        const uint32_t y1 = y + 1 < Y ? y + 1 : Y - 1;
+        const uint32_t z1 = z + 1 < Z ? z + 1 : Z - 1;
+        const uint64_t XY = (uint64_t)X * Y;
+        const uint64_t b0 = (uint64_t)z * XY, b1 = (uint64_t)z1 * XY;
+        // trilinear order: llB lrB urB ulB llF lrF urF ulF
+        v[0] = src[b0 + (uint64_t)y * X + x];
+        v[1] = src[b0 + (uint64_t)y * X + x1];
+        v[2] = src[b0 + (uint64_t)y1 * X + x1];
+        v[3] = src[b0 + (uint64_t)y1 * X + x];
+        v[4] = src[b1 + (uint64_t)y * X + x];
+        v[5] = src[b1 + (uint64_t)y * X + x1];
+        v[6] = src[b1 + (uint64_t)y1 * X + x1];
+        v[7] = src[b1 + (uint64_t)y1 * X + x];
+    }
+    if (sizeof(T) == 2) {
+        ((uint4*)dst)[i] = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16),
+                                      v[6] | (v[7] << 16));
+    } else {
+        ((uint2*)dst)[i] = make_uint2(v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
+                                      v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+    }
+}
+
 template <typename T>
 void build_layouts(hipdvr_volume* v, hipStream_t st) {
     const uint32_t X = v->desc.dims[0], Y = v->desc.dims[1], Z = v->desc.dims[2];
@@ -171,8 +210,43 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
     return v;
 }
 
+uint64_t cell_record_bytes(const hipdvr_volume* v) {
+    uint64_t n = 8;
+    for (int i = 0; i < 3; ++i) n *= (v->desc.dims[i] + 1) / 2;
+    return n * (v->desc.dtype == HIPDVR_U8 ? 8 : 16);
+}
+
+bool cells_allowed(const hipdvr_volume* v) { return cell_record_bytes(v) <= kMaxCellBytes; }
+
+void ensure_cells(hipdvr_volume* v, hipStream_t st) {
+    if (v->d_cells) return;
+    const uint64_t bytes = cell_record_bytes(v);
+    if (bytes > kMaxCellBytes) fail(HIPDVR_ERR_UNSUPPORTED, "cell records of this volume exceed the size limit");
+    const uint64_t total = bytes / (v->desc.dtype == HIPDVR_U8 ? 8 : 16);
+    const uint32_t gx = (v->desc.dims[0] + 1) / 2, gy = (v->desc.dims[1] + 1) / 2;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc(cell records)");
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (v->desc.dtype == HIPDVR_U8)
+        hipLaunchKernelGGL(cells_kernel<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)v->d_linear, p,
+                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], gx, gy, total);
+    else
+        hipLaunchKernelGGL(cells_kernel<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)v->d_linear, p,
+                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], gx, gy, total);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(st);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)hipFree(p);
+        hip_check(e1 != hipSuccess ? e1 : e2, "cell record build");
+    }
+    v->d_cells = p;
+    v->cell_bytes = bytes;
+    v->cells_wide = bytes > 0xffffff00ull;
+}
+
 void volume_destroy(hipdvr_volume* v) {
     if (!v) return;
+    if (v->d_cells) (void)hipFree(v->d_cells);
     if (v->d_linear) (void)hipFree(v->d_linear);
     if (v->d_tiled) (void)hipFree(v->d_tiled);
     if (v->d_minmax) (void)hipFree(v->d_minmax);

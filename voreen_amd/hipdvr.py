@@ -18,7 +18,7 @@ CLASS_TF, CLASS_PREINT = 0, 1
 COMPOSITE_DVR = 0
 SHADE_NONE, SHADE_PHONG = 0, 1
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
-KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED = 0, 1, 2
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL = 0, 1, 2, 3
 
 # every symbol include/hipdvr.h and include/hipdvr_synth.h declare
 EXPORTS = (
@@ -61,7 +61,7 @@ class Params(C.Structure):
                 ("light_pos", C.c_float * 3), ("light_ambient", C.c_float * 3),
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
-                ("loop_cap", C.c_int32), ("kernel", C.c_int32)]
+                ("loop_cap", C.c_int32), ("kernel", C.c_int32), ("batch", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -220,7 +220,7 @@ def synth_shells(seed, dims):
 
 
 def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading=SHADE_NONE,
-                   kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5)):
+                   kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5), batch=0):
     """Render parameters with Voreen's VolumeRenderer light defaults
     (src/core/processors/volumerenderer.cpp:41-54)."""
     p = Params()
@@ -241,6 +241,7 @@ def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading
     p.camera_pos[:] = list(camera_pos)
     p.loop_cap = 0
     p.kernel = kernel
+    p.batch = batch
     return p
 
 

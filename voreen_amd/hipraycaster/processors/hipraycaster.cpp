@@ -79,6 +79,7 @@ HIPRaycaster::HIPRaycaster()
     hipKernel_.addOption("auto", "Fastest exact kernel");
     hipKernel_.addOption("reference", "Reference layout, no skipping");
     hipKernel_.addOption("bricked", "Bricked, empty-space skipping");
+    hipKernel_.addOption("cell", "Cell records, empty-space skipping");
     hipKernel_.selectByKey("auto");
     addProperty(hipKernel_);
 
@@ -221,7 +222,9 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
     const std::string k = hipKernel_.getKey();
     p.kernel = k == "reference" ? HIPDVR_KERNEL_REFERENCE
              : k == "bricked"   ? HIPDVR_KERNEL_BRICKED
+             : k == "cell"      ? HIPDVR_KERNEL_CELL
                                 : HIPDVR_KERNEL_AUTO;
+    p.batch = 0;
 }
 
 // Output exactly as CPURaycaster::process (cpuraycaster.cpp:157-159,197-207).
