@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c1", choices=["c0", "c1"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell", "cellpipe"])
     ap.add_argument("--batch", type=int, default=0, choices=[0, 2, 4, 8],
                     help="cell kernel: samples fetched together per ray (0 = library default)")
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
@@ -92,7 +92,8 @@ def main():
     wimg, himg = cfg["image"]
     nviews = args.views or cfg["views"]
     kernel = {"auto": H.KERNEL_AUTO, "reference": H.KERNEL_REFERENCE,
-              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL}[args.kernel]
+              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL,
+              "cellpipe": H.KERNEL_CELL_PIPE}[args.kernel]
     t_setup = time.perf_counter()
     vol = W.make_volume(args.config)
     keys, tf = W.make_tf(args.config)

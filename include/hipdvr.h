@@ -83,8 +83,18 @@ enum {
     HIPDVR_CLASS_PREINT = 1     /* "pre-integrated-cpu" / "-approximation" (table given) */
 };
 
-/* compositing option keys (volumeraycaster.cpp:379-388) */
-enum { HIPDVR_COMPOSITE_DVR = 0 };
+/* compositing option keys (volumeraycaster.cpp:379-388); the modes other
+ * than DVR follow modules/core/glsl/modules/mod_compositing.frag:85-213 and
+ * need transfer-function classification */
+enum {
+    HIPDVR_COMPOSITE_DVR = 0,   /* "dvr" */
+    HIPDVR_COMPOSITE_MIP = 1,   /* "mip" */
+    HIPDVR_COMPOSITE_MOP = 2,   /* "mop" */
+    HIPDVR_COMPOSITE_MIDA = 3,  /* "mida" (gamma) */
+    HIPDVR_COMPOSITE_ISO = 4,   /* "iso" (iso_value) */
+    HIPDVR_COMPOSITE_FHP = 5,   /* "fhp" first-hit position */
+    HIPDVR_COMPOSITE_FHN = 6    /* "fhn" first-hit normal */
+};
 
 /* shading option keys (src/core/utils/glsl.cpp:397-409) */
 enum { HIPDVR_SHADE_NONE = 0, HIPDVR_SHADE_PHONG = 1 };
@@ -97,8 +107,10 @@ enum {
     HIPDVR_KERNEL_AUTO = 0,      /* fastest exact kernel for the options */
     HIPDVR_KERNEL_REFERENCE = 1, /* linear layout, one lane per ray, no skipping */
     HIPDVR_KERNEL_BRICKED = 2,   /* bricked layout + exact empty-space skipping */
-    HIPDVR_KERNEL_CELL = 3       /* corner-replicated cell records + exact empty-space
+    HIPDVR_KERNEL_CELL = 3,      /* corner-replicated cell records + exact empty-space
                                     skipping; linear filter, TF classification */
+    HIPDVR_KERNEL_CELL_PIPE = 4  /* the cell kernel with software-pipelined fetches
+                                    and LDS occupancy bits */
 };
 
 typedef struct hipdvr_ctx hipdvr_ctx;
@@ -133,6 +145,8 @@ typedef struct hipdvr_params {
     int32_t kernel;         /* HIPDVR_KERNEL_* */
     int32_t batch;          /* cell kernel: samples fetched together per ray (2, 4, 8;
                                0 = default) */
+    float gamma;            /* MIDA gammaValue in [-1, 1] (volumeraycaster.cpp:67) */
+    float iso_value;        /* ISO isoValue in [0, 1] (volumeraycaster.cpp:57) */
 } hipdvr_params;
 
 typedef struct hipdvr_stats {

@@ -27,6 +27,17 @@
  *                       misses are exactly zero; brute-force ray marching.
  *   Phong               single-term checks (ambient only, N parallel to L) and
  *                       the gradient of an affine field.
+ *   compositing modes   MIP on a linear ramp along the ray = the colour of the
+ *                       TF entry of the largest intensity; MOP = the entry of the
+ *                       largest TF alpha; ISO on a ramp = the first sample whose
+ *                       alpha is within 0.02 of the iso value; FHP = the entry
+ *                       point of a ray entering an opaque volume (t = 0 sample);
+ *                       FHN of an affine field = its normalized down-gradient
+ *                       mapped to [0,1]; MIDA with gamma = 1 equals MIP with the
+ *                       corrected alpha and with a constant volume the
+ *                       closed-form recurrence (delta = voxel at every sample).
+ *                       A zero FHN gradient gives (0.5, 0.5, 0.5) where GLSL's
+ *                       normalize is undefined: parity unpinned there.
  */
 #include "dvr_oracle.h"
 
@@ -540,6 +551,119 @@ static inline void tex_to_phys_f(const dvr_volume* v, const float s[3], float ou
         out[i] = (((s[i] * (float)v->dims[i]) - 0.5f) * v->spacing[i]) + v->offset[i];
 }
 
+/* compositeMIDAhelper, mod_compositing.frag:112-129: maxIntensity is read,
+ * never written, here. */
+static void mida_helper(float res[4], float voxel, const float col[4], float max_int, float gamma) {
+    float delta = 0.0f;
+    if (voxel > max_int) delta = voxel - max_int;
+    const float beta = 1.0f - delta * (1.0f + gamma);
+    const float k = (1.0f - beta * res[3]) * col[3];
+    for (int i = 0; i < 3; ++i) res[i] = beta * res[i] + k * col[i];
+    res[3] = beta * res[3] + (1.0f - beta * res[3]) * col[3];
+}
+
+/* Compositing modes other than DVR (TF classification): the GLSL compositing
+ * functions of mod_compositing.frag:85-213 applied where the TF alpha is > 0
+ * (rc_singlevolume.frag:222-236), shading first, on the CPURaycaster sample
+ * sequence; early termination for ISO/FHP/FHN (volumeraycaster.cpp:225-233,
+ * mod_raysetup.frag:56-63). */
+static int64_t march_modes_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* P,
+                             const float first[3], const float dir[3], float tend,
+                             const float* entry, int w, int h, int px, int py, float res[4]) {
+    const float step = P->step;
+    const int W1 = tf->width - 1;
+    const int mode = P->compositing;
+    const int ert = mode == DVR_COMP_ISO || mode == DVR_COMP_FHP || mode == DVR_COMP_FHN;
+    float out[4] = {0.f, 0.f, 0.f, 0.f};
+    float max_int = 0.0f;       /* rc_singlevolume.frag:121 */
+    float t = 0.0f;
+    int finished = 0;
+    int64_t n = 0;
+    const int cap = P->loop_cap > 0 ? P->loop_cap : 255 * 255;
+    for (int loop = 0; !finished && loop < cap; ++loop) {
+        ++n;
+        float s[3] = { first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2] };
+        const float inten = reconstruct(v, P->filter, s);
+        const float rw = (inten * v->rwm_scale) + v->rwm_offset;
+        const float x = rw_to_norm(rw, tf->domain_lo, tf->domain_hi);
+        int idx = iroundf_tgt(x * (float)W1);
+        idx = idx < 0 ? 0 : (idx > W1 ? W1 : idx);
+        const uint8_t* c8 = tf->rgba8 + 4 * (size_t)idx;
+        float col[4] = {(float)c8[0] / 255.f, (float)c8[1] / 255.f, (float)c8[2] / 255.f,
+                        (float)c8[3] / 255.f};
+        if (col[3] > 0.0f) {
+            float gr[3] = {0.f, 0.f, 0.f};
+            const int need_grad = P->shading == DVR_SHADE_PHONG || mode == DVR_COMP_FHN;
+            if (need_grad) {
+                if (t == 0.0f) clip_border_gradient(entry, w, h, px, py, s, dir, gr);
+                else gradient_f(v, P->filter, s, gr);
+            }
+            if (P->shading == DVR_SHADE_PHONG) {
+                float pos[3], c[3] = {col[0], col[1], col[2]}, sh[3];
+                tex_to_phys_f(v, s, pos);
+                phong_f(P, gr, pos, c, sh);
+                col[0] = sh[0]; col[1] = sh[1]; col[2] = sh[2];
+            }
+            switch (mode) {
+            case DVR_COMP_MIP:      /* :85-96 */
+                if (rw > max_int) {
+                    memcpy(out, col, sizeof out);
+                    max_int = rw;
+                }
+                break;
+            case DVR_COMP_MOP:      /* :98-107 */
+                if (col[3] > out[3]) memcpy(out, col, sizeof out);
+                break;
+            case DVR_COMP_MIDA: {   /* :136-152 */
+                col[3] = dvr_opacity_correct(col[3], step);
+                if (P->gamma <= 0.0f) {
+                    mida_helper(out, rw, col, max_int, P->gamma);
+                } else {
+                    float mip[4], mida[4];
+                    memcpy(mip, out, sizeof mip);
+                    memcpy(mida, out, sizeof mida);
+                    if (rw > max_int) {
+                        memcpy(mip, col, sizeof mip);
+                        max_int = rw;
+                    }
+                    mida_helper(mida, rw, col, max_int, 0.0f);
+                    for (int i = 0; i < 4; ++i) out[i] = P->gamma * mip[i] + (1.0f - P->gamma) * mida[i];
+                }
+                break;
+            }
+            case DVR_COMP_ISO:      /* :159-170 */
+                if (col[3] >= P->iso_value - 0.02f && col[3] <= P->iso_value + 0.02f) {
+                    out[0] = col[0]; out[1] = col[1]; out[2] = col[2]; out[3] = 1.0f;
+                }
+                break;
+            case DVR_COMP_FHP:      /* :175-186 */
+                if (out[0] == 0.f && out[1] == 0.f && out[2] == 0.f) {
+                    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = 1.0f;
+                }
+                break;
+            case DVR_COMP_FHN:      /* :191-202 */
+                if (out[0] == 0.f && out[1] == 0.f && out[2] == 0.f) {
+                    const float l2 = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+                    const float rl = l2 > 0.f ? 1.0f / sqrtf(l2) : 0.f;
+                    for (int i = 0; i < 3; ++i) out[i] = ((gr[i] * rl) / 2.0f) + 0.5f;
+                    out[3] = 1.0f;
+                }
+                break;
+            default:
+                break;
+            }
+        }
+        if (ert && out[3] >= ERT_OPACITY) {
+            out[3] = 1.0f;
+            finished = 1;
+        }
+        t += step;
+        finished = finished || (t > tend);
+    }
+    memcpy(res, out, 4 * sizeof(float));
+    return n;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Ray march, FP32: CPURaycaster::directRendering, cpuraycaster.cpp:210-326   */
 /* ------------------------------------------------------------------------ */
@@ -571,6 +695,8 @@ static int64_t march_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
     int finished = 0;
     int64_t n = 0;
     const int cap = P->loop_cap > 0 ? P->loop_cap : 255 * 255;
+    if (P->compositing != DVR_COMP_DVR)
+        return march_modes_f(v, tf, P, first, dir, tend, entry, w, h, px, py, res);
     for (int loop = 0; !finished && loop < cap; ++loop) {
         ++n;
         float s[3] = { first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2] };
@@ -644,6 +770,13 @@ static int64_t march_d(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
         tend = sqrtf(lsq);
         float rl = 1.0f / sqrtf(lsq);
         dir[0] *= rl; dir[1] *= rl; dir[2] *= rl;
+    }
+    if (P->compositing != DVR_COMP_DVR) {
+        /* the FP64 envelope is defined for DVR only; other modes are the FP32 result */
+        float rf[4];
+        const int64_t k = march_modes_f(v, tf, P, first, dir, tend, entry, w, h, px, py, rf);
+        for (int i = 0; i < 4; ++i) res[i] = rf[i];
+        return k;
     }
     const int W1 = tf->width - 1;
     const double e = (double)step * 200.0;

@@ -24,6 +24,12 @@
  *   Phong (shading=1)     modules/core/glsl/modules/mod_gradients.frag:39-46,202-214,
  *                         mod_shading.frag:134-151, mod_sampler3d.frag:179-181,
  *                         modules/base/glsl/rc_singlevolume.frag:222-235
+ *   compositing modes     MIP/MOP/MIDA/ISO/FHP/FHN of
+ *                         modules/core/glsl/modules/mod_compositing.frag:85-213 as
+ *                         invoked by rc_singlevolume.frag:200-266 (composite only
+ *                         where the TF alpha is > 0; early termination only for
+ *                         DVR/ISO/FHP/FHN, src/core/processors/volumeraycaster.cpp:
+ *                         225-233), on the CPURaycaster sample sequence (FP32 only)
  *
  * Parity pinning of every function is listed in the header of dvr_oracle.c.
  */
@@ -40,6 +46,9 @@ enum { DVR_U8 = 0, DVR_U16 = 1 };
 enum { DVR_FILTER_NEAREST = 0, DVR_FILTER_LINEAR = 1 };
 enum { DVR_CLASS_TF = 0, DVR_CLASS_PREINT = 1 };
 enum { DVR_SHADE_NONE = 0, DVR_SHADE_PHONG = 1 };
+/* compositing modes, mod_compositing.frag:47-230 / volumeraycaster.cpp:250-271 */
+enum { DVR_COMP_DVR = 0, DVR_COMP_MIP = 1, DVR_COMP_MOP = 2, DVR_COMP_MIDA = 3, DVR_COMP_ISO = 4,
+       DVR_COMP_FHP = 5, DVR_COMP_FHN = 6 };
 /* TransFuncBase::AlphaMode (include/voreen/core/datastructures/transfunc/transfuncbase.h:165-169) */
 enum { DVR_ALPHA_ZERO = 0, DVR_ALPHA_USE = 1, DVR_ALPHA_ONE = 2 };
 
@@ -74,6 +83,9 @@ typedef struct {
     float shininess;
     float camera_pos[3];    /* physical space */
     int32_t loop_cap;       /* 255*255 in the reference */
+    int32_t compositing;    /* DVR_COMP_*; modes other than DVR need TF classification */
+    float gamma;            /* MIDA gammaValue, [-1, 1] (volumeraycaster.cpp:67) */
+    float iso_value;        /* ISO isoValue (volumeraycaster.cpp:57) */
 } dvr_params;
 
 typedef struct {

@@ -22,6 +22,7 @@ U8, U16 = 0, 1
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 CLASS_TF, CLASS_PREINT = 0, 1
 SHADE_NONE, SHADE_PHONG = 0, 1
+COMP_DVR, COMP_MIP, COMP_MOP, COMP_MIDA, COMP_ISO, COMP_FHP, COMP_FHN = range(7)
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
 
 
@@ -42,7 +43,8 @@ class Params(C.Structure):
                 ("light_pos", C.c_float * 3), ("light_ambient", C.c_float * 3),
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
-                ("loop_cap", C.c_int32)]
+                ("loop_cap", C.c_int32), ("compositing", C.c_int32), ("gamma", C.c_float),
+                ("iso_value", C.c_float)]
 
 
 class TFKey(C.Structure):
@@ -169,8 +171,12 @@ def _volume(vol, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0)
     return v, vol
 
 
-def _params(step, filter_, classification, preint, shading, light):
+def _params(step, filter_, classification, preint, shading, light, compositing=0, gamma=0.0,
+            iso_value=0.5):
     p = Params()
+    p.compositing = compositing
+    p.gamma = gamma
+    p.iso_value = iso_value
     p.step = step
     p.filter = filter_
     p.classification = classification
@@ -195,7 +201,7 @@ def _params(step, filter_, classification, preint, shading, light):
 def render(vol, tf_rgba8, domain, entry, exit_, step, filter_=FILTER_LINEAR,
            classification=CLASS_TF, preint=None, shading=SHADE_NONE, light=None,
            rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0),
-           fp64=False, nthreads=0):
+           fp64=False, nthreads=0, compositing=COMP_DVR, gamma=0.0, iso_value=0.5):
     """Render one frame; vol is a (Z, Y, X) uint8/uint16 array, entry/exit
     (H, W, 4) float32.  Returns (rgba (H, W, 4), samples)."""
     v, vol_keep = _volume(vol, rwm, spacing, offset)
@@ -206,7 +212,7 @@ def render(vol, tf_rgba8, domain, entry, exit_, step, filter_=FILTER_LINEAR,
     t.domain_lo, t.domain_hi = domain
     if preint is not None:
         preint = np.ascontiguousarray(preint, np.float32)
-    p = _params(step, filter_, classification, preint, shading, light)
+    p = _params(step, filter_, classification, preint, shading, light, compositing, gamma, iso_value)
     entry = np.ascontiguousarray(entry, np.float32)
     exit_ = np.ascontiguousarray(exit_, np.float32)
     h, w = entry.shape[:2]

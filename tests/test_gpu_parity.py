@@ -41,7 +41,8 @@ def say(msg):
 
 def render_both(ctx, oracle, vol, tf, domain, en, ex, step, kernel=None, rwm=(1.0, 0.0),
                 spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0), filter_=1, shading=0,
-                classification=0, preint=None, camera_pos=(0.0, 0.0, 3.5), fp64=False):
+                classification=0, preint=None, camera_pos=(0.0, 0.0, 3.5), fp64=False,
+                compositing=0, gamma=0.0, iso_value=0.5):
     from voreen_amd import hipdvr as H
     kernel = H.KERNEL_REFERENCE if kernel is None else kernel
     v = ctx.volume(vol, rwm=rwm, spacing=spacing, offset=offset)
@@ -49,7 +50,8 @@ def render_both(ctx, oracle, vol, tf, domain, en, ex, step, kernel=None, rwm=(1.
     if preint is not None:
         t.set_preint(preint)
     p = H.default_params(step, filter_=filter_, classification=classification, shading=shading,
-                         kernel=kernel, camera_pos=camera_pos)
+                         kernel=kernel, camera_pos=camera_pos, compositing=compositing, gamma=gamma,
+                         iso_value=iso_value)
     try:
         gpu, st = ctx.render(v, t, en, ex, p)
     finally:
@@ -58,7 +60,8 @@ def render_both(ctx, oracle, vol, tf, domain, en, ex, step, kernel=None, rwm=(1.
     light = dict(camera=camera_pos)
     cpu, n = oracle.render(vol, tf, domain, en, ex, step, filter_=filter_,
                            classification=classification, preint=preint, shading=shading,
-                           light=light, rwm=rwm, spacing=spacing, offset=offset)
+                           light=light, rwm=rwm, spacing=spacing, offset=offset,
+                           compositing=compositing, gamma=gamma, iso_value=iso_value)
     out = dict(gpu=gpu, cpu=cpu, n=n, stats=st)
     if fp64:
         out["fp64"], _ = oracle.render(vol, tf, domain, en, ex, step, filter_=filter_,
@@ -251,6 +254,56 @@ def test_edge_cases(ctx, oracle, kernel):
         assert report(f"tf2 {dom}", r, oracle).max() <= TOL
 
 
+MODES = [("mip", 1, 0.0), ("mop", 2, 0.0), ("mida", 3, 0.0), ("mida", 3, -0.5), ("mida", 3, 0.6),
+         ("iso", 4, 0.0), ("fhp", 5, 0.0), ("fhn", 6, 0.0)]
+
+
+@pytest.mark.parametrize("mode,code,gamma", MODES, ids=[f"{m}{g}" for m, _, g in MODES])
+def test_compositing_modes(ctx, oracle, mode, code, gamma):
+    """MIP/MOP/MIDA/ISO/FHP/FHN (mod_compositing.frag) against the oracle on the
+    blob volume and the walnut fixture, both filters, with and without Phong."""
+    from voreen_amd import workloads as W
+    vol, tf, en, ex, step = c0_inputs(oracle, size=96, view=7, nviews=36)
+    for filt in (0, 1):
+        for shading in (0, 1):
+            say(f"compositing {mode} gamma {gamma} filter {filt} shading {shading}")
+            r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, filter_=filt, shading=shading,
+                            compositing=code, gamma=gamma, iso_value=0.3)
+            d = report(f"{mode} f{filt} s{shading}", r, oracle).max(axis=2)
+            assert np.mean(d > TOL) < 0.002 if shading else d.max() <= TOL
+            assert r["stats"]["samples"] == r["n"]
+    v = F.load_vvd("walnut")
+    t = F.load_tfi("walnut")
+    wtf = oracle.tf_build(t["keys"], t["width"], t["gamma"], t["threshold"], t["alpha_mode"])
+    llf, urb = W.world_box(v["dims"], v["spacing"])
+    cam = W.orbit_camera(5, 24, radius=2.5)
+    wen, wex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 96, 72, llf=llf, urb=urb)
+    say(f"compositing {mode} walnut")
+    r = render_both(ctx, oracle, v["data"], wtf, t["domain"], wen, wex, W.step_for(v["dims"], 2.0),
+                    rwm=(65535.0, 0.0), compositing=code, gamma=gamma, iso_value=0.3)
+    assert report(f"{mode} walnut", r, oracle).max() <= TOL
+    assert np.mean(r["gpu"][..., 3] > 0) > 0.02
+
+
+def test_compositing_modes_reject_skipping_kernels(ctx):
+    from voreen_amd import hipdvr as H
+    v = ctx.volume(np.zeros((4, 4, 4), np.uint8))
+    t = ctx.transfunc(np.zeros((256, 4), np.uint8))
+    en = np.zeros((2, 2, 4), np.float32)
+    say("compositing modes: invalid combinations")
+    try:
+        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL):
+            with pytest.raises(H.HipDvrError) as e:
+                ctx.render(v, t, en, en, H.default_params(0.1, kernel=k, compositing=H.COMPOSITE_MIP))
+            assert e.value.code == H.ERR_UNSUPPORTED
+        with pytest.raises(H.HipDvrError) as e:
+            ctx.render(v, t, en, en, H.default_params(0.1, compositing=9))
+        assert e.value.code == H.ERR_UNSUPPORTED
+    finally:
+        v.close()
+        t.close()
+
+
 def test_invalid_arguments_return_codes(ctx):
     from voreen_amd import hipdvr as H
     vol = np.zeros((4, 4, 4), np.uint8)
@@ -266,7 +319,7 @@ def test_invalid_arguments_return_codes(ctx):
             ctx.render(v, t, en, en, H.default_params(0.1, classification=H.CLASS_PREINT))
         assert e.value.code == H.ERR_INVALID
         p = H.default_params(0.1)
-        p.compositing = 5
+        p.compositing = 42
         with pytest.raises(H.HipDvrError) as e:
             ctx.render(v, t, en, en, p)
         assert e.value.code == H.ERR_UNSUPPORTED
@@ -301,6 +354,7 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             outs = {}
             runs = [("ref", H.KERNEL_REFERENCE, 0), ("brick", H.KERNEL_BRICKED, 0)]
             runs += [(f"cell{b}", H.KERNEL_CELL, b) for b in (2, 4, 8)]
+            runs += [(f"cellpipe{b}", H.KERNEL_CELL_PIPE, b) for b in (2, 4)]
             for name, k, batch in runs:
                 for shading in (0, 1):
                     say(f"bricked-vs-reference tf{seed} {name} shading {shading}")
@@ -310,9 +364,9 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             t.close()
             for shading in (0, 1):
                 a, sa = outs[("ref", shading)]
-                for b in (2, 4, 8):
-                    c, sc = outs[(f"cell{b}", shading)]
-                    assert np.array_equal(a, c), f"cell batch {b} tf{seed} shading {shading}"
+                for name in ("cell2", "cell4", "cell8", "cellpipe2", "cellpipe4"):
+                    c, sc = outs[(name, shading)]
+                    assert np.array_equal(a, c), f"{name} tf{seed} shading {shading}"
                     assert sa["samples"] == sc["samples"] and sa["shaded"] == sc["shaded"]
                     assert sc["fetched"] < sa["fetched"] or seed == 1
                 b, sb = outs[("brick", shading)]

@@ -184,3 +184,22 @@ def test_process_phong_and_preintegration(rc, oracle):
     table = oracle.preint_build(keys, 256, step)
     ref, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, classification=1, preint=table)
     assert np.abs(out - ref).max() <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,code", [("mip", 1), ("mida", 3), ("iso", 4), ("fhn", 6)])
+def test_process_compositing_modes(rc, oracle, mode, code):
+    vol, keys, en, ex, cam = _c0_case(oracle, size=96)
+    print(f"[case] HIPRaycaster.process compositing {mode}", flush=True)
+    rc.set_volume(vol)
+    rc.set_tf(keys)
+    rc.set_images(en, ex)
+    rc.set("compositing", mode)
+    rc.set("gammaValue", "0.25")
+    rc.set("isoValue", "0.3")
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    tf = oracle.tf_build(keys, 256)
+    ref, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, rc.step(), compositing=code, gamma=0.25,
+                           iso_value=0.3)
+    assert np.abs(out - ref).max() <= 1e-3

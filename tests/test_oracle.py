@@ -376,3 +376,116 @@ def test_bundled_tfs_build(oracle):
         for c in range(4):
             ref = np.interp(x, [k[0] for k in keys], [k[1][c] for k in keys])
             assert np.all(np.abs(tf[:, c].astype(float) - ref) <= 1.0 + 1e-6)
+
+
+# --------------------------------------------------------------------------
+# compositing modes (mod_compositing.frag:85-213, rc_singlevolume.frag:200-266)
+# --------------------------------------------------------------------------
+
+def _slab_volume(values, width=8):
+    """Volume of len(values) slabs along x, each `width` voxels of one value."""
+    nx = width * len(values)
+    vol = np.zeros((4, 4, nx), np.uint8)
+    for i, v in enumerate(values):
+        vol[:, :, i * width:(i + 1) * width] = v
+    return vol
+
+
+def test_mip_returns_the_colour_of_the_largest_intensity(oracle):
+    vol = _slab_volume([40, 200, 90, 150])
+    tf = np.zeros((256, 4), np.uint8)
+    for v, c in ((40, (255, 0, 0, 20)), (200, (0, 255, 0, 10)), (90, (0, 0, 255, 250)),
+                 (150, (9, 9, 9, 200))):
+        tf[v] = c
+    en, ex = _unit_ray_x()
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 256, filter_=oracle.FILTER_NEAREST,
+                           compositing=oracle.COMP_MIP)
+    assert np.array_equal(img[0, 0], np.array([0, 255, 0, 10], np.float32) / np.float32(255))
+    assert n == 257                                         # no early termination for MIP
+
+
+def test_mop_returns_the_colour_of_the_largest_alpha(oracle):
+    vol = _slab_volume([40, 200, 90, 150])
+    tf = np.zeros((256, 4), np.uint8)
+    for v, c in ((40, (255, 0, 0, 20)), (200, (0, 255, 0, 10)), (90, (0, 0, 255, 250)),
+                 (150, (9, 9, 9, 200))):
+        tf[v] = c
+    en, ex = _unit_ray_x()
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 256, filter_=oracle.FILTER_NEAREST,
+                           compositing=oracle.COMP_MOP)
+    assert np.array_equal(img[0, 0], np.array([0, 0, 255, 250], np.float32) / np.float32(255))
+
+
+def test_iso_stops_at_the_first_alpha_within_the_band(oracle):
+    # alphas 0.2, 0.51, 0.9 along the ray; iso 0.5 +- 0.02 matches the second slab
+    vol = _slab_volume([40, 200, 90])
+    tf = np.zeros((256, 4), np.uint8)
+    tf[40] = (1, 2, 3, 51)
+    tf[200] = (100, 110, 120, 130)
+    tf[90] = (5, 5, 5, 230)
+    en, ex = _unit_ray_x()
+    step = 1 / 240
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, step, filter_=oracle.FILTER_NEAREST,
+                           compositing=oracle.COMP_ISO, iso_value=0.5)
+    assert np.array_equal(img[0, 0, :3], np.array([100, 110, 120], np.float32) / np.float32(255))
+    assert img[0, 0, 3] == 1.0
+    # the first sample in slab 2 is the first whose nearest voxel x >= 8: u >= 7.5
+    t_hit = (7.5 + 0.5) / 24
+    assert abs((n - 1) - math.ceil(t_hit / step)) <= 1
+
+
+def test_fhp_is_the_entry_point_of_an_opaque_volume(oracle):
+    vol = np.full((8, 8, 8), 200, np.uint8)
+    tf = _const_tf(256, (1, 1, 1, 1))
+    en = np.array([[[0.25, 0.5, 0.75, 1.0]]], np.float32)
+    ex = np.array([[[0.9, 0.4, 0.1, 1.0]]], np.float32)
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 64, compositing=oracle.COMP_FHP)
+    assert np.array_equal(img[0, 0], np.array([0.25, 0.5, 0.75, 1.0], np.float32)) and n == 1
+
+
+def test_fhn_of_an_affine_field_is_its_down_gradient(oracle):
+    # v = 4x + 2y + z (voxel units); TF transparent below index 60 so the first
+    # hit is inside (t > 0, central differences away from the faces)
+    nz = ny = nx = 32
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vol = (4 * x + 2 * y + z).astype(np.uint8)
+    tf = np.zeros((256, 4), np.uint8)
+    tf[60:] = (1, 1, 1, 255)
+    en, ex = _unit_ray_x(y=0.5, z=0.5)
+    img, n = oracle.render(vol, tf, (0, 1), en, ex, 1 / 128, compositing=oracle.COMP_FHN)
+    g = -np.array([4.0, 2.0, 1.0]) / math.sqrt(21.0)
+    assert np.allclose(img[0, 0, :3], g / 2 + 0.5, atol=1e-6) and img[0, 0, 3] == 1.0
+
+
+def test_mida_gamma_one_is_mip_with_corrected_alpha(oracle):
+    vol = _slab_volume([40, 200, 90, 150])
+    tf = np.zeros((256, 4), np.uint8)
+    tf[:] = (7, 8, 9, 30)
+    tf[200] = (0, 255, 0, 10)
+    en, ex = _unit_ray_x()
+    step = 1 / 256
+    kw = dict(filter_=oracle.FILTER_NEAREST)
+    mip, _ = oracle.render(vol, tf, (0, 1), en, ex, step, compositing=oracle.COMP_MIP, **kw)
+    mida, _ = oracle.render(vol, tf, (0, 1), en, ex, step, compositing=oracle.COMP_MIDA, gamma=1.0, **kw)
+    assert np.array_equal(mida[0, 0, :3], mip[0, 0, :3])
+    assert mida[0, 0, 3] == oracle.opacity_correct(np.float32(10 / 255), step)
+
+
+def test_mida_gamma_le_zero_converges_to_its_fixed_point(oracle):
+    # constant volume: delta = voxel at every sample (maxIntensity is never
+    # raised for gamma <= 0), beta = 1 - c(1 + gamma); the recurrence
+    # A' = beta*A + (1 - beta*A)*a has the fixed point A* = a / (1 - beta(1 - a)),
+    # R* = (1 - beta*A*) * a * c_rgb / (1 - beta)
+    c8 = 128
+    vol = np.full((4, 4, 64), c8, np.uint8)
+    tf = _const_tf(256, (200, 100, 50, 90))
+    en, ex = _unit_ray_x()
+    step = 1 / 128
+    for gamma in (0.0, -0.5):
+        img, n = oracle.render(vol, tf, (0, 1), en, ex, step, compositing=oracle.COMP_MIDA, gamma=gamma)
+        c = c8 / 255.0
+        beta = 1 - c * (1 + gamma)
+        a = 1 - (1 - 90 / 255.0) ** (step * 200)
+        A = a / (1 - beta * (1 - a))
+        R = (1 - beta * A) * a * np.array([200, 100, 50]) / 255.0 / (1 - beta)
+        assert abs(img[0, 0, 3] - A) < 1e-5 and np.allclose(img[0, 0, :3], R, atol=1e-5)

@@ -68,6 +68,7 @@ void ensure_occupancy(hipdvr_ctx* ctx, hipdvr_volume* vol, const hipdvr_tf* tf) 
     o.dom_hi = tf->domain_hi;
     o.tf_width = (int32_t)tf->width;
     o.alpha_prefix = tf->d_alpha_prefix;
+    o.bits = vol->d_occ_bits;
     launch_occupancy(o, ctx->stream);
     hip_check(hipGetLastError(), "occupancy kernel launch");
     std::memcpy(vol->occ_key, key, sizeof key);
@@ -92,11 +93,15 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported classification mode");
     if (p->classification == HIPDVR_CLASS_PREINT && (!tf->d_preint || tf->preint_n < 2))
         fail(HIPDVR_ERR_INVALID, "pre-integrated classification needs a table (hipdvr_tf_set_preint)");
-    if (p->compositing != HIPDVR_COMPOSITE_DVR)
-        fail(HIPDVR_ERR_UNSUPPORTED, "only DVR compositing is implemented");
+    if (p->compositing < HIPDVR_COMPOSITE_DVR || p->compositing > HIPDVR_COMPOSITE_FHN)
+        fail(HIPDVR_ERR_UNSUPPORTED, "unknown compositing mode");
+    if (p->compositing != HIPDVR_COMPOSITE_DVR && p->classification != HIPDVR_CLASS_TF)
+        fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR need transfer-function classification");
+    if (!std::isfinite(p->gamma) || !std::isfinite(p->iso_value))
+        fail(HIPDVR_ERR_INVALID, "gamma and iso_value must be finite");
     if (p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
-    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
+    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL_PIPE)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
     if (p->batch != 0 && p->batch != 2 && p->batch != 4 && p->batch != 8)
         fail(HIPDVR_ERR_INVALID, "batch must be 0, 2, 4 or 8");
@@ -110,6 +115,8 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.vox = vol->d_linear;
     a.bricks = vol->d_tiled;
     a.brick_occ = vol->d_occ;
+    a.occ_bits = vol->d_occ_bits;
+    a.occ_words = vol->occ_words;
     a.brick_log2 = v.brick_log2;
     for (int i = 0; i < 3; ++i) {
         a.dims[i] = v.dims[i];
@@ -149,6 +156,9 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.pix_offset = 0;
     a.cond_flag = nullptr;
     a.counters = counters;
+    a.compositing = p->compositing;
+    a.gamma = p->gamma;
+    a.iso_value = p->iso_value;
     fill_alpha_table(p->step, a.alpha_corr);
 }
 
@@ -173,28 +183,37 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     // the skipping kernels drop samples, which the pre-integrated mode cannot
     // (its lookup pairs every sample with the previous one); the cell records
     // serve the linear filter
+    const bool modes = p->compositing != HIPDVR_COMPOSITE_DVR;
+    if (modes && kernel != HIPDVR_KERNEL_AUTO && kernel != HIPDVR_KERNEL_REFERENCE)
+        fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR run on the reference kernel");
     if (kernel == HIPDVR_KERNEL_AUTO) {
-        if (p->classification != HIPDVR_CLASS_TF) kernel = HIPDVR_KERNEL_REFERENCE;
+        if (p->classification != HIPDVR_CLASS_TF || modes) kernel = HIPDVR_KERNEL_REFERENCE;
         else if (p->filter == HIPDVR_FILTER_LINEAR && cells_allowed(vol)) kernel = HIPDVR_KERNEL_CELL;
         else kernel = HIPDVR_KERNEL_BRICKED;
     }
-    if ((kernel == HIPDVR_KERNEL_BRICKED || kernel == HIPDVR_KERNEL_CELL) && p->classification != HIPDVR_CLASS_TF)
+    const bool cell = kernel == HIPDVR_KERNEL_CELL || kernel == HIPDVR_KERNEL_CELL_PIPE;
+    if ((kernel == HIPDVR_KERNEL_BRICKED || cell) && p->classification != HIPDVR_CLASS_TF)
         fail(HIPDVR_ERR_UNSUPPORTED, "the skipping kernels support transfer-function classification only");
-    if (kernel == HIPDVR_KERNEL_CELL && p->filter != HIPDVR_FILTER_LINEAR)
+    if (cell && p->filter != HIPDVR_FILTER_LINEAR)
         fail(HIPDVR_ERR_UNSUPPORTED, "the cell kernel supports the linear filter only");
     hipdvr_volume* mvol = const_cast<hipdvr_volume*>(vol);
-    if (kernel == HIPDVR_KERNEL_BRICKED || kernel == HIPDVR_KERNEL_CELL) ensure_occupancy(ctx, mvol, tf);
-    if (kernel == HIPDVR_KERNEL_CELL) {
+    if (kernel == HIPDVR_KERNEL_BRICKED || cell) ensure_occupancy(ctx, mvol, tf);
+    if (cell) {
         ensure_cells(mvol, ctx->stream);
         a.cells = mvol->d_cells;
     }
     if (count) hip_check(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     const bool timed = ctx_timing_start(ctx);
-    if (kernel == HIPDVR_KERNEL_REFERENCE) {
+    if (modes) {
+        launch_march_modes(a, vol->desc.dtype, p->filter, phong, ctx->stream);
+    } else if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else if (kernel == HIPDVR_KERNEL_CELL) {
         launch_march_cell(a, vol->desc.dtype, phong, count, mvol->cells_wide, p->batch ? p->batch : kDefaultBatch,
                           ctx->stream);
+    } else if (kernel == HIPDVR_KERNEL_CELL_PIPE) {
+        launch_march_cell_pipe(a, vol->desc.dtype, phong, count, mvol->cells_wide,
+                               p->batch ? p->batch : kDefaultBatch, ctx->stream);
     } else {
         // buffer descriptors address at most 2^32 - 1 bytes
         const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;

@@ -18,6 +18,9 @@ constexpr float kErtOpacity = 0.95f;        // mod_raysetup.frag:36, cpuraycaste
 constexpr int kMaxTfWidth = 4096;           // transfunc1dkeysproperty.cpp:95
 constexpr int kDefaultLoopCap = 255 * 255;  // cpuraycaster.cpp:239
 constexpr int kNumCounters = 6;
+// compositing modes (values of hipdvr_params::compositing)
+constexpr int kCompDVR = 0, kCompMIP = 1, kCompMOP = 2, kCompMIDA = 3, kCompISO = 4, kCompFHP = 5,
+              kCompFHN = 6;
 
 // Arguments of one march launch (plain data; passed by value as kernarg).
 struct MarchArgs {
@@ -25,6 +28,8 @@ struct MarchArgs {
     const void* bricks;         // 4^3-block tiled voxels (bricked kernel)
     const void* cells;          // corner-replicated cell records (cell kernel)
     const uint8_t* brick_occ;   // per-brick occupancy (1 = may hold alpha > 0) for this TF
+    const uint32_t* occ_bits;   // the same occupancy, one bit per brick (bit b&31 of word b>>5)
+    uint32_t occ_words;
     uint32_t dims[3];
     uint32_t nbricks[3];
     uint32_t brick_log2;
@@ -54,6 +59,8 @@ struct MarchArgs {
     // (gx*stride + pix_offset, gy*stride + pix_offset), writes out[gy*grid_w + gx]
     int32_t grid_w, grid_h, stride, pix_offset;
     const int* cond_flag;       // if set, the launch does nothing unless *cond_flag != 0
+    int32_t compositing;        // kComp*
+    float gamma, iso_value;     // MIDA gamma, ISO value
     unsigned long long* counters;  // [rays, samples, fetched, shaded, lane_slots, shade_slots]
     float alpha_corr[256];      // 1 - pow(1 - a/255, step*200) per alpha byte
 };

@@ -175,6 +175,171 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
     }
 }
 
+// compositeMIDAhelper, mod_compositing.frag:112-129 (maxIntensity read only).
+__device__ __forceinline__ void mida_helper(float res[4], float voxel, const float col[4], float max_int,
+                                            float gamma) {
+    float delta = 0.0f;
+    if (voxel > max_int) delta = voxel - max_int;
+    const float beta = 1.0f - delta * (1.0f + gamma);
+    const float k = (1.0f - beta * res[3]) * col[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) res[i] = beta * res[i] + k * col[i];
+    res[3] = beta * res[3] + (1.0f - beta * res[3]) * col[3];
+}
+
+// Compositing modes other than DVR with TF classification: the GLSL
+// compositing functions (mod_compositing.frag:85-213) applied where the TF
+// alpha is > 0, after shading (rc_singlevolume.frag:222-236), on the
+// CPURaycaster sample sequence; early termination only for ISO, FHP and FHN
+// (volumeraycaster.cpp:225-233).
+template <typename T, int FILTER, bool PHONG>
+__global__ __launch_bounds__(256) void march_modes_kernel(const MarchArgs a) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
+    s_alpha[tid] = a.alpha_corr[tid];
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool inside = px < a.w && py < a.h;
+    unsigned long long n_rays = 0, n_samples = 0, n_shaded = 0;
+    LinearFetch<T> f;
+    f.p = (const T*)a.vox;
+    f.X = a.dims[0];
+    f.XY = (uint64_t)a.dims[0] * a.dims[1];
+    const int mode = a.compositing;
+    const bool ert = mode == kCompISO || mode == kCompFHP || mode == kCompFHN;
+    const size_t pix = (size_t)py * a.w + px;
+    const float4 e = inside ? a.entry[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 l = inside ? a.exit[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float out[4] = {0.f, 0.f, 0.f, 0.f};
+    if (inside && !is_background(e, l)) {
+        n_rays = 1;
+        const float first[3] = {e.x, e.y, e.z};
+        const float last[3] = {l.x, l.y, l.z};
+        float dir[3];
+        const float tend = ray_setup(first, last, a.step, dir);
+        float t = 0.0f, max_int = 0.0f;
+        bool finished = false;
+        for (int loop = 0; !finished && loop < a.loop_cap; ++loop) {
+            ++n_samples;
+            const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2]};
+            const float inten = reconstruct<T, FILTER>(f, a, s);
+            const float rw = (inten * a.rwm_scale) + a.rwm_offset;
+            const uint32_t c8 = s_tf[tf_index(inten, a)];
+            float col[4] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu), byte_norm((c8 >> 16) & 0xffu),
+                            byte_norm(c8 >> 24)};
+            if (col[3] > 0.0f) {
+                float gr[3] = {0.f, 0.f, 0.f};
+                if (PHONG || mode == kCompFHN) {
+                    if (t == 0.0f) {
+                        clip_border_gradient(a, px, py, s, dir, gr);
+                    } else {
+                        float vals[6];
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            const int axis = k % 3;
+                            const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
+                            float q[3] = {s[0], s[1], s[2]};
+                            q[axis] = s[axis] + off;
+                            q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
+                            q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
+                            vals[k] = reconstruct<T, FILTER>(f, a, q);
+                        }
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+                    }
+                }
+                if (PHONG) {
+                    float pos[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
+                    phong(a, gr, pos, col);
+                    ++n_shaded;
+                }
+                if (mode == kCompMIP) {
+                    if (rw > max_int) {
+                        for (int i = 0; i < 4; ++i) out[i] = col[i];
+                        max_int = rw;
+                    }
+                } else if (mode == kCompMOP) {
+                    if (col[3] > out[3])
+                        for (int i = 0; i < 4; ++i) out[i] = col[i];
+                } else if (mode == kCompMIDA) {
+                    col[3] = s_alpha[c8 >> 24];
+                    if (a.gamma <= 0.0f) {
+                        mida_helper(out, rw, col, max_int, a.gamma);
+                    } else {
+                        float mip[4], mida[4];
+                        for (int i = 0; i < 4; ++i) mip[i] = mida[i] = out[i];
+                        if (rw > max_int) {
+                            for (int i = 0; i < 4; ++i) mip[i] = col[i];
+                            max_int = rw;
+                        }
+                        mida_helper(mida, rw, col, max_int, 0.0f);
+                        for (int i = 0; i < 4; ++i) out[i] = a.gamma * mip[i] + (1.0f - a.gamma) * mida[i];
+                    }
+                } else if (mode == kCompISO) {
+                    if (col[3] >= a.iso_value - 0.02f && col[3] <= a.iso_value + 0.02f) {
+                        out[0] = col[0];
+                        out[1] = col[1];
+                        out[2] = col[2];
+                        out[3] = 1.0f;
+                    }
+                } else if (mode == kCompFHP) {
+                    if (out[0] == 0.f && out[1] == 0.f && out[2] == 0.f) {
+                        out[0] = s[0];
+                        out[1] = s[1];
+                        out[2] = s[2];
+                        out[3] = 1.0f;
+                    }
+                } else if (mode == kCompFHN) {
+                    if (out[0] == 0.f && out[1] == 0.f && out[2] == 0.f) {
+                        const float l2 = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+                        const float rl = l2 > 0.f ? 1.0f / sqrtf(l2) : 0.f;
+                        for (int i = 0; i < 3; ++i) out[i] = ((gr[i] * rl) / 2.0f) + 0.5f;
+                        out[3] = 1.0f;
+                    }
+                }
+            }
+            if (ert && out[3] >= kErtOpacity) {
+                out[3] = 1.0f;
+                finished = true;
+            }
+            t += a.step;
+            finished = finished || (t > tend);
+        }
+    }
+    if (inside) a.out[pix] = make_float4(out[0], out[1], out[2], out[3]);
+    wave_add(a.counters + 0, n_rays);
+    wave_add(a.counters + 1, n_samples);
+    wave_add(a.counters + 2, n_samples);
+    wave_add(a.counters + 3, n_shaded);
+}
+
+void launch_march_modes(const MarchArgs& a, int dtype, int filter, bool phong, hipStream_t st) {
+    const dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);
+    if (dtype == 0) {
+        if (filter == 0) {
+            if (phong) hipLaunchKernelGGL((march_modes_kernel<uint8_t, 0, true>), grid, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((march_modes_kernel<uint8_t, 0, false>), grid, dim3(256), 0, st, a);
+        } else {
+            if (phong) hipLaunchKernelGGL((march_modes_kernel<uint8_t, 1, true>), grid, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((march_modes_kernel<uint8_t, 1, false>), grid, dim3(256), 0, st, a);
+        }
+    } else {
+        if (filter == 0) {
+            if (phong) hipLaunchKernelGGL((march_modes_kernel<uint16_t, 0, true>), grid, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((march_modes_kernel<uint16_t, 0, false>), grid, dim3(256), 0, st, a);
+        } else {
+            if (phong) hipLaunchKernelGGL((march_modes_kernel<uint16_t, 1, true>), grid, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((march_modes_kernel<uint16_t, 1, false>), grid, dim3(256), 0, st, a);
+        }
+    }
+}
+
 template <typename T, int FILTER, int CLASSIF, bool PHONG>
 static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
     dim3 grid((a.w + 15) / 16, (a.h + 15) / 16);

@@ -54,6 +54,8 @@ struct hipdvr_volume {
     void* d_tiled = nullptr;
     uint2* d_minmax = nullptr;
     uint8_t* d_occ = nullptr;        // per-brick occupancy for the last TF mapping
+    uint32_t* d_occ_bits = nullptr;  // the same, one bit per brick
+    uint32_t occ_words = 0;
     uint64_t occ_key[4] = {0, 0, 0, 0};  // TF identity/domain/rwm the occupancy belongs to
     uint32_t nblocks[3] = {0, 0, 0}; // 4^3 blocks per axis
     uint32_t nbricks[3] = {0, 0, 0};
@@ -88,6 +90,7 @@ struct OccArgs {
     float rwm_scale, rwm_offset, dom_lo, dom_hi;
     int32_t tf_width;
     const uint32_t* alpha_prefix;   // [w+1]: number of non-zero alpha bytes in [0, i)
+    uint32_t* bits;                 // [ceil(count/32)] occupancy bits (written after occ)
 };
 void launch_occupancy(const OccArgs& a, hipStream_t st);
 

@@ -158,10 +158,22 @@ __global__ __launch_bounds__(256) void occupancy_kernel(const OccArgs a) {
     a.occ[i] = (a.alpha_prefix[i1 + 1] - a.alpha_prefix[i0]) != 0u ? 1 : 0;
 }
 
+__global__ __launch_bounds__(256) void occupancy_bits_kernel(const uint8_t* __restrict__ occ,
+                                                             uint32_t* __restrict__ bits, uint64_t count) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 32 >= count) return;
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < 32 && w * 32 + k < count; ++k) word |= (occ[w * 32 + k] != 0 ? 1u : 0u) << k;
+    bits[w] = word;
+}
+
 }  // namespace
 
 void launch_occupancy(const OccArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(occupancy_kernel, dim3((unsigned)((a.count + 255) / 256)), dim3(256), 0, st, a);
+    const uint64_t words = (a.count + 31) / 32;
+    hipLaunchKernelGGL(occupancy_bits_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, a.occ, a.bits,
+                       a.count);
 }
 
 hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volume_desc* d,
@@ -197,6 +209,8 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
         hip_check(hipMalloc(&v->d_tiled, v->tiled_elems * vsz), "hipMalloc(tiled volume)");
         hip_check(hipMalloc((void**)&v->d_minmax, v->brick_count * sizeof(uint2)), "hipMalloc(minmax)");
         hip_check(hipMalloc((void**)&v->d_occ, v->brick_count), "hipMalloc(occupancy)");
+        v->occ_words = (uint32_t)((v->brick_count + 31) / 32);
+        hip_check(hipMalloc((void**)&v->d_occ_bits, (size_t)v->occ_words * 4), "hipMalloc(occupancy bits)");
         hip_check(hipMemcpyAsync(v->d_linear, voxels, bytes,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
                   "volume upload");
@@ -251,6 +265,7 @@ void volume_destroy(hipdvr_volume* v) {
     if (v->d_tiled) (void)hipFree(v->d_tiled);
     if (v->d_minmax) (void)hipFree(v->d_minmax);
     if (v->d_occ) (void)hipFree(v->d_occ);
+    if (v->d_occ_bits) (void)hipFree(v->d_occ_bits);
     delete v;
 }
 

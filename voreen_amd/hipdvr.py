@@ -15,10 +15,10 @@ OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 U8, U16 = 0, 1
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 CLASS_TF, CLASS_PREINT = 0, 1
-COMPOSITE_DVR = 0
+COMPOSITE_DVR, COMPOSITE_MIP, COMPOSITE_MOP, COMPOSITE_MIDA, COMPOSITE_ISO, COMPOSITE_FHP, COMPOSITE_FHN = range(7)
 SHADE_NONE, SHADE_PHONG = 0, 1
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
-KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL, KERNEL_CELL_PIPE = 0, 1, 2, 3, 4
 
 # every symbol include/hipdvr.h and include/hipdvr_synth.h declare
 EXPORTS = (
@@ -61,7 +61,8 @@ class Params(C.Structure):
                 ("light_pos", C.c_float * 3), ("light_ambient", C.c_float * 3),
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
-                ("loop_cap", C.c_int32), ("kernel", C.c_int32), ("batch", C.c_int32)]
+                ("loop_cap", C.c_int32), ("kernel", C.c_int32), ("batch", C.c_int32),
+                ("gamma", C.c_float), ("iso_value", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -220,14 +221,17 @@ def synth_shells(seed, dims):
 
 
 def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading=SHADE_NONE,
-                   kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5), batch=0):
+                   kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5), batch=0,
+                   compositing=COMPOSITE_DVR, gamma=0.0, iso_value=0.5):
     """Render parameters with Voreen's VolumeRenderer light defaults
     (src/core/processors/volumerenderer.cpp:41-54)."""
     p = Params()
     p.step = step
     p.filter = filter_
     p.classification = classification
-    p.compositing = COMPOSITE_DVR
+    p.compositing = compositing
+    p.gamma = gamma
+    p.iso_value = iso_value
     p.shading = shading
     lt = dict(pos=(2.3, 1.5, 1.5), ambient=(0.4,) * 3, diffuse=(0.8,) * 3, specular=(0.6,) * 3,
               shininess=60.0)

@@ -63,6 +63,8 @@ HIPRaycaster::HIPRaycaster()
     addProperty(shadeMode_);
     shadeMode_.select("none");
     addProperty(compositingMode1_);
+    addProperty(isoValue_);
+    addProperty(gammaValue1_);
     addProperty(lightPosition_);
     addProperty(lightAmbient_);
     addProperty(lightDiffuse_);
@@ -204,7 +206,17 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
     p.step = step;
     p.filter = texFilterMode_.getValue() == GL_NEAREST ? HIPDVR_FILTER_NEAREST : HIPDVR_FILTER_LINEAR;
     p.classification = HIPDVR_CLASS_TF;
-    p.compositing = HIPDVR_COMPOSITE_DVR;
+    // compositing keys of volumeraycaster.cpp:379-388
+    const std::string comp = compositingMode1_.getKey();
+    p.compositing = comp == "mip"    ? HIPDVR_COMPOSITE_MIP
+                  : comp == "mop"    ? HIPDVR_COMPOSITE_MOP
+                  : comp == "mida"   ? HIPDVR_COMPOSITE_MIDA
+                  : comp == "iso"    ? HIPDVR_COMPOSITE_ISO
+                  : comp == "fhp"    ? HIPDVR_COMPOSITE_FHP
+                  : comp == "fhn"    ? HIPDVR_COMPOSITE_FHN
+                                     : HIPDVR_COMPOSITE_DVR;
+    p.gamma = gammaValue1_.get();
+    p.iso_value = isoValue_.get();
     p.shading = shadeMode_.getKey() == "phong" ? HIPDVR_SHADE_PHONG : HIPDVR_SHADE_NONE;
     // light and camera to the volume's physical space (src/core/utils/glsl.cpp:61-72)
     const tgt::mat4 worldToPhysical = vb->getWorldToPhysicalMatrix();
@@ -266,8 +278,8 @@ void HIPRaycaster::process() {
         return;
     }
     const std::string comp = compositingMode1_.getKey();
-    if (comp != "dvr") {
-        fail("compositing mode '" + comp + "' is not implemented by HIPRaycaster (dvr only)");
+    if (comp != "dvr" && classificationMode_.getKey() != "transfer-function") {
+        fail("compositing mode '" + comp + "' needs transfer-function classification");
         writeOutput(std::vector<float>(), size.x, size.y);
         return;
     }

@@ -44,6 +44,7 @@ public:
         , outport1_(Port::OUTPORT, "image.output", "Image Output", true, Processor::INVALID_PROGRAM)
         , samplingRate_("samplingRate", "Sampling Rate", 2.f, 0.01f, 20.f)
         , isoValue_("isoValue", "Iso Value", 0.5f, 0.0f, 1.0f)
+        , gammaValue1_("gammaValue", "Gamma Value (OP1)", 0.f, -1.f, 1.f)
         , gradientMode_("gradient", "Gradient Calculation", Processor::INVALID_PROGRAM)
         , classificationMode_("classification", "Classification", Processor::INVALID_PROGRAM)
         , shadeMode_("shading", "Shading", Processor::INVALID_PROGRAM)
@@ -94,6 +95,7 @@ protected:
     RenderPort outport1_;
     FloatProperty samplingRate_;
     FloatProperty isoValue_;
+    FloatProperty gammaValue1_;
     StringOptionProperty gradientMode_;
     StringOptionProperty classificationMode_;
     StringOptionProperty shadeMode_;
