@@ -84,9 +84,11 @@ def test_unsupported_options_log_and_clear(rc):
     rc.set_images(en, ex)
     rc.set_tf([(0.0, (255, 255, 255, 255))])
     assert rc.set("compositing", "mip") == 0
+    assert rc.set("classification", "pre-integrated-cpu") == 0
     out, nerr = rc.process()
     assert nerr == 1 and "compositing" in rc.last_error() and np.all(out == 0)
     rc.set("compositing", "dvr")
+    rc.set("classification", "transfer-function")
     rc.set("shading", "toon")
     out, nerr = rc.process()
     assert nerr == 1 and "shading" in rc.last_error() and np.all(out == 0)
