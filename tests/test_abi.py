@@ -128,3 +128,57 @@ def test_workload_helpers():
     assert np.isclose(urb[0] - llf[0], 1.0) and np.isclose(urb[1] - llf[1], 0.75)
     keys, tf = W.make_tf("c1")
     assert tf.shape == (4096, 4) and 0.68 < np.mean(tf[:, 3] == 0) < 0.72
+
+
+# --------------------------------------------------------------------------
+# .vvd parsing (host only; vvdformat.cpp:53-90)
+# --------------------------------------------------------------------------
+
+def test_vvd_parse_matches_the_bundled_fixtures():
+    from voreen_amd import hipdvr as H
+    for name in ("nucleon", "walnut"):
+        ref = F.load_vvd(name)
+        d, raw = H.vvd_parse(os.path.join(F.GOLDEN, name + ".vvd"))
+        assert tuple(d.dims) == ref["dims"]
+        assert d.dtype == (H.U8 if ref["data"].dtype == np.uint8 else H.U16)
+        assert np.array_equal(np.float32(d.spacing[:]), np.float32(ref["spacing"]))
+        assert np.array_equal(np.float32(d.offset[:]), np.float32(ref["offset"]))
+        assert (d.rwm_scale, d.rwm_offset) == (1.0, 0.0)
+        assert os.path.samefile(raw, ref["raw_path"])
+
+
+def test_vvd_parse_paths_list_rwm_comments_and_errors(tmp_path):
+    from voreen_amd import hipdvr as H
+    doc = """<!-- a comment with <Volume> inside -->
+<?xml version="1.0" ?>
+<VoreenData version="1">
+    <Volumes>
+        <Volume>
+            <RawData format="uint16" x="7" y="5" z="3">
+                <Paths noPathSet="false"><paths><item value="sub/data.raw" /></paths></Paths>
+            </RawData>
+            <MetaData>
+                <MetaItem name="Offset" type="Vec3MetaData"><value x="-1.5" y="2" z="0.25" /></MetaItem>
+                <MetaItem name="RealWorldMapping" type="RealWorldMappingMetaData">
+                    <value scale="4095" offset="-1024" unit="HU" />
+                </MetaItem>
+                <MetaItem name="Spacing" type="Vec3MetaData"><value x="0.5" y="0.75" z="2" /></MetaItem>
+            </MetaData>
+        </Volume>
+    </Volumes>
+</VoreenData>
+"""
+    p = tmp_path / "v.vvd"
+    p.write_text(doc)
+    d, raw = H.vvd_parse(str(p))
+    assert tuple(d.dims) == (7, 5, 3) and d.dtype == H.U16
+    assert tuple(d.offset) == (-1.5, 2.0, 0.25) and tuple(d.spacing) == (0.5, 0.75, 2.0)
+    assert (d.rwm_scale, d.rwm_offset) == (4095.0, -1024.0)
+    assert raw == str(tmp_path / "sub" / "data.raw")
+    (tmp_path / "f.vvd").write_text(doc.replace('format="uint16"', 'format="float"'))
+    with pytest.raises(H.HipDvrError) as e:
+        H.vvd_parse(str(tmp_path / "f.vvd"))
+    assert e.value.code == H.ERR_UNSUPPORTED and "float" in str(e.value)
+    with pytest.raises(H.HipDvrError) as e:
+        H.vvd_parse(str(tmp_path / "missing.vvd"))
+    assert e.value.code == H.ERR_INVALID

@@ -424,3 +424,31 @@ def test_bricked_brick_sizes(ctx, oracle, brick_log2):
         t.close()
     cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
     assert np.abs(img - cpu).max() <= TOL and st["samples"] == n
+
+
+@pytest.mark.parametrize("name", ["nucleon", "walnut"])
+def test_vvd_load_and_volume_minmax(ctx, oracle, name):
+    """A volume loaded from the .vvd renders like the same voxels uploaded from
+    memory, and its min/max match the fixture's VolumeMinMax derived data."""
+    from voreen_amd import hipdvr as H, workloads as W
+    ref = F.load_vvd(name)
+    say(f"vvd load {name}")
+    v = ctx.load_vvd(os.path.join(F.GOLDEN, name + ".vvd"))
+    vm = ctx.volume(ref["data"])
+    tf = oracle.tf_build(W.random_keys(3, 5), 256 if ref["data"].dtype == np.uint8 else 4096)
+    t = ctx.transfunc(tf)
+    try:
+        mn = v.minmax()
+        assert abs(mn[0] - ref["derived"]["min"]) <= 1e-7 and abs(mn[1] - ref["derived"]["max"]) <= 1e-7
+        assert mn[0] == np.float32(ref["data"].min()) / np.float32(255 if ref["data"].dtype == np.uint8 else 65535)
+        llf, urb = W.world_box(ref["dims"], ref["spacing"])
+        cam = W.orbit_camera(3, 12)
+        en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 64, 64, llf=llf, urb=urb)
+        p = H.default_params(W.step_for(ref["dims"], 2.0))
+        a, _ = ctx.render(v, t, en, ex, p)
+        b, _ = ctx.render(vm, t, en, ex, p)
+        assert np.array_equal(a, b) and np.mean(a[..., 3] > 0) > 0.01
+    finally:
+        v.close()
+        vm.close()
+        t.close()

@@ -76,6 +76,8 @@ void volume_destroy(hipdvr_volume* vol);
 constexpr uint64_t kMaxCellBytes = 96ull << 30;
 uint64_t cell_record_bytes(const hipdvr_volume* vol);
 bool cells_allowed(const hipdvr_volume* vol);
+// Raw minimum and maximum voxel of the whole volume (synchronous).
+void volume_raw_minmax(const hipdvr_volume* vol, hipStream_t st, uint32_t out[2]);
 // Build the cell records if they are not there yet (synchronous).
 void ensure_cells(hipdvr_volume* vol, hipStream_t st);
 

@@ -224,6 +224,51 @@ hipdvr_volume* volume_create(hipdvr_ctx* ctx, hipStream_t st, const hipdvr_volum
     return v;
 }
 
+namespace {
+// One workgroup: minimum and maximum over the brick table (every voxel lies
+// in some brick's range).
+__global__ __launch_bounds__(1024) void minmax_reduce_kernel(const uint2* __restrict__ mm, uint64_t n,
+                                                            uint32_t* __restrict__ out) {
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint2 v = mm[i];
+        lo = v.x < lo ? v.x : lo;
+        hi = v.y > hi ? v.y : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    __shared__ uint32_t s_lo[16], s_hi[16];
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        out[0] = lo;
+        out[1] = hi;
+    }
+}
+}  // namespace
+
+void volume_raw_minmax(const hipdvr_volume* v, hipStream_t st, uint32_t out[2]) {
+    uint32_t* d = nullptr;
+    hip_check(hipMalloc((void**)&d, 8), "hipMalloc(minmax)");
+    hipLaunchKernelGGL(minmax_reduce_kernel, dim3(1), dim3(1024), 0, st, v->d_minmax, v->brick_count, d);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = e1 == hipSuccess ? hipMemcpyAsync(out, d, 8, hipMemcpyDeviceToHost, st) : e1;
+    const hipError_t e3 = e2 == hipSuccess ? hipStreamSynchronize(st) : e2;
+    (void)hipFree(d);
+    hip_check(e3, "volume min/max");
+}
+
 uint64_t cell_record_bytes(const hipdvr_volume* v) {
     uint64_t n = 8;
     for (int i = 0; i < 3; ++i) n *= (v->desc.dims[i] + 1) / 2;

@@ -37,6 +37,8 @@ EXPORTS = (
     "hipdvr_learned_create", "hipdvr_learned_destroy", "hipdvr_learned_prepare",
     "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
     "hipdvr_render_learned_device",
+    "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
+    "hipdvr_volume_minmax",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -153,6 +155,11 @@ def load():
         "hipdvr_learned_collect": ([vp, C.c_float, C.c_float, vp, vp, u32, P(u32)], C.c_int),
         "hipdvr_render_learned_device": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
                                           P(LearnedStats)], C.c_int),
+        "hipdvr_vvd_parse": ([C.c_char_p, P(VolumeDesc), C.c_char_p, u32], C.c_int),
+        "hipdvr_vvd_last_error": ([], C.c_char_p),
+        "hipdvr_volume_load_raw": ([vp, C.c_char_p, P(VolumeDesc), C.c_uint64, P(vp)], C.c_int),
+        "hipdvr_volume_load_vvd": ([vp, C.c_char_p, u32, P(vp), P(VolumeDesc)], C.c_int),
+        "hipdvr_volume_minmax": ([vp, P(C.c_float)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -249,6 +256,17 @@ def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading
     return p
 
 
+def vvd_parse(path):
+    """Parse a .vvd file (host only): (VolumeDesc, raw file path)."""
+    lib = load()
+    d = VolumeDesc()
+    buf = C.create_string_buffer(4096)
+    rc = lib.hipdvr_vvd_parse(os.fsencode(path), C.byref(d), buf, len(buf))
+    if rc != OK:
+        raise HipDvrError(rc, f"hipdvr_vvd_parse({path}): {lib.hipdvr_vvd_last_error().decode()}")
+    return d, buf.value.decode()
+
+
 class Context:
     """One device context: stream, events, counters (hipdvr_create)."""
 
@@ -285,6 +303,27 @@ class Context:
 
     def transfunc(self, rgba8, domain=(0.0, 1.0)):
         return TransFunc(self, rgba8, domain)
+
+    def load_vvd(self, path, brick_log2=4):
+        h, d = C.c_void_p(), VolumeDesc()
+        self.check(self.lib.hipdvr_volume_load_vvd(self.h, os.fsencode(path), brick_log2, C.byref(h),
+                                                   C.byref(d)), "hipdvr_volume_load_vvd")
+        d.brick_log2 = brick_log2
+        return Volume.from_handle(self, h, d)
+
+    def load_raw(self, path, dims, dtype, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0),
+                 brick_log2=4, byte_offset=0):
+        d = VolumeDesc()
+        d.dims[:] = list(dims)
+        d.dtype = dtype
+        d.rwm_scale, d.rwm_offset = rwm
+        d.spacing[:] = list(spacing)
+        d.offset[:] = list(offset)
+        d.brick_log2 = brick_log2
+        h = C.c_void_p()
+        self.check(self.lib.hipdvr_volume_load_raw(self.h, os.fsencode(path), C.byref(d), byte_offset,
+                                                   C.byref(h)), "hipdvr_volume_load_raw")
+        return Volume.from_handle(self, h, d)
 
     # ---- buffers ----
     def alloc(self, nbytes):
@@ -377,6 +416,18 @@ class Volume:
         self.h = C.c_void_p()
         ctx.check(ctx.lib.hipdvr_volume_create(ctx.h, C.byref(d), _ptr(voxels), C.byref(self.h)),
                   "hipdvr_volume_create")
+
+    @classmethod
+    def from_handle(cls, ctx, h, desc):
+        v = cls.__new__(cls)
+        v.ctx, v.h, v.desc = ctx, h, desc
+        return v
+
+    def minmax(self):
+        """(minNorm, maxNorm, min, max) of the whole volume (VolumeMinMax)."""
+        out = (C.c_float * 4)()
+        self.ctx.check(self.ctx.lib.hipdvr_volume_minmax(self.h, out), "hipdvr_volume_minmax")
+        return tuple(out)
 
     def close(self):
         if self.h:
