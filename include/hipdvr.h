@@ -34,6 +34,12 @@
  *        src/core/datastructures/transfunc/1d/preintegrationtable.cpp:256-279).
  *   hipdvr_preint_build
  *       PreIntegrationTable::computeTable (preintegrationtable.cpp:67-204).
+ *   hipdvr_tf_preint_device
+ *       the same table computed on the device and attached to the TF: the
+ *       "pre-integrated-gpu" classification mode (classificationmodes.cpp:84-90,
+ *       PreIntegrationTable::computeTable's GPU path preintegrationtable.cpp:
+ *       206-254 in the reference; here the exact/integral CPU algorithm runs
+ *       one lane per table entry).
  *   hipdvr_render
  *       the pixel loop and ray march of CPURaycaster::process/directRendering
  *       (cpuraycaster.cpp:157-208 with :210-326), host entry/exit/output buffers
@@ -204,6 +210,12 @@ int hipdvr_tf_set_preint(hipdvr_tf* tf, const float* table_rgba, uint32_t n);
 int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, float gamma, float thr_lo,
                         float thr_hi, int32_t alpha_mode, uint32_t n, float step,
                         int32_t use_integral, float* out_rgba);
+/* Build the N x N pre-integration table on the device (exact variant, or the
+ * integral approximation when use_integral != 0) from the TF keys and attach
+ * it to tf, replacing any table set before. */
+int hipdvr_tf_preint_device(hipdvr_tf* tf, const hipdvr_tf_key* keys, int32_t nkeys, float gamma,
+                            float thr_lo, float thr_hi, int32_t alpha_mode, uint32_t n, float step,
+                            int32_t use_integral);
 
 /* ---- rendering ---- */
 /* Host buffers (W*H RGBA float, row 0 = bottom), synchronous. */

@@ -202,6 +202,31 @@ def test_phong_on_shells(ctx, oracle, kernel):
 
 
 @pytest.mark.parametrize("integral", [False, True])
+def test_preint_table_on_the_device(ctx, oracle, integral):
+    """hipdvr_tf_preint_device against the oracle's table (the device's pow/exp
+    are within an ulp of libm's) and the render with it."""
+    from voreen_amd import hipdvr as H, workloads as W
+    vol, tf, en, ex, step = c0_inputs(oracle, size=128)
+    for keys, n in ((W.make_tf("c0")[0], 256), (W.random_keys(8, 7), 1024)):
+        say(f"device preint integral={integral} n={n}")
+        ref = oracle.preint_build(keys, n, step, use_integral=integral)
+        t = ctx.transfunc(tf)
+        v = ctx.volume(vol)
+        try:
+            t.preint_device(keys, n, step, use_integral=integral)
+            img, st = ctx.render(v, t, en, ex, H.default_params(step, classification=H.CLASS_PREINT))
+            t.set_preint(ref)
+            img_ref, _ = ctx.render(v, t, en, ex, H.default_params(step, classification=H.CLASS_PREINT))
+        finally:
+            t.close()
+            v.close()
+        cpu, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, classification=1, preint=ref)
+        print(f"n={n}: render max {np.abs(img - cpu).max():.3g}", flush=True)
+        assert np.array_equal(img_ref, cpu)
+        assert np.abs(img - cpu).max() <= TOL
+
+
+@pytest.mark.parametrize("integral", [False, True])
 def test_preintegrated_classification(ctx, oracle, integral):
     vol, tf, en, ex, step = c0_inputs(oracle, size=128)
     from voreen_amd import workloads as W

@@ -116,17 +116,6 @@ def test_process_without_device_logs_an_error():
         r.close()
 
 
-def test_pre_integrated_gpu_falls_back_to_cpu(rc):
-    """cpuraycaster.cpp:123-129: 'pre-integrated-gpu' selects '-cpu' and invalidates."""
-    rc.set_volume(np.full((8, 8, 8), 100, np.uint8))
-    en = np.full((2, 2, 4), 0.3, np.float32)
-    rc.set_images(en, en)
-    rc.set("classification", "pre-integrated-gpu")
-    out, nerr = rc.process()
-    assert nerr == 1
-    assert rc.describe()["properties"]["classification"]["value"] == "pre-integrated-cpu"
-
-
 # ---------------------------------------------------------------------------
 # GPU: process() renders what the oracle renders
 # ---------------------------------------------------------------------------
@@ -185,6 +174,12 @@ def test_process_phong_and_preintegration(rc, oracle):
     assert nerr == 0, rc.last_error()
     table = oracle.preint_build(keys, 256, step)
     ref, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, classification=1, preint=table)
+    assert np.abs(out - ref).max() <= 1e-3
+    print("[case] HIPRaycaster.process pre-integrated-gpu", flush=True)
+    rc.set("classification", "pre-integrated-gpu")
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    assert rc.describe()["properties"]["classification"]["value"] == "pre-integrated-gpu"
     assert np.abs(out - ref).max() <= 1e-3
 
 

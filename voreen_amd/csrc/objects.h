@@ -30,5 +30,8 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
                      const float* d_entry, const float* d_exit, float* d_out, uint32_t w, uint32_t h,
                      unsigned long long* counters);
 void ensure_occupancy(hipdvr_ctx* ctx, hipdvr_volume* vol, const hipdvr_tf* tf);
+// TF colour per pre-integration table index (RGBA float x n); false on bad keys.
+bool preint_index_colors(const hipdvr_tf_key* keys, int32_t nkeys, float gamma, float thr_lo, float thr_hi,
+                         int32_t alpha_mode, uint32_t n, float* tfb);
 
 }  // namespace hipdvr

@@ -96,13 +96,15 @@ extern "C" int hipdvr_tf_from_keys(const hipdvr_tf_key* keys, int32_t nkeys, int
     return HIPDVR_OK;
 }
 
-extern "C" int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, float gamma,
-                                   float thr_lo, float thr_hi, int32_t alpha_mode, uint32_t n_,
-                                   float step, int32_t use_integral, float* table) {
-    if (!table || n_ < 2 || n_ > 4096 || !keys_valid(keys, nkeys) || !(step > 0.f))
-        return HIPDVR_ERR_INVALID;
+namespace hipdvr {
+
+// TF colour per pre-integration table index (preintegrationtable.cpp:74-96):
+// index i maps intensity i/(n-1) through gamma, the keys and the threshold.
+bool preint_index_colors(const hipdvr_tf_key* keys, int32_t nkeys, float gamma, float thr_lo, float thr_hi,
+                         int32_t alpha_mode, uint32_t n_, float* tfb) {
+    if (n_ < 2 || !keys_valid(keys, nkeys)) return false;
     const int n = (int)n_;
-    std::vector<float> tfb((size_t)n * 4, 0.f);
+    for (size_t i = 0; i < (size_t)n * 4; ++i) tfb[i] = 0.f;
     const int front_end = iround_tgt(thr_lo * (float)n);
     const int back_start = iround_tgt(thr_hi * (float)n);
     for (int i = front_end < 0 ? 0 : front_end; i < back_start && i < n; ++i) {
@@ -114,6 +116,19 @@ extern "C" int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, flo
         if (alpha_mode == HIPDVR_ALPHA_ONE && v[3] != 0.f) v[3] = 1.f;
         else if (alpha_mode == HIPDVR_ALPHA_ZERO) v[3] = 0.f;
     }
+    return true;
+}
+
+}  // namespace hipdvr
+
+extern "C" int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, float gamma,
+                                   float thr_lo, float thr_hi, int32_t alpha_mode, uint32_t n_,
+                                   float step, int32_t use_integral, float* table) {
+    if (!table || n_ < 2 || n_ > 4096 || !keys_valid(keys, nkeys) || !(step > 0.f))
+        return HIPDVR_ERR_INVALID;
+    const int n = (int)n_;
+    std::vector<float> tfb((size_t)n * 4, 0.f);
+    hipdvr::preint_index_colors(keys, nkeys, gamma, thr_lo, thr_hi, alpha_mode, n_, tfb.data());
     const float e_full = step * 200.0f;
     if (!use_integral) {
         parallel_rows(n, [&](int sb) {

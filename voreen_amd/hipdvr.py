@@ -38,7 +38,7 @@ EXPORTS = (
     "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
     "hipdvr_render_learned_device",
     "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
-    "hipdvr_volume_minmax",
+    "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -160,6 +160,8 @@ def load():
         "hipdvr_volume_load_raw": ([vp, C.c_char_p, P(VolumeDesc), C.c_uint64, P(vp)], C.c_int),
         "hipdvr_volume_load_vvd": ([vp, C.c_char_p, u32, P(vp), P(VolumeDesc)], C.c_int),
         "hipdvr_volume_minmax": ([vp, P(C.c_float)], C.c_int),
+        "hipdvr_tf_preint_device": ([vp, P(TFKey), i32, C.c_float, C.c_float, C.c_float, i32, u32, C.c_float,
+                                     i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -443,6 +445,13 @@ class TransFunc:
         self.h = C.c_void_p()
         ctx.check(ctx.lib.hipdvr_tf_create(ctx.h, _ptr(rgba8), self.width, domain[0], domain[1],
                                            C.byref(self.h)), "hipdvr_tf_create")
+
+    def preint_device(self, keys, n, step, use_integral=False, gamma=1.0, thr=(0.0, 1.0),
+                      alpha_mode=ALPHA_USE):
+        """Build the pre-integration table on the device and attach it."""
+        arr, nk = make_keys(keys)
+        self.ctx.check(self.ctx.lib.hipdvr_tf_preint_device(self.h, arr, nk, gamma, thr[0], thr[1], alpha_mode, n,
+                                                            step, int(use_integral)), "hipdvr_tf_preint_device")
 
     def set_preint(self, table):
         table = np.ascontiguousarray(table, np.float32)

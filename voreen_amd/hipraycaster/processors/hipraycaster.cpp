@@ -270,13 +270,6 @@ void HIPRaycaster::process() {
         writeOutput(std::vector<float>(), size.x, size.y);
         return;
     }
-    if (classificationMode_.getKey() == "pre-integrated-gpu") {
-        // same fallback as cpuraycaster.cpp:123-129
-        LERROR("HIPRaycaster computes the pre-integration table on the CPU. Falling back to CPU.");
-        classificationMode_.select("pre-integrated-cpu");
-        invalidate();
-        return;
-    }
     const std::string comp = compositingMode1_.getKey();
     if (comp != "dvr" && classificationMode_.getKey() != "transfer-function") {
         fail("compositing mode '" + comp + "' needs transfer-function classification");
@@ -327,6 +320,38 @@ void HIPRaycaster::process() {
                                                 static_cast<uint32_t>(table->getDimension()));
             if (rc != HIPDVR_OK) {
                 fail(std::string("pre-integration table upload failed: ") + hipdvr_last_error(ctx_));
+                writeOutput(std::vector<float>(), size.x, size.y);
+                return;
+            }
+            uploadedPreintKey_ = key;
+            uploadedPreintStep_ = step;
+        }
+        params.classification = HIPDVR_CLASS_PREINT;
+    } else if (cls == "pre-integrated-gpu") {
+        // the exact table computed on the device from the TF keys
+        // (preintegrationtable.cpp:67-204 run per entry; :206-254 is the GL path)
+        const std::string key = cls + ":" + std::to_string(preIntegrationTableSize_.getValue());
+        if (uploadedPreintKey_ != key || uploadedPreintStep_ != step) {
+            const TransFunc1DKeys* tf = tFunc1DKeys_.get();
+            std::vector<hipdvr_tf_key> keys((size_t)tf->getNumKeys());
+            for (int i = 0; i < tf->getNumKeys(); ++i) {
+                const TransFuncMappingKey* k = tf->getKey(i);
+                keys[(size_t)i].intensity = k->getIntensity();
+                keys[(size_t)i].split = k->isSplit() ? 1 : 0;
+                const tgt::col4 cl = k->getColorL(), cr = k->getColorR();
+                for (int c = 0; c < 4; ++c) {
+                    keys[(size_t)i].color_l[c] = cl[(size_t)c];
+                    keys[(size_t)i].color_r[c] = cr[(size_t)c];
+                }
+            }
+            const int width = tf->getWidth();
+            const int dim = preIntegrationTableSize_.getValue();
+            const uint32_t n = (uint32_t)(dim ? dim : (width < 1024 ? width : 1024));
+            const tgt::vec2 thr = tf->getThreshold();
+            const int rc = hipdvr_tf_preint_device(dvrTF_, keys.data(), (int32_t)keys.size(), tf->getGammaValue(),
+                                                   thr.x, thr.y, (int32_t)tf->getAlphaMode(), n, step, 0);
+            if (rc != HIPDVR_OK) {
+                fail(std::string("pre-integration table on the device failed: ") + hipdvr_last_error(ctx_));
                 writeOutput(std::vector<float>(), size.x, size.y);
                 return;
             }

@@ -13,6 +13,22 @@
 
 namespace voreen {
 
+// TransFuncMappingKey accessors (1d/1dkeys/utils/transfuncmappingkey.h:97-172)
+class TransFuncMappingKey {
+public:
+    explicit TransFuncMappingKey(const hipdvr_tf_key& k) : k_(k) {}
+    float getIntensity() const { return k_.intensity; }
+    tgt::col4 getColorL() const { return tgt::col4(k_.color_l[0], k_.color_l[1], k_.color_l[2], k_.color_l[3]); }
+    tgt::col4 getColorR() const { return tgt::col4(k_.color_r[0], k_.color_r[1], k_.color_r[2], k_.color_r[3]); }
+    bool isSplit() const { return k_.split != 0; }
+
+private:
+    hipdvr_tf_key k_;
+};
+
+// TransFuncBase::AlphaMode (transfuncbase.h:165-169)
+enum AlphaModeShim { TF_ZERO_ALPHA = 0, TF_USE_ALPHA = 1, TF_ONE_ALPHA = 2 };
+
 class TransFunc1DKeys {
 public:
     TransFunc1DKeys(int width = 1024) : width_(width), domain_(0.f, 1.f) {
@@ -20,7 +36,24 @@ public:
         hipdvr_tf_key k1 = {1.f, 0, {255, 255, 255, 255}, {255, 255, 255, 255}};
         keys_ = {k0, k1};   // setToStandardFunc (transfunc1dkeys.cpp:413-421)
     }
-    void setKeys(const std::vector<hipdvr_tf_key>& k) { keys_ = k; invalidate(); }
+    void setKeys(const std::vector<hipdvr_tf_key>& k) {
+        keys_ = k;
+        views_.clear();
+        for (const hipdvr_tf_key& key : keys_) views_.emplace_back(key);
+        invalidate();
+    }
+    // transfunc1dkeys.h:91-102, transfunc1d.h:104,148, transfuncbase.h:173
+    int getNumKeys() const { return (int)keys_.size(); }
+    const TransFuncMappingKey* getKey(int i) const {
+        if (views_.size() != keys_.size()) {
+            views_.clear();
+            for (const hipdvr_tf_key& key : keys_) views_.emplace_back(key);
+        }
+        return &views_[(size_t)(i < 0 ? 0 : (i >= (int)views_.size() ? (int)views_.size() - 1 : i))];
+    }
+    float getGammaValue() const { return 1.f; }
+    tgt::vec2 getThreshold() const { return tgt::vec2(0.f, 1.f); }
+    AlphaModeShim getAlphaMode() const { return TF_USE_ALPHA; }
     void resize(int width) { if (width != width_) { width_ = width; invalidate(); } }
     int getWidth() const { return width_; }
     void setDomain(const tgt::vec2& d) { domain_ = d; table_.reset(); }
@@ -57,6 +90,7 @@ private:
     int width_;
     tgt::vec2 domain_;
     std::vector<hipdvr_tf_key> keys_;
+    mutable std::vector<TransFuncMappingKey> views_;
     std::unique_ptr<tgt::Texture> tex_;
     std::unique_ptr<PreIntegrationTable> table_;
     size_t table_n_ = 0;
