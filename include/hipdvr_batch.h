@@ -1,0 +1,70 @@
+/*
+ * hipdvr_batch.h -- many volumes, each with its own TF and camera, marched in
+ * one persistent launch.
+ *
+ * No counterpart in the reference: Voreen renders one volume per raycaster
+ * process() call (cpuraycaster.cpp:90-208; a network of N raycasters makes N
+ * calls).  A batch is the library's answer for scenes of many uneven volumes
+ * (multi-view dashboards, ensembles): every job is the plain DVR march of
+ * hipdvr_render_device (linear filter, TF classification, DVR compositing,
+ * optional Phong) with the same result bit for bit, but all jobs' image tiles
+ * are handed out by one device work queue to a persistent grid of workgroups,
+ * largest jobs first.  The launch can be captured into a hipGraph and
+ * replayed; an autotuner picks the wave tile shape, the empty-space brick
+ * size and the number of samples fetched together per ray.
+ */
+#ifndef HIPDVR_BATCH_H
+#define HIPDVR_BATCH_H
+
+#include <stdint.h>
+
+#include "hipdvr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hipdvr_batch hipdvr_batch;
+
+typedef struct hipdvr_batch_job {
+    const hipdvr_volume* volume;
+    const hipdvr_tf* tf;
+    const float* d_entry;   /* device RGBA32F w*h (hipdvr_entry_exit_device) */
+    const float* d_exit;
+    float* d_out;           /* device RGBA32F w*h */
+    uint32_t w, h;
+    hipdvr_params params;   /* step, shading, light, camera; filter must be linear,
+                               classification TF, compositing DVR */
+} hipdvr_batch_job;
+
+/* wave tile shapes (pixels per wavefront) */
+enum { HIPDVR_TILE_8x8 = 0, HIPDVR_TILE_16x4 = 1, HIPDVR_TILE_32x2 = 2 };
+
+typedef struct hipdvr_batch_config {
+    int32_t tile;           /* HIPDVR_TILE_* */
+    int32_t brick_log2;     /* empty-space bricks of 8^3, 16^3, 32^3: 3, 4, 5 */
+    int32_t batch;          /* samples fetched together per ray: 2 or 4 */
+} hipdvr_batch_config;
+
+/* Jobs are copied; the volumes, TFs and buffers must outlive the batch.
+ * Builds each volume's cell records and, per candidate brick size, each
+ * job's occupancy table for its TF. */
+int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t n, hipdvr_batch** out);
+int hipdvr_batch_destroy(hipdvr_batch* b);
+/* select a configuration (drops a captured graph) */
+int hipdvr_batch_set_config(hipdvr_batch* b, const hipdvr_batch_config* cfg);
+int hipdvr_batch_get_config(const hipdvr_batch* b, hipdvr_batch_config* cfg);
+/* Time every configuration over reps launches each (HIP events), keep the
+ * fastest; best_ms (may be NULL) receives its mean launch time. */
+int hipdvr_batch_autotune(hipdvr_batch* b, uint32_t reps, hipdvr_batch_config* best, float* best_ms);
+/* Capture one batched launch (queue reset + persistent march) as a hipGraph
+ * on the context stream; hipdvr_batch_launch then replays it. */
+int hipdvr_batch_capture(hipdvr_batch* b);
+/* Asynchronous on the context stream: the graph when captured, else a direct
+ * launch; with stats != NULL it is a counted, synchronous launch. */
+int hipdvr_batch_launch(hipdvr_batch* b, hipdvr_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

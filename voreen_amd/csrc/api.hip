@@ -114,6 +114,8 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     const VolumeDesc& v = vol->desc;
     a.vox = vol->d_linear;
     a.bricks = vol->d_tiled;
+    a.cells = vol->d_cells;
+    a.dtype = v.dtype;
     a.brick_occ = vol->d_occ;
     a.occ_bits = vol->d_occ_bits;
     a.occ_words = vol->occ_words;

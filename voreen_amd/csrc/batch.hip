@@ -1,0 +1,425 @@
+// batch.hip -- many volumes in one persistent launch (include/hipdvr_batch.h).
+//
+// Every job is the cell-record march of hipdvr_render_device (march_cell.h),
+// so each job's image is bit-identical to its single-frame render.  The image
+// tiles of all jobs form one list, jobs in decreasing order of estimated cost
+// (pixels x longest volume axis: largest first); a persistent grid of
+// workgroups (CUs x resident workgroups per CU) takes tiles from a device
+// counter until the list is exhausted, reloading the TF and opacity table in
+// LDS when a tile belongs to another job.  Every workgroup leaves when the
+// counter passes the end of the list, so the grid always drains.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/hipdvr_batch.h"
+#include "ctx.h"
+#include "errors.h"
+#include "kernels.h"
+#include "march_cell.h"
+#include "objects.h"
+#include "volume.h"
+
+namespace hipdvr {
+
+namespace {
+
+constexpr int kTileShapes = 3;
+constexpr int kBrickSizes = 3;   // brick_log2 3, 4, 5
+
+// pixels per wavefront: 8x8, 16x4, 32x2; a workgroup is 2x2 wavefronts
+__host__ __device__ constexpr int wave_w(int tile) { return tile == 0 ? 8 : (tile == 1 ? 16 : 32); }
+__host__ __device__ constexpr int wave_h(int tile) { return 64 / wave_w(tile); }
+
+__global__ void batch_reset_kernel(uint32_t* counter) {
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+template <bool PHONG, bool COUNT, int K, int TILE>
+__global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ tile_start, uint32_t njobs,
+                                                    uint32_t total_tiles, uint32_t* __restrict__ counter,
+                                                    unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    __shared__ uint32_t s_tile;
+    constexpr int WW = wave_w(TILE), WH = wave_h(TILE);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int cur_job = -1;
+    RayCounters n;
+    while (true) {
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();   // also: every wave has finished the previous tile
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();   // s_tile is read before the next iteration writes it
+        if (tile >= total_tiles) break;
+        // job of this tile: last j with tile_start[j] <= tile
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tile_start[mid] <= tile) lo = mid;
+            else hi = mid;
+        }
+        const int job = (int)__builtin_amdgcn_readfirstlane(lo);
+        const MarchArgs& a = jobs[job];
+        if (job != cur_job) {
+            for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
+            s_alpha[tid] = a.alpha_corr[tid];
+            __syncthreads();
+            cur_job = job;
+        }
+        const uint32_t lt = tile - tile_start[job];
+        const uint32_t tiles_x = (uint32_t)(a.w + 2 * WW - 1) / (2 * WW);
+        const int px = (int)(lt % tiles_x) * (2 * WW) + (wave & 1) * WW + (lane % WW);
+        const int py = (int)(lt / tiles_x) * (2 * WH) + (wave >> 1) * WH + (lane / WW);
+        if (px < a.w && py < a.h) {
+            const size_t pix = (size_t)py * a.w + px;
+            const float4 e = a.entry[pix];
+            const float4 l = a.exit[pix];
+            float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!is_background(e, l)) {
+                if (a.dtype == 0) {
+                    const CellFetch<uint8_t, false> f = cell_fetch_of<uint8_t, false>(a);
+                    res = cell_ray<uint8_t, false, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                } else {
+                    const CellFetch<uint16_t, false> f = cell_fetch_of<uint16_t, false>(a);
+                    res = cell_ray<uint16_t, false, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                }
+            }
+            a.out[pix] = res;
+        }
+    }
+    if (COUNT) {
+        wave_add(counters + 0, n.rays);
+        wave_add(counters + 1, n.samples);
+        wave_add(counters + 2, n.fetched);
+        wave_add(counters + 3, n.shaded);
+        wave_add_max64(counters + 4, n.slots);
+        wave_add_max64(counters + 5, n.shade_slots);
+    }
+}
+
+typedef void (*BatchKernel)(const MarchArgs*, const uint32_t*, uint32_t, uint32_t, uint32_t*,
+                            unsigned long long*);
+
+template <bool PHONG, bool COUNT>
+BatchKernel pick_kernel(int tile, int k) {
+    if (k <= 2) {
+        if (tile == 0) return batch_kernel<PHONG, COUNT, 2, 0>;
+        if (tile == 1) return batch_kernel<PHONG, COUNT, 2, 1>;
+        return batch_kernel<PHONG, COUNT, 2, 2>;
+    }
+    if (tile == 0) return batch_kernel<PHONG, COUNT, 4, 0>;
+    if (tile == 1) return batch_kernel<PHONG, COUNT, 4, 1>;
+    return batch_kernel<PHONG, COUNT, 4, 2>;
+}
+
+BatchKernel kernel_for(bool phong, bool count, int tile, int k) {
+    if (phong) return count ? pick_kernel<true, true>(tile, k) : pick_kernel<true, false>(tile, k);
+    return count ? pick_kernel<false, true>(tile, k) : pick_kernel<false, false>(tile, k);
+}
+
+}  // namespace
+
+}  // namespace hipdvr
+
+using namespace hipdvr;
+
+struct hipdvr_batch {
+    hipdvr_ctx* ctx = nullptr;
+    std::vector<hipdvr_batch_job> jobs;    // in launch order (largest first)
+    bool phong = false;
+    int sms = 0;
+    // per brick size: device brick tables per job and the jobs' MarchArgs
+    std::vector<void*> allocations;
+    MarchArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
+    uint32_t* d_tile_start[kTileShapes] = {nullptr, nullptr, nullptr};
+    uint32_t total_tiles[kTileShapes] = {0, 0, 0};
+    uint32_t* d_counter = nullptr;
+    hipdvr_batch_config cfg{0, 4, 4};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void drop_graph(hipdvr_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_batch(hipdvr_batch* b) {
+    if (!b) return;
+    drop_graph(b);
+    for (void* p : b->allocations) (void)hipFree(p);
+    delete b;
+}
+
+template <typename P>
+P* dev_alloc(hipdvr_batch* b, size_t bytes) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes ? bytes : 4), "hipMalloc(batch)");
+    b->allocations.push_back(p);
+    return (P*)p;
+}
+
+void validate_config(const hipdvr_batch_config& c) {
+    if (c.tile < HIPDVR_TILE_8x8 || c.tile > HIPDVR_TILE_32x2) fail(HIPDVR_ERR_INVALID, "unknown tile shape");
+    if (c.brick_log2 < 3 || c.brick_log2 > 5) fail(HIPDVR_ERR_INVALID, "brick_log2 must be 3, 4 or 5");
+    if (c.batch != 2 && c.batch != 4) fail(HIPDVR_ERR_INVALID, "batch must be 2 or 4");
+}
+
+void launch_batch(hipdvr_batch* b, bool count) {
+    hipStream_t st = ctx_stream(b->ctx);
+    const int t = b->cfg.tile, bs = b->cfg.brick_log2 - 3;
+    BatchKernel k = kernel_for(b->phong, count, t, b->cfg.batch);
+    int per_cu = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 256, 0), "occupancy query");
+    if (per_cu < 1) per_cu = 1;
+    uint32_t grid = (uint32_t)(b->sms * per_cu);
+    if (grid > b->total_tiles[t]) grid = b->total_tiles[t] ? b->total_tiles[t] : 1;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->d_counter);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, (const MarchArgs*)b->d_args[bs],
+                       (const uint32_t*)b->d_tile_start[t], (uint32_t)b->jobs.size(), b->total_tiles[t],
+                       b->d_counter, b->ctx->d_counters);
+    hip_check(hipGetLastError(), "batch launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t n, hipdvr_batch** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(ctx, [&] {
+        if (!out || !jobs || n == 0) fail(HIPDVR_ERR_INVALID, "a batch needs at least one job");
+        int shade = -1;
+        for (uint32_t i = 0; i < n; ++i) {
+            const hipdvr_batch_job& j = jobs[i];
+            if (!j.volume || !j.tf || !j.d_entry || !j.d_exit || !j.d_out || !j.w || !j.h)
+                fail(HIPDVR_ERR_INVALID, "job " + std::to_string(i) + ": NULL argument or empty image");
+            if (j.volume->ctx != ctx || j.tf->ctx != ctx) fail(HIPDVR_ERR_INVALID, "handle from another context");
+            validate_params(&j.params, j.tf);
+            if (j.params.filter != HIPDVR_FILTER_LINEAR || j.params.classification != HIPDVR_CLASS_TF ||
+                j.params.compositing != HIPDVR_COMPOSITE_DVR)
+                fail(HIPDVR_ERR_UNSUPPORTED, "batched jobs use the linear filter, TF classification and DVR");
+            const int ph = j.params.shading == HIPDVR_SHADE_PHONG ? 1 : 0;
+            if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading mode");
+            shade = ph;
+            if (cell_record_bytes(j.volume) > 0xffffff00ull)
+                fail(HIPDVR_ERR_UNSUPPORTED, "batched volumes must have cell records below 4 GiB");
+        }
+        ctx_set_device(ctx);
+        hipStream_t st = ctx_stream(ctx);
+        hipdvr_batch* b = new hipdvr_batch();
+        b->ctx = ctx;
+        b->phong = shade == 1;
+        try {
+            int dev = 0;
+            hip_check(hipGetDevice(&dev), "hipGetDevice");
+            hip_check(hipDeviceGetAttribute(&b->sms, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+            // largest estimated cost first
+            std::vector<uint32_t> order(n);
+            std::iota(order.begin(), order.end(), 0u);
+            auto cost = [&](uint32_t i) {
+                const hipdvr_volume* v = jobs[i].volume;
+                const uint32_t m = std::max(v->desc.dims[0], std::max(v->desc.dims[1], v->desc.dims[2]));
+                return (double)jobs[i].w * jobs[i].h * m;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+            for (uint32_t i : order) b->jobs.push_back(jobs[i]);
+            // cell records, then per brick size the brick tables of every job
+            for (const hipdvr_batch_job& j : b->jobs) ensure_cells(const_cast<hipdvr_volume*>(j.volume), st);
+            std::vector<MarchArgs> args(n);
+            for (int bs = 0; bs < kBrickSizes; ++bs) {
+                const uint32_t bl = (uint32_t)bs + 3;
+                for (uint32_t i = 0; i < n; ++i) {
+                    const hipdvr_batch_job& j = b->jobs[i];
+                    const hipdvr_volume* v = j.volume;
+                    uint32_t nb[3];
+                    uint64_t count = 1;
+                    for (int k = 0; k < 3; ++k) {
+                        nb[k] = (v->desc.dims[k] + (1u << bl) - 1) >> bl;
+                        count *= nb[k];
+                    }
+                    uint2* mm = dev_alloc<uint2>(b, count * sizeof(uint2));
+                    uint8_t* occ = dev_alloc<uint8_t>(b, count);
+                    const uint32_t words = (uint32_t)((count + 31) / 32);
+                    uint32_t* bits = dev_alloc<uint32_t>(b, (size_t)words * 4);
+                    launch_brick_minmax(v, bl, nb, mm, st);
+                    OccArgs o;
+                    o.minmax = mm;
+                    o.occ = occ;
+                    o.count = count;
+                    o.dtype = v->desc.dtype;
+                    o.rwm_scale = v->desc.rwm_scale;
+                    o.rwm_offset = v->desc.rwm_offset;
+                    o.dom_lo = j.tf->domain_lo;
+                    o.dom_hi = j.tf->domain_hi;
+                    o.tf_width = (int32_t)j.tf->width;
+                    o.alpha_prefix = j.tf->d_alpha_prefix;
+                    o.bits = bits;
+                    launch_occupancy(o, st);
+                    MarchArgs& a = args[i];
+                    fill_march_args(a, v, j.tf, &j.params, j.d_entry, j.d_exit, j.d_out, j.w, j.h, nullptr);
+                    a.cells = v->d_cells;
+                    a.brick_log2 = bl;
+                    for (int k = 0; k < 3; ++k) a.nbricks[k] = nb[k];
+                    a.brick_occ = occ;
+                    a.occ_bits = bits;
+                    a.occ_words = words;
+                }
+                b->d_args[bs] = dev_alloc<MarchArgs>(b, n * sizeof(MarchArgs));
+                hip_check(hipMemcpyAsync(b->d_args[bs], args.data(), n * sizeof(MarchArgs), hipMemcpyHostToDevice, st),
+                          "upload");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");   // args is reused
+            }
+            for (int t = 0; t < kTileShapes; ++t) {
+                std::vector<uint32_t> start(n + 1, 0);
+                const uint32_t tw = 2 * wave_w(t), th = 2 * wave_h(t);
+                for (uint32_t i = 0; i < n; ++i) {
+                    const uint64_t tiles = (uint64_t)((b->jobs[i].w + tw - 1) / tw) * ((b->jobs[i].h + th - 1) / th);
+                    if (start[i] + tiles > 0xfffffff0ull) fail(HIPDVR_ERR_UNSUPPORTED, "batch too large");
+                    start[i + 1] = start[i] + (uint32_t)tiles;
+                }
+                b->total_tiles[t] = start[n];
+                b->d_tile_start[t] = dev_alloc<uint32_t>(b, (n + 1) * 4);
+                hip_check(hipMemcpyAsync(b->d_tile_start[t], start.data(), (n + 1) * 4, hipMemcpyHostToDevice, st),
+                          "upload");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            }
+            b->d_counter = dev_alloc<uint32_t>(b, 64);
+            hip_check(hipGetLastError(), "batch setup");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        } catch (...) {
+            free_batch(b);
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int hipdvr_batch_destroy(hipdvr_batch* b) {
+    if (!b) return HIPDVR_OK;
+    (void)hipStreamSynchronize(ctx_stream(b->ctx));
+    free_batch(b);
+    return HIPDVR_OK;
+}
+
+int hipdvr_batch_set_config(hipdvr_batch* b, const hipdvr_batch_config* cfg) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        if (!cfg) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        validate_config(*cfg);
+        b->cfg = *cfg;
+        drop_graph(b);
+    });
+}
+
+int hipdvr_batch_get_config(const hipdvr_batch* b, hipdvr_batch_config* cfg) {
+    if (!b || !cfg) return HIPDVR_ERR_INVALID;
+    *cfg = b->cfg;
+    return HIPDVR_OK;
+}
+
+int hipdvr_batch_capture(hipdvr_batch* b) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        ctx_set_device(b->ctx);
+        hipStream_t st = ctx_stream(b->ctx);
+        drop_graph(b);
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launch_batch(b, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st, &b->graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    });
+}
+
+int hipdvr_batch_launch(hipdvr_batch* b, hipdvr_stats* stats) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        ctx_set_device(b->ctx);
+        hipStream_t st = ctx_stream(b->ctx);
+        if (stats) {
+            launch_zero_u64(b->ctx->d_counters, kNumCounters, st);
+            hip_check(hipEventRecord(b->ctx->ev0, st), "hipEventRecord");
+            launch_batch(b, true);
+            hip_check(hipEventRecord(b->ctx->ev1, st), "hipEventRecord");
+            unsigned long long c[kNumCounters];
+            hip_check(hipMemcpyAsync(c, b->ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st), "download");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1), "hipEventElapsedTime");
+            stats->rays = c[0];
+            stats->samples = c[1];
+            stats->fetched = c[2];
+            stats->shaded = c[3];
+            stats->lane_slots = c[4];
+            stats->shade_slots = c[5];
+            stats->kernel_ms = ms;
+            return;
+        }
+        const bool timed = ctx_timing_start(b->ctx);
+        if (b->exec) hip_check(hipGraphLaunch(b->exec, st), "hipGraphLaunch");
+        else launch_batch(b, false);
+        if (timed) ctx_timing_stop(b->ctx);
+    });
+}
+
+int hipdvr_batch_autotune(hipdvr_batch* b, uint32_t reps, hipdvr_batch_config* best, float* best_ms) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        if (reps == 0) fail(HIPDVR_ERR_INVALID, "reps must be > 0");
+        ctx_set_device(b->ctx);
+        hipStream_t st = ctx_stream(b->ctx);
+        drop_graph(b);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hipdvr_batch_config win = b->cfg;
+        float win_ms = 1e30f;
+        try {
+            for (int t = 0; t < kTileShapes; ++t)
+                for (int bl = 3; bl <= 5; ++bl)
+                    for (int k : {2, 4}) {
+                        b->cfg = hipdvr_batch_config{t, bl, k};
+                        launch_batch(b, false);   // warm-up
+                        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+                        for (uint32_t r = 0; r < reps; ++r) launch_batch(b, false);
+                        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                        float ms = 0.f;
+                        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                        ms /= (float)reps;
+                        if (ms < win_ms) {
+                            win_ms = ms;
+                            win = b->cfg;
+                        }
+                    }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->cfg = win;
+        if (best) *best = win;
+        if (best_ms) *best_ms = win_ms;
+    });
+}
+
+}  // extern "C"

@@ -1,131 +1,11 @@
-// march_cell.hip -- ray march over corner-replicated cell records.
-//
-// The record of cell (x, y, z) holds the eight voxels of its trilinear
-// footprint, (x..x+1, y..y+1, z..z+1) clamped to the volume, in the order the
-// trilinear sum visits them (llB, lrB, urB, ulB, llF, lrF, urF, ulF of
-// VolumeRAM::getVoxelNormalizedLinear, volumeram.cpp:78-97).  One sample is
-// one 16-byte (uint16) or 8-byte (uint8) load instead of eight 2- or 1-byte
-// gathers: the march is bound by the vector-memory issue rate per load
-// instruction, not by bytes, so one wide load per sample is the MI355X trade
-// of HBM capacity (8x the voxel bytes) for load instructions.
-//
-// Sample positions, corners, weights, the 8-term sum, TF classification,
-// opacity correction, Phong's six central-difference lookups
-// (mod_gradients.frag:202-214) and the lighting are the same expressions as
-// the reference-layout kernel (march_ref.hip, restating
-// CPURaycaster::directRendering, cpuraycaster.cpp:210-326), so the image is
-// bit-identical to it, shaded or not.  Empty-space skipping is the exact
-// brick fast-forward of the bricked kernel; the samples after a non-empty
-// sample are fetched K at a time (their loads in flight together) and
-// composited in order; a sample fetched past early termination is dropped.
+// march_cell.hip -- per-frame launches of the cell-record ray march (the
+// device code and its parity argument are in march_cell.h): one workgroup per
+// 16x16 pixel tile in XCD-aware order, and a software-pipelined variant.
 #include "kernels.h"
+#include "march_cell.h"
 #include "march_common.h"
 
 namespace hipdvr {
-
-template <typename T>
-struct CellRecord;
-template <>
-struct CellRecord<uint16_t> {
-    typedef uint4 type;
-    static __device__ __forceinline__ void unpack(const uint4& r, float v[8]) {
-        v[0] = VoxNorm<uint16_t>::apply(r.x & 0xffffu);
-        v[1] = VoxNorm<uint16_t>::apply(r.x >> 16);
-        v[2] = VoxNorm<uint16_t>::apply(r.y & 0xffffu);
-        v[3] = VoxNorm<uint16_t>::apply(r.y >> 16);
-        v[4] = VoxNorm<uint16_t>::apply(r.z & 0xffffu);
-        v[5] = VoxNorm<uint16_t>::apply(r.z >> 16);
-        v[6] = VoxNorm<uint16_t>::apply(r.w & 0xffffu);
-        v[7] = VoxNorm<uint16_t>::apply(r.w >> 16);
-    }
-};
-template <>
-struct CellRecord<uint8_t> {
-    typedef uint2 type;
-    static __device__ __forceinline__ void unpack(const uint2& r, float v[8]) {
-        v[0] = VoxNorm<uint8_t>::apply(r.x & 0xffu);
-        v[1] = VoxNorm<uint8_t>::apply((r.x >> 8) & 0xffu);
-        v[2] = VoxNorm<uint8_t>::apply((r.x >> 16) & 0xffu);
-        v[3] = VoxNorm<uint8_t>::apply(r.x >> 24);
-        v[4] = VoxNorm<uint8_t>::apply(r.y & 0xffu);
-        v[5] = VoxNorm<uint8_t>::apply((r.y >> 8) & 0xffu);
-        v[6] = VoxNorm<uint8_t>::apply((r.y >> 16) & 0xffu);
-        v[7] = VoxNorm<uint8_t>::apply(r.y >> 24);
-    }
-};
-
-// Record addressing: 2x2x2 cells per group (one 128-byte line for uint16),
-// groups x-fastest; the record index is separable per axis.
-template <typename T, bool WIDE>
-struct CellFetch {
-    typedef typename CellRecord<T>::type rec_t;
-    typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type off_t;
-    const rec_t* p;
-    __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t sy, sz;   // records per group row (8*GX) and per group slab (8*GX*GY)
-
-    __device__ __forceinline__ off_t index(const uint32_t lo[3]) const {
-        return (off_t)(((lo[0] >> 1) << 3) | (lo[0] & 1)) + (off_t)(lo[1] >> 1) * sy + ((lo[1] & 1) << 1) +
-               (off_t)(lo[2] >> 1) * sz + ((lo[2] & 1) << 2);
-    }
-    __device__ __forceinline__ rec_t load(off_t i) const {
-        if constexpr (WIDE) {
-            return p[i];
-        } else if constexpr (sizeof(T) == 2) {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (uint32_t)i * 16u, 0, 0);
-            return rec_t{v[0], v[1], v[2], v[3]};
-        } else {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (uint32_t)i * 8u, 0, 0);
-            rec_t r;
-            r.x = v[0];
-            r.y = v[1];
-            return r;
-        }
-    }
-};
-
-// Voxel position, corners and record index of the texture-space point s.
-template <typename T, bool WIDE>
-__device__ __forceinline__ typename CellFetch<T, WIDE>::off_t cell_of(const CellFetch<T, WIDE>& f,
-                                                                      const MarchArgs& a, const float s[3],
-                                                                      float p[3]) {
-    float vp[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
-    Corners c;
-    corners(vp, a.dims, c);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) p[i] = c.p[i];
-    return f.index(c.lo);
-}
-
-// Central differences of six trilinear lookups at s +- one voxel per axis,
-// as march_ref.hip computes them; the six loads are issued together.
-template <typename T, bool WIDE>
-__device__ __forceinline__ void cell_gradient(const CellFetch<T, WIDE>& f, const MarchArgs& a,
-                                              const float s[3], float gr[3]) {
-    typename CellFetch<T, WIDE>::rec_t rec[6];
-    float p[6][3];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int axis = k % 3;
-        const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
-        float q[3] = {s[0], s[1], s[2]};
-        q[axis] = s[axis] + off;
-        q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
-        q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
-        rec[k] = f.load(cell_of<T, WIDE>(f, a, q, p[k]));
-    }
-    float vals[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        float v[8];
-        CellRecord<T>::unpack(rec[k], v);
-        vals[k] = trilerp(v, p[k]);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
-}
 
 template <typename T, bool WIDE, bool PHONG, bool COUNT, int K>
 __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, const uint32_t tiles_x,
@@ -149,156 +29,23 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
     const int gy = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
     const bool inside = (gx < a.grid_w) && (gy < a.grid_h) && (px < a.w) && (py < a.h);
-
-    typedef typename CellFetch<T, WIDE>::rec_t rec_t;
-    CellFetch<T, WIDE> f;
-    f.p = (const rec_t*)a.cells;
-    const uint32_t GX = (a.dims[0] + 1) >> 1, GY = (a.dims[1] + 1) >> 1, GZ = (a.dims[2] + 1) >> 1;
-    f.sy = GX << 3;
-    f.sz = f.sy * GY;
-    if (!WIDE) {
-        const uint32_t bytes = f.sz * GZ * (uint32_t)sizeof(rec_t);
-        f.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.cells, (short)0, (int)bytes, 0x00020000);
-    }
-    const uint32_t bl = a.brick_log2;
-    const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
-
-    unsigned long long n_slots = 0, n_shade_slots = 0, n_samples = 0, n_fetched = 0, n_shaded = 0,
-                       n_rays = 0;
+    const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
+    RayCounters n;
     if (inside) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix];
         const float4 l = a.exit[pix];
         float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!is_background(e, l)) {
-            n_rays = 1;
-            const float first[3] = {e.x, e.y, e.z};
-            const float last[3] = {l.x, l.y, l.z};
-            float dir[3];
-            const float tend = ray_setup(first, last, a.step, dir);
-            const int cap = a.loop_cap;
-            float t = 0.0f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
-            uint32_t cur_brick = 0xffffffffu;
-            bool cur_occ = false;
-            float t_lim = 0.0f;
-            bool finished = false;
-            int loop = 0;   // samples taken (the reference loop counter)
-            while (!finished && loop < cap) {
-                const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2]};
-                float vp[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
-                const uint32_t brick = (((uint32_t)vp[2] >> bl) * nbxy) + (((uint32_t)vp[1] >> bl) * nbx) +
-                                       ((uint32_t)vp[0] >> bl);
-                if (brick != cur_brick) {
-                    cur_brick = brick;
-                    cur_occ = a.brick_occ[brick] != 0;
-                    if (!cur_occ) t_lim = empty_brick_limit(a, first, dir, vp, bl);
-                }
-                if (!cur_occ) {
-                    // every sample at t' < t_lim lies in this empty brick: only
-                    // the count and the float sequence t += step remain
-                    ++n_samples;
-                    if (COUNT) ++n_slots;
-                    float tn = t + a.step;
-                    while (tn < t_lim && !(tn > tend) && loop + 1 < cap) {
-                        t = tn;
-                        ++loop;
-                        ++n_samples;
-                        if (COUNT) ++n_slots;
-                        tn = t + a.step;
-                    }
-                    ++loop;
-                    t = tn;
-                    finished = t > tend;
-                    continue;
-                }
-                // K samples t, t+step, ... (each within the ray and the cap);
-                // all loads issued before the first is composited
-                float tk[K], pk[K][3];
-                rec_t rk[K];
-                bool valid[K];
-                {
-                    float tt = t;
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        valid[k] = (k == 0) || (valid[k - 1] && !(tt > tend) && loop + k < cap);
-                        tk[k] = tt;
-                        if (valid[k]) {
-                            float sk[3];
-                            if (k == 0) {
-                                sk[0] = s[0];
-                                sk[1] = s[1];
-                                sk[2] = s[2];
-                            } else {
-                                sk[0] = first[0] + tt * dir[0];
-                                sk[1] = first[1] + tt * dir[1];
-                                sk[2] = first[2] + tt * dir[2];
-                            }
-                            rk[k] = f.load(cell_of<T, WIDE>(f, a, sk, pk[k]));
-                        }
-                        tt = tt + a.step;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (!valid[k] || finished) break;
-                    bool shade_now = false;
-                    ++loop;
-                    ++n_samples;
-                    ++n_fetched;
-                    float v[8];
-                    CellRecord<T>::unpack(rk[k], v);
-                    const float inten = trilerp(v, pk[k]);
-                    const uint32_t c8 = s_tf[tf_index(inten, a)];
-                    const float ca = s_alpha[c8 >> 24];
-                    if (ca > 0.0f) {
-                        float col[3] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu),
-                                        byte_norm((c8 >> 16) & 0xffu)};
-                        if (PHONG) {
-                            shade_now = true;
-                            const float tt = tk[k];
-                            const float ss[3] = {first[0] + tt * dir[0], first[1] + tt * dir[1],
-                                                 first[2] + tt * dir[2]};
-                            float gr[3];
-                            if (tt == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                            else cell_gradient<T, WIDE>(f, a, ss, gr);
-                            float pos[3];
-#pragma unroll
-                            for (int i = 0; i < 3; ++i)
-                                pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                            phong(a, gr, pos, col);
-                            ++n_shaded;
-                        }
-                        const float wgt = (1.0f - A) * ca;
-                        r = r + wgt * col[0];
-                        g = g + wgt * col[1];
-                        bb = bb + wgt * col[2];
-                        A = A + (1.0f - A) * ca;
-                    }
-                    if (COUNT) {
-                        ++n_slots;
-                        n_shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
-                    }
-                    if (A >= kErtOpacity) {
-                        A = 1.0f;
-                        finished = true;
-                    }
-                    t = tk[k] + a.step;
-                    finished = finished || (t > tend);
-                }
-            }
-            res = make_float4(r, g, bb, A);
-        }
+        if (!is_background(e, l)) res = cell_ray<T, WIDE, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
         a.out[(size_t)gy * a.grid_w + gx] = res;
     }
     if (COUNT) {
-        wave_add(a.counters + 0, n_rays);
-        wave_add(a.counters + 1, n_samples);
-        wave_add(a.counters + 2, n_fetched);
-        wave_add(a.counters + 3, n_shaded);
-        wave_add_max64(a.counters + 4, n_slots);
-        wave_add_max64(a.counters + 5, n_shade_slots);
+        wave_add(a.counters + 0, n.rays);
+        wave_add(a.counters + 1, n.samples);
+        wave_add(a.counters + 2, n.fetched);
+        wave_add(a.counters + 3, n.shaded);
+        wave_add_max64(a.counters + 4, n.slots);
+        wave_add_max64(a.counters + 5, n.shade_slots);
     }
 }
 

@@ -38,6 +38,7 @@ struct MarchArgs {
     float rwm_scale, rwm_offset;
     float dom_lo, dom_hi;
     int32_t dom_unit;           // dom_hi - dom_lo == 1: the division is exact, x = rw - lo
+    int32_t dtype;              // 0 uint8, 1 uint16
     int32_t tf_width;
     const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
     const float4* preint;       // N*N, [fe*N + fs]

@@ -76,6 +76,9 @@ void volume_destroy(hipdvr_volume* vol);
 constexpr uint64_t kMaxCellBytes = 96ull << 30;
 uint64_t cell_record_bytes(const hipdvr_volume* vol);
 bool cells_allowed(const hipdvr_volume* vol);
+// Brick min/max table of the volume for bricks of 2^bl voxels (nb bricks per
+// axis), as built at creation for the volume's own brick size.
+void launch_brick_minmax(const hipdvr_volume* vol, uint32_t bl, const uint32_t nb[3], uint2* out, hipStream_t st);
 // Raw minimum and maximum voxel of the whole volume (synchronous).
 void volume_raw_minmax(const hipdvr_volume* vol, hipStream_t st, uint32_t out[2]);
 // Build the cell records if they are not there yet (synchronous).

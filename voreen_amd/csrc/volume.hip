@@ -269,6 +269,17 @@ void volume_raw_minmax(const hipdvr_volume* v, hipStream_t st, uint32_t out[2]) 
     hip_check(e3, "volume min/max");
 }
 
+void launch_brick_minmax(const hipdvr_volume* v, uint32_t bl, const uint32_t nb[3], uint2* out, hipStream_t st) {
+    const uint32_t X = v->desc.dims[0], Y = v->desc.dims[1], Z = v->desc.dims[2];
+    const dim3 grid((unsigned)((uint64_t)nb[0] * nb[1] * nb[2]));
+    if (v->desc.dtype == HIPDVR_U8)
+        hipLaunchKernelGGL(minmax_kernel<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)v->d_linear, out, X, Y,
+                           Z, bl, nb[0], nb[1]);
+    else
+        hipLaunchKernelGGL(minmax_kernel<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)v->d_linear, out, X,
+                           Y, Z, bl, nb[0], nb[1]);
+}
+
 uint64_t cell_record_bytes(const hipdvr_volume* v) {
     uint64_t n = 8;
     for (int i = 0; i < 3; ++i) n *= (v->desc.dims[i] + 1) / 2;

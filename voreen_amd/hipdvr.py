@@ -39,6 +39,8 @@ EXPORTS = (
     "hipdvr_render_learned_device",
     "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
+    "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
+    "hipdvr_batch_autotune", "hipdvr_batch_capture", "hipdvr_batch_launch",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -87,6 +89,18 @@ class LearnedStats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("volume", C.c_void_p), ("tf", C.c_void_p), ("d_entry", C.c_void_p), ("d_exit", C.c_void_p),
+                ("d_out", C.c_void_p), ("w", C.c_uint32), ("h", C.c_uint32), ("params", Params)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("brick_log2", C.c_int32), ("batch", C.c_int32)]
+
+
+TILE_8x8, TILE_16x4, TILE_32x2 = 0, 1, 2
 
 
 class TFKey(C.Structure):
@@ -162,6 +176,13 @@ def load():
         "hipdvr_volume_minmax": ([vp, P(C.c_float)], C.c_int),
         "hipdvr_tf_preint_device": ([vp, P(TFKey), i32, C.c_float, C.c_float, C.c_float, i32, u32, C.c_float,
                                      i32], C.c_int),
+        "hipdvr_batch_create": ([vp, P(BatchJob), u32, P(vp)], C.c_int),
+        "hipdvr_batch_destroy": ([vp], C.c_int),
+        "hipdvr_batch_set_config": ([vp, P(BatchConfig)], C.c_int),
+        "hipdvr_batch_get_config": ([vp, P(BatchConfig)], C.c_int),
+        "hipdvr_batch_autotune": ([vp, u32, P(BatchConfig), P(C.c_float)], C.c_int),
+        "hipdvr_batch_capture": ([vp], C.c_int),
+        "hipdvr_batch_launch": ([vp, P(Stats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -568,4 +589,48 @@ class Learned:
             self.h, tf.h, d_entry.ptr, d_exit.ptr, w, h, C.byref(params),
             C.byref(opts) if opts else None, d_out.ptr, C.byref(st) if stats else None),
             "hipdvr_render_learned_device")
+        return st.as_dict() if stats else None
+
+
+class Batch:
+    """Many volumes in one persistent launch (include/hipdvr_batch.h).
+    jobs: sequence of (volume, tf, d_entry, d_exit, d_out, w, h, params)."""
+
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        arr = (BatchJob * len(jobs))()
+        for i, (vol, tf, de, dx, do, w, h, prm) in enumerate(jobs):
+            arr[i].volume, arr[i].tf = vol.h, tf.h
+            arr[i].d_entry, arr[i].d_exit, arr[i].d_out = de.ptr, dx.ptr, do.ptr
+            arr[i].w, arr[i].h, arr[i].params = w, h, prm
+        self.h = C.c_void_p()
+        ctx.check(ctx.lib.hipdvr_batch_create(ctx.h, arr, len(jobs), C.byref(self.h)), "hipdvr_batch_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hipdvr_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_config(self, tile, brick_log2, batch):
+        c = BatchConfig(tile, brick_log2, batch)
+        self.ctx.check(self.ctx.lib.hipdvr_batch_set_config(self.h, C.byref(c)), "hipdvr_batch_set_config")
+
+    def config(self):
+        c = BatchConfig()
+        self.ctx.check(self.ctx.lib.hipdvr_batch_get_config(self.h, C.byref(c)), "hipdvr_batch_get_config")
+        return (c.tile, c.brick_log2, c.batch)
+
+    def autotune(self, reps=3):
+        c, ms = BatchConfig(), C.c_float()
+        self.ctx.check(self.ctx.lib.hipdvr_batch_autotune(self.h, reps, C.byref(c), C.byref(ms)),
+                       "hipdvr_batch_autotune")
+        return (c.tile, c.brick_log2, c.batch), ms.value
+
+    def capture(self):
+        self.ctx.check(self.ctx.lib.hipdvr_batch_capture(self.h), "hipdvr_batch_capture")
+
+    def launch(self, stats=False):
+        st = Stats()
+        self.ctx.check(self.ctx.lib.hipdvr_batch_launch(self.h, C.byref(st) if stats else None),
+                       "hipdvr_batch_launch")
         return st.as_dict() if stats else None

@@ -104,3 +104,20 @@ def make_tf(cfg):
     c = CONFIGS[cfg]
     keys = random_keys(c["tf_seed"], 4) if cfg == "c0" else shells_keys(c["tf_seed"])
     return keys, H.tf_from_keys(keys, c["tf_width"])
+
+
+def batch_specs(seed=0xC2, n=64, lo=64, hi=384, image=512):
+    """Uneven batch: per job dims log-uniform per axis in [lo, hi], uint8 or
+    uint16 by coin flip, a blob volume (the single-volume generator), a
+    seeded 4-8-key TF and an orbit camera; all seeded from `seed`."""
+    r = SplitMix64(seed)
+    specs = []
+    for j in range(n):
+        dims = tuple(int(round(math.exp(math.log(lo) + r.uniform() * (math.log(hi) - math.log(lo)))))
+                     for _ in range(3))
+        dtype = H.U8 if r.randint(2) == 0 else H.U16
+        keys = random_keys(r.next(), 4 + r.randint(5))
+        cam = orbit_camera(r.randint(36), 36, elevation_deg=-30.0 + 60.0 * r.uniform())
+        specs.append(dict(dims=dims, dtype=dtype, vol_seed=r.next(), keys=keys, camera=cam,
+                          image=(image, image)))
+    return specs
