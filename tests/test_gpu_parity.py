@@ -304,8 +304,13 @@ def test_compositing_modes(ctx, oracle, mode, code, gamma):
     cam = W.orbit_camera(5, 24, radius=2.5)
     wen, wex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 96, 72, llf=llf, urb=urb)
     say(f"compositing {mode} walnut")
-    r = render_both(ctx, oracle, v["data"], wtf, t["domain"], wen, wex, W.step_for(v["dims"], 2.0),
-                    rwm=(65535.0, 0.0), compositing=code, gamma=gamma, iso_value=0.3)
+    # MIDA weighs the real-world intensity directly (beta = 1 - delta*(1+gamma),
+    # mod_compositing.frag:116-119): it is meaningful on normalized values, so
+    # its walnut case uses the identity mapping and the domain in [0, 1]
+    rwm, dom = ((1.0, 0.0), tuple(d / 65535.0 for d in t["domain"])) if mode == "mida" else \
+        ((65535.0, 0.0), t["domain"])
+    r = render_both(ctx, oracle, v["data"], wtf, dom, wen, wex, W.step_for(v["dims"], 2.0),
+                    rwm=rwm, compositing=code, gamma=gamma, iso_value=0.3)
     assert report(f"{mode} walnut", r, oracle).max() <= TOL
     assert np.mean(r["gpu"][..., 3] > 0) > 0.02
 
