@@ -309,8 +309,8 @@ int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_lear
         if (!vol || !out) fail(HIPDVR_ERR_INVALID, "NULL argument");
         if (vol->ctx != ctx) fail(HIPDVR_ERR_INVALID, "volume from another context");
         const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
-        if (vol->tiled_elems * vsz > 0xffffff00ull)
-            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping supports volumes below 4 GiB");
+        if (vol->tiled_elems * vsz > 0xffffff00ull || cell_record_bytes(vol) > 0xffffff00ull)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping supports volumes whose cell records are below 4 GiB");
         ctx_set_device(ctx);
         hipdvr_learned* l = new hipdvr_learned();
         l->ctx = ctx;
@@ -470,9 +470,11 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(l->ctx);
-        // 1. learned frame
+        // 1. learned frame (on the cell records)
+        ensure_cells(vol, st);
         LearnedArgs la;
         fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
+        la.m.cells = vol->d_cells;
         launch_learned(la, phong, count, st);
         // 2. sparse plain re-march (bricked exact kernel on every stride-th pixel)
         MarchArgs sp;

@@ -7,6 +7,7 @@
 
 #include "kernels.h"
 #include "learned.h"
+#include "march_cell.h"
 #include "march_common.h"
 
 namespace hipdvr {
@@ -389,10 +390,16 @@ void launch_targets(const float* err, const unsigned int* visits, const float* f
 }
 
 // ---------------------------------------------------------------------------
-// the learned marcher
+// the learned marcher: the cell-record march (march_cell.h) with the stepping
+// table in place of exact empty-space skipping.  Per 16^3 brick and ray
+// octant the table says skip (jump to the first step-grid point past the
+// brick) or march with a step multiple m in {1, 2, 4, 8}; the last step in a
+// brick is cut so the next brick starts on the base grid, and each sample's
+// opacity correction is the one of the step it stands for.
 // ---------------------------------------------------------------------------
-template <bool PHONG, bool COUNT>
+template <typename T, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
+    constexpr int K = 4;
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[kLMaxMult][256];   // multipliers 1 .. 8
     const MarchArgs& a = la.m;
@@ -406,87 +413,101 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    typedef typename CellFetch<T, false>::rec_t rec_t;
+    const CellFetch<T, false> f = cell_fetch_of<T, false>(a);
     unsigned long long n_samples = 0, n_shaded = 0;
     if (tile < tiles_total && px < a.w && py < a.h) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix], l = a.exit[pix];
         float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!is_background(e, l)) {
-            LFetch f;
-            lfetch_init(f, la);
             const float first[3] = {e.x, e.y, e.z}, last[3] = {l.x, l.y, l.z};
             float dir[3];
             const float tend = ray_setup(first, last, a.step, dir);
             const uint32_t oct = (dir[0] < 0.f ? 1u : 0u) | (dir[1] < 0.f ? 2u : 0u) | (dir[2] < 0.f ? 4u : 0u);
             float t = 0.f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
-            uint32_t cur = 0xffffffffu;
-            int mlog = 0;
-            float t_next_brick = 0.f;
             int guard = 0;
-            while (!(t > tend) && guard++ < a.loop_cap) {
-                float s[3], u[3], vp[3];
+            bool finished = false;
+            while (!finished && !(t > tend) && guard < a.loop_cap) {
+                float s[3], vp[3];
                 for (int i = 0; i < 3; ++i) {
                     s[i] = first[i] + t * dir[i];
-                    u[i] = (a.fdims[i] * s[i]) + (-0.5f);
-                    vp[i] = sminf(smaxf(u[i], 0.0f), a.fdm1[i]);
+                    vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
                 }
                 uint32_t bc[3];
                 for (int i = 0; i < 3; ++i) bc[i] = (uint32_t)vp[i] >> kLBrickLog2;
                 const uint32_t brick = (bc[2] * la.nb16[1] + bc[1]) * la.nb16[0] + bc[0];
-                if (brick != cur) {
-                    cur = brick;
-                    const uint8_t ent = la.table[brick * 8 + oct];
-                    // exact emptiness of the 8^3 ESS bricks is not needed here:
-                    // a 16^3 brick whose range maps to zero alpha is in the table
-                    // as skip by construction (occ16 == 0)
-                    if ((ent & 0x80) || !la.occ16[brick]) {
-                        // skip to the last step-grid point inside the brick; the
-                        // next iteration finds the following brick one step on
-                        const float tx = box_exit(a, first, dir, bc, kLBrick, la.nb16);
-                        const float k = floorf((tx - t) / a.step);
-                        t = t + fmaxf(k, 1.f) * a.step;
-                        cur = 0xffffffffu;
-                        continue;
+                const uint8_t ent = la.table[brick * 8 + oct];
+                const float t_exit = box_exit(a, first, dir, bc, kLBrick, la.nb16);
+                if ((ent & 0x80) || !la.occ16[brick]) {
+                    // skip to the last step-grid point inside the brick; the
+                    // next pass finds the following brick one step on
+                    const float k = floorf((t_exit - t) / a.step);
+                    t = t + fmaxf(k, 1.f) * a.step;
+                    ++guard;
+                    continue;
+                }
+                const int mmax = 1 << (ent & 3);
+                // up to K samples of this brick, their loads in flight together
+                float tk[K], pk[K][3];
+                int mk[K];
+                rec_t rk[K];
+                bool valid[K];
+                float tt = t;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    valid[k] = (k == 0) || (valid[k - 1] && !(tt > tend) && tt < t_exit);
+                    tk[k] = tt;
+                    int m = mmax;
+                    if (m > 1 && tt + (float)m * a.step > t_exit) {
+                        const float c = ceilf((t_exit - tt) / a.step);
+                        m = c < 1.f ? 1 : (c < (float)m ? (int)c : m);
                     }
-                    mlog = ent & 3;
-                    t_next_brick = box_exit(a, first, dir, bc, kLBrick, la.nb16);
-                }
-                // step multiple for this sample; the last one in a brick is cut
-                // so the next brick starts on the base step grid
-                int mult = 1 << mlog;
-                if (mult > 1 && t + (float)mult * a.step > t_next_brick) {
-                    const float k = ceilf((t_next_brick - t) / a.step);
-                    mult = k < 1.f ? 1 : (k < (float)mult ? (int)k : mult);
-                }
-                ++n_samples;
-                Corners c;
-                const float inten = lsample(f, vp, c);
-                const uint32_t c8 = s_tf[tf_index(inten, a)];
-                const float ca = s_alpha[mult - 1][c8 >> 24];
-                if (ca > 0.f) {
-                    float col[3] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu),
-                                    byte_norm((c8 >> 16) & 0xffu)};
-                    if (PHONG) {
-                        float gr[3];
-                        if (t == 0.0f) clip_border_gradient(a, px, py, s, dir, gr);
-                        else lgradient(f, a, u, c, gr);
-                        float pos[3];
-                        for (int i = 0; i < 3; ++i)
-                            pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                        phong_fast(a, gr, pos, col);
-                        ++n_shaded;
+                    mk[k] = m;
+                    if (valid[k]) {
+                        const float sk[3] = {first[0] + tt * dir[0], first[1] + tt * dir[1], first[2] + tt * dir[2]};
+                        rk[k] = f.load(cell_of<T, false>(f, a, sk, pk[k]));
                     }
-                    const float w = (1.f - A) * ca;
-                    r += w * col[0];
-                    g += w * col[1];
-                    bb += w * col[2];
-                    A += w;
+                    tt = tt + (float)m * a.step;
                 }
-                if (A >= kErtOpacity) {
-                    A = 1.f;
-                    break;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (!valid[k] || finished) break;
+                    ++n_samples;
+                    ++guard;
+                    float v[8];
+                    CellRecord<T>::unpack(rk[k], v);
+                    const float inten = trilerp(v, pk[k]);
+                    const uint32_t c8 = s_tf[tf_index(inten, a)];
+                    const float ca = s_alpha[mk[k] - 1][c8 >> 24];
+                    if (ca > 0.f) {
+                        float col[3] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu),
+                                        byte_norm((c8 >> 16) & 0xffu)};
+                        if (PHONG) {
+                            const float t0 = tk[k];
+                            const float ss[3] = {first[0] + t0 * dir[0], first[1] + t0 * dir[1],
+                                                 first[2] + t0 * dir[2]};
+                            float gr[3];
+                            if (t0 == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
+                            else cell_gradient<T, false>(f, a, ss, gr);
+                            float pos[3];
+                            for (int i = 0; i < 3; ++i)
+                                pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
+                            phong_fast(a, gr, pos, col);
+                            ++n_shaded;
+                        }
+                        const float w = (1.f - A) * ca;
+                        r += w * col[0];
+                        g += w * col[1];
+                        bb += w * col[2];
+                        A += w;
+                    }
+                    if (A >= kErtOpacity) {
+                        A = 1.f;
+                        finished = true;
+                    }
+                    t = tk[k] + (float)mk[k] * a.step;
                 }
-                t += (float)mult * a.step;
             }
             res = make_float4(r, g, bb, A);
         }
@@ -501,12 +522,22 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
 void launch_learned(const LearnedArgs& la, bool phong, bool count, hipStream_t st) {
     const uint32_t tiles = ((la.m.w + 15) / 16) * ((la.m.h + 15) / 16);
     const dim3 grid(((tiles + 7) / 8) * 8);
-    if (phong) {
-        if (count) hipLaunchKernelGGL((learned_kernel<true, true>), grid, dim3(256), 0, st, la);
-        else hipLaunchKernelGGL((learned_kernel<true, false>), grid, dim3(256), 0, st, la);
+    if (la.u16) {
+        if (phong) {
+            if (count) hipLaunchKernelGGL((learned_kernel<uint16_t, true, true>), grid, dim3(256), 0, st, la);
+            else hipLaunchKernelGGL((learned_kernel<uint16_t, true, false>), grid, dim3(256), 0, st, la);
+        } else {
+            if (count) hipLaunchKernelGGL((learned_kernel<uint16_t, false, true>), grid, dim3(256), 0, st, la);
+            else hipLaunchKernelGGL((learned_kernel<uint16_t, false, false>), grid, dim3(256), 0, st, la);
+        }
     } else {
-        if (count) hipLaunchKernelGGL((learned_kernel<false, true>), grid, dim3(256), 0, st, la);
-        else hipLaunchKernelGGL((learned_kernel<false, false>), grid, dim3(256), 0, st, la);
+        if (phong) {
+            if (count) hipLaunchKernelGGL((learned_kernel<uint8_t, true, true>), grid, dim3(256), 0, st, la);
+            else hipLaunchKernelGGL((learned_kernel<uint8_t, true, false>), grid, dim3(256), 0, st, la);
+        } else {
+            if (count) hipLaunchKernelGGL((learned_kernel<uint8_t, false, true>), grid, dim3(256), 0, st, la);
+            else hipLaunchKernelGGL((learned_kernel<uint8_t, false, false>), grid, dim3(256), 0, st, la);
+        }
     }
 }
 
