@@ -39,7 +39,7 @@ def parse():
                     help="c2: 64 uneven volumes at 512^2 in one batched persistent launch")
     ap.add_argument("--jobs", type=int, default=64, help="c2: number of volumes")
     ap.add_argument("--no-autotune", action="store_true", help="c2: keep the default configuration")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell", "cellpipe"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
     ap.add_argument("--batch", type=int, default=0, choices=[0, 2, 4, 8],
                     help="cell kernel: samples fetched together per ray (0 = library default)")
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
@@ -161,8 +161,7 @@ def main():
     wimg, himg = cfg["image"]
     nviews = args.views or cfg["views"]
     kernel = {"auto": H.KERNEL_AUTO, "reference": H.KERNEL_REFERENCE,
-              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL,
-              "cellpipe": H.KERNEL_CELL_PIPE}[args.kernel]
+              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL}[args.kernel]
     t_setup = time.perf_counter()
     vol = W.make_volume(args.config)
     keys, tf = W.make_tf(args.config)

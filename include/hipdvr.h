@@ -113,10 +113,8 @@ enum {
     HIPDVR_KERNEL_AUTO = 0,      /* fastest exact kernel for the options */
     HIPDVR_KERNEL_REFERENCE = 1, /* linear layout, one lane per ray, no skipping */
     HIPDVR_KERNEL_BRICKED = 2,   /* bricked layout + exact empty-space skipping */
-    HIPDVR_KERNEL_CELL = 3,      /* corner-replicated cell records + exact empty-space
+    HIPDVR_KERNEL_CELL = 3       /* corner-replicated cell records + exact empty-space
                                     skipping; linear filter, TF classification */
-    HIPDVR_KERNEL_CELL_PIPE = 4  /* the cell kernel with software-pipelined fetches
-                                    and LDS occupancy bits */
 };
 
 typedef struct hipdvr_ctx hipdvr_ctx;

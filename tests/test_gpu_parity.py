@@ -384,7 +384,6 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             outs = {}
             runs = [("ref", H.KERNEL_REFERENCE, 0), ("brick", H.KERNEL_BRICKED, 0)]
             runs += [(f"cell{b}", H.KERNEL_CELL, b) for b in (2, 4, 8)]
-            runs += [(f"cellpipe{b}", H.KERNEL_CELL_PIPE, b) for b in (2, 4)]
             for name, k, batch in runs:
                 for shading in (0, 1):
                     say(f"bricked-vs-reference tf{seed} {name} shading {shading}")
@@ -394,7 +393,7 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             t.close()
             for shading in (0, 1):
                 a, sa = outs[("ref", shading)]
-                for name in ("cell2", "cell4", "cell8", "cellpipe2", "cellpipe4"):
+                for name in ("cell2", "cell4", "cell8"):
                     c, sc = outs[(name, shading)]
                     assert np.array_equal(a, c), f"{name} tf{seed} shading {shading}"
                     assert sa["samples"] == sc["samples"] and sa["shaded"] == sc["shaded"]

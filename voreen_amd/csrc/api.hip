@@ -101,7 +101,7 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_INVALID, "gamma and iso_value must be finite");
     if (p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
-    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL_PIPE)
+    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
     if (p->batch != 0 && p->batch != 2 && p->batch != 4 && p->batch != 8)
         fail(HIPDVR_ERR_INVALID, "batch must be 0, 2, 4 or 8");
@@ -193,7 +193,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         else if (p->filter == HIPDVR_FILTER_LINEAR && cells_allowed(vol)) kernel = HIPDVR_KERNEL_CELL;
         else kernel = HIPDVR_KERNEL_BRICKED;
     }
-    const bool cell = kernel == HIPDVR_KERNEL_CELL || kernel == HIPDVR_KERNEL_CELL_PIPE;
+    const bool cell = kernel == HIPDVR_KERNEL_CELL;
     if ((kernel == HIPDVR_KERNEL_BRICKED || cell) && p->classification != HIPDVR_CLASS_TF)
         fail(HIPDVR_ERR_UNSUPPORTED, "the skipping kernels support transfer-function classification only");
     if (cell && p->filter != HIPDVR_FILTER_LINEAR)
@@ -213,9 +213,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     } else if (kernel == HIPDVR_KERNEL_CELL) {
         launch_march_cell(a, vol->desc.dtype, phong, count, mvol->cells_wide, p->batch ? p->batch : kDefaultBatch,
                           ctx->stream);
-    } else if (kernel == HIPDVR_KERNEL_CELL_PIPE) {
-        launch_march_cell_pipe(a, vol->desc.dtype, phong, count, mvol->cells_wide,
-                               p->batch ? p->batch : kDefaultBatch, ctx->stream);
+
     } else {
         // buffer descriptors address at most 2^32 - 1 bytes
         const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;

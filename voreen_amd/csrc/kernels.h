@@ -29,11 +29,7 @@ void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bo
 void launch_march_cell(const MarchArgs& a, int dtype, bool phong, bool count, bool wide, int batch,
                        hipStream_t st);
 constexpr int kDefaultBatch = 4;
-// software-pipelined cell kernel (batch 2 or 4 samples per chunk)
-void launch_march_cell_pipe(const MarchArgs& a, int dtype, bool phong, bool count, bool wide, int batch,
-                            hipStream_t st);
-// occupancy bit words staged in LDS up to this count (16 KB)
-constexpr uint32_t kMaxLdsOccWords = 4096;
+
 // Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // Streaming copy of n 16-byte vectors (bandwidth calibration).

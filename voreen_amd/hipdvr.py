@@ -18,7 +18,7 @@ CLASS_TF, CLASS_PREINT = 0, 1
 COMPOSITE_DVR, COMPOSITE_MIP, COMPOSITE_MOP, COMPOSITE_MIDA, COMPOSITE_ISO, COMPOSITE_FHP, COMPOSITE_FHN = range(7)
 SHADE_NONE, SHADE_PHONG = 0, 1
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
-KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL, KERNEL_CELL_PIPE = 0, 1, 2, 3, 4
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL = 0, 1, 2, 3
 
 # every symbol include/hipdvr.h and include/hipdvr_synth.h declare
 EXPORTS = (
