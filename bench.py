@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--shading", default="default", choices=["default", "none", "phong"],
                     help="override the workload's shading (changes the workload name)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-stepping leg")
+    ap.add_argument("--learned-tfs", type=int, default=200, help="random TFs traced for training")
+    ap.add_argument("--learned-views", type=int, default=16, help="orbit views traced per TF")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
@@ -78,6 +81,58 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
         "gsamples_per_s": round(sps / 1e9, 5),
         "sample": (f"rows {y0}..{y0 + rows - 1} of view 0 ({rows}x{en.shape[1]} rays, {n} samples, "
                    f"{dt:.1f} s), no shading; frames/s = samples/s / plain samples per frame"),
+    }
+
+
+def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samples, args):
+    """Learned stepping on the same frames: train the stepping network on
+    plain traces of seeded random TFs (the library's own kernels), then time
+    the learned render of every view (learned march + 1/64 sparse plain
+    re-march + judge + conditional fallback) and compare each frame with the
+    plain one (PSNR over RGBA)."""
+    from voreen_amd import hipdvr as H, stepper
+    t0 = time.perf_counter()
+    mlp, info = stepper.train(ctx, v, dims, n_tfs=args.learned_tfs, n_views=args.learned_views)
+    train_s = time.perf_counter() - t0
+    lr = H.Learned(ctx, v)
+    nviews = len(bufs)
+    dl = [ctx.alloc(wimg * himg * 16) for _ in range(nviews)]
+    try:
+        lr.prepare(t, params[0], mlp)
+        per_view = [lr.render_device(t, de, dx, wimg, himg, p, o, stats=True)
+                    for (de, dx, _), p, o in zip(bufs, params, dl)]
+        for _ in range(args.warmup):
+            for (de, dx, _), p, o in zip(bufs, params, dl):
+                lr.render_device(t, de, dx, wimg, himg, p, o)
+        ctx.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            for (de, dx, _), p, o in zip(bufs, params, dl):
+                lr.render_device(t, de, dx, wimg, himg, p, o)
+        ctx.synchronize()
+        fps = nviews * args.steps / (time.perf_counter() - ta)
+        psnrs = []
+        for (_, _, do), o in zip(bufs, dl):
+            a = do.download((himg, wimg, 4), np.float32).astype(np.float64)
+            b = o.download((himg, wimg, 4), np.float32).astype(np.float64)
+            mse = float(np.mean((a - b) ** 2))
+            psnrs.append(999.0 if mse == 0 else 10 * np.log10(1 / mse))
+    finally:
+        for o in dl:
+            o.free()
+        lr.close()
+        mlp.close()
+    return {
+        "fps": round(fps, 3),
+        "speedup_vs_plain": round(fps / plain_fps, 3),
+        "samples_equiv_gsamples_per_s": round(plain_samples * fps / 1e9, 3),
+        "psnr_min_db": round(min(psnrs), 2),
+        "psnr_mean_db": round(float(np.mean(psnrs)), 2),
+        "views_fell_back": int(sum(s["fell_back"] for s in per_view)),
+        "reconstructed_per_frame": int(sum(s["samples"] for s in per_view) / nviews),
+        "train": {"tfs": info["tfs"], "views": info["views"], "rows": info["rows"], "steps": info["steps"],
+                  "loss_first": round(info["loss_first"], 4), "loss_last": round(info["loss_last"], 4),
+                  "seconds": round(train_s, 2)},
     }
 
 
@@ -267,6 +322,9 @@ def main():
         a.free()
         b.free()
         out["copy_calibration_bytes"] = nb
+    if not args.no_learned:
+        out["learned"] = learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, fps, samples / nviews,
+                                     args)
     if not args.no_cpu:
         de, dx, _ = bufs[0]
         en = de.download((himg, wimg, 4), np.float32)

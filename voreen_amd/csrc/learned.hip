@@ -476,16 +476,14 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
         la.m.cells = vol->d_cells;
         launch_learned(la, phong, count, st);
-        // 2. sparse plain re-march (bricked exact kernel on every stride-th pixel)
+        // 2. sparse plain re-march (the plain cell kernel on every stride-th pixel)
         MarchArgs sp;
         fill_march_args(sp, vol, tf, prm, d_entry, d_exit, (float*)l->d_sparse, w, h, nullptr);
         sp.grid_w = swc;
         sp.grid_h = shc;
         sp.stride = (int32_t)stride;
         sp.pix_offset = (int32_t)offs;
-        const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
-        const bool wide = vol->tiled_elems * vsz > 0xffffff00ull;
-        launch_march_fast(sp, vol->desc.dtype, prm->filter, phong, false, wide, st);
+        launch_march_cell(sp, vol->desc.dtype, phong, false, vol->cells_wide, kDefaultBatch, st);
         // 3. judge: flag = PSNR below the floor
         const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);
         launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)offs,
@@ -494,7 +492,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         MarchArgs fb;
         fill_march_args(fb, vol, tf, prm, d_entry, d_exit, d_out, w, h, nullptr);
         fb.cond_flag = l->d_flag;
-        launch_march_fast(fb, vol->desc.dtype, prm->filter, phong, false, wide, st);
+        launch_march_cell(fb, vol->desc.dtype, phong, false, vol->cells_wide, kDefaultBatch, st);
         hip_check(hipGetLastError(), "learned render launch");
         if (timed) ctx_timing_stop(l->ctx);
         if (count) {
