@@ -151,6 +151,9 @@ typedef struct hipdvr_params {
                                0 = default) */
     float gamma;            /* MIDA gammaValue in [-1, 1] (volumeraycaster.cpp:67) */
     float iso_value;        /* ISO isoValue in [0, 1] (volumeraycaster.cpp:57) */
+    int32_t exact_shading;  /* Phong in the cell kernel: 1 = bit-identical to the reference
+                               kernel; 0 = interior gradient from the sample's own records and
+                               hardware rsq/exp2/log2 lighting (within the oracle tolerance) */
 } hipdvr_params;
 
 typedef struct hipdvr_stats {

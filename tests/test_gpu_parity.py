@@ -382,13 +382,14 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             tf = oracle.tf_build(tfk, 4096)
             t = ctx.transfunc(tf)
             outs = {}
-            runs = [("ref", H.KERNEL_REFERENCE, 0), ("brick", H.KERNEL_BRICKED, 0)]
-            runs += [(f"cell{b}", H.KERNEL_CELL, b) for b in (2, 4, 8)]
-            for name, k, batch in runs:
+            runs = [("ref", H.KERNEL_REFERENCE, 0, 0), ("brick", H.KERNEL_BRICKED, 0, 0)]
+            runs += [(f"cell{b}", H.KERNEL_CELL, b, 1) for b in (2, 4, 8)]
+            runs += [("cellfast", H.KERNEL_CELL, 4, 0)]
+            for name, k, batch, exact in runs:
                 for shading in (0, 1):
                     say(f"bricked-vs-reference tf{seed} {name} shading {shading}")
                     p = H.default_params(step, shading=shading, kernel=k, camera_pos=tuple(cam.eye),
-                                         batch=batch)
+                                         batch=batch, exact_shading=exact)
                     outs[(name, shading)] = ctx.render(v, t, en, ex, p)
             t.close()
             for shading in (0, 1):
@@ -398,6 +399,13 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
                     assert np.array_equal(a, c), f"{name} tf{seed} shading {shading}"
                     assert sa["samples"] == sc["samples"] and sa["shaded"] == sc["shaded"]
                     assert sc["fetched"] < sa["fetched"] or seed == 1
+                c, sc = outs[("cellfast", shading)]
+                if shading == 0:
+                    assert np.array_equal(a, c)
+                else:
+                    d = np.abs(a - c).max(axis=2)
+                    print(f"tf{seed} phong cell-fast-vs-reference max {d.max():.3g}", flush=True)
+                    assert np.mean(d > TOL) < 0.002
                 b, sb = outs[("brick", shading)]
                 if shading == 0:
                     assert np.array_equal(a, b), f"tf{seed} shading {shading}"

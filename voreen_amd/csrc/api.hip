@@ -211,7 +211,8 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     } else if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else if (kernel == HIPDVR_KERNEL_CELL) {
-        launch_march_cell(a, vol->desc.dtype, phong, count, mvol->cells_wide, p->batch ? p->batch : kDefaultBatch,
+        launch_march_cell(a, vol->desc.dtype, phong ? (p->exact_shading ? 2 : 1) : 0, count, mvol->cells_wide,
+                          p->batch ? p->batch : kDefaultBatch,
                           ctx->stream);
 
     } else {

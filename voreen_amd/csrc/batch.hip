@@ -38,7 +38,7 @@ __global__ void batch_reset_kernel(uint32_t* counter) {
     if (threadIdx.x == 0) counter[0] = 0;
 }
 
-template <bool PHONG, bool COUNT, int K, int TILE>
+template <int SHADE, bool COUNT, int K, int TILE>
 __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict__ jobs,
                                                     const uint32_t* __restrict__ tile_start, uint32_t njobs,
                                                     uint32_t total_tiles, uint32_t* __restrict__ counter,
@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
             if (!is_background(e, l)) {
                 if (a.dtype == 0) {
                     const CellFetch<uint8_t, false> f = cell_fetch_of<uint8_t, false>(a);
-                    res = cell_ray<uint8_t, false, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                    res = cell_ray<uint8_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
                 } else {
                     const CellFetch<uint16_t, false> f = cell_fetch_of<uint16_t, false>(a);
-                    res = cell_ray<uint16_t, false, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                    res = cell_ray<uint16_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
                 }
             }
             a.out[pix] = res;
@@ -105,21 +105,22 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
 typedef void (*BatchKernel)(const MarchArgs*, const uint32_t*, uint32_t, uint32_t, uint32_t*,
                             unsigned long long*);
 
-template <bool PHONG, bool COUNT>
+template <int SHADE, bool COUNT>
 BatchKernel pick_kernel(int tile, int k) {
     if (k <= 2) {
-        if (tile == 0) return batch_kernel<PHONG, COUNT, 2, 0>;
-        if (tile == 1) return batch_kernel<PHONG, COUNT, 2, 1>;
-        return batch_kernel<PHONG, COUNT, 2, 2>;
+        if (tile == 0) return batch_kernel<SHADE, COUNT, 2, 0>;
+        if (tile == 1) return batch_kernel<SHADE, COUNT, 2, 1>;
+        return batch_kernel<SHADE, COUNT, 2, 2>;
     }
-    if (tile == 0) return batch_kernel<PHONG, COUNT, 4, 0>;
-    if (tile == 1) return batch_kernel<PHONG, COUNT, 4, 1>;
-    return batch_kernel<PHONG, COUNT, 4, 2>;
+    if (tile == 0) return batch_kernel<SHADE, COUNT, 4, 0>;
+    if (tile == 1) return batch_kernel<SHADE, COUNT, 4, 1>;
+    return batch_kernel<SHADE, COUNT, 4, 2>;
 }
 
-BatchKernel kernel_for(bool phong, bool count, int tile, int k) {
-    if (phong) return count ? pick_kernel<true, true>(tile, k) : pick_kernel<true, false>(tile, k);
-    return count ? pick_kernel<false, true>(tile, k) : pick_kernel<false, false>(tile, k);
+BatchKernel kernel_for(int shade, bool count, int tile, int k) {
+    if (shade == 2) return count ? pick_kernel<2, true>(tile, k) : pick_kernel<2, false>(tile, k);
+    if (shade == 1) return count ? pick_kernel<1, true>(tile, k) : pick_kernel<1, false>(tile, k);
+    return count ? pick_kernel<0, true>(tile, k) : pick_kernel<0, false>(tile, k);
 }
 
 }  // namespace
@@ -131,7 +132,7 @@ using namespace hipdvr;
 struct hipdvr_batch {
     hipdvr_ctx* ctx = nullptr;
     std::vector<hipdvr_batch_job> jobs;    // in launch order (largest first)
-    bool phong = false;
+    int shade = 0;                         // 0 none, 1 Phong, 2 exact Phong
     int sms = 0;
     // per brick size: device brick tables per job and the jobs' MarchArgs
     std::vector<void*> allocations;
@@ -177,7 +178,7 @@ void validate_config(const hipdvr_batch_config& c) {
 void launch_batch(hipdvr_batch* b, bool count) {
     hipStream_t st = ctx_stream(b->ctx);
     const int t = b->cfg.tile, bs = b->cfg.brick_log2 - 3;
-    BatchKernel k = kernel_for(b->phong, count, t, b->cfg.batch);
+    BatchKernel k = kernel_for(b->shade, count, t, b->cfg.batch);
     int per_cu = 0;
     hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 256, 0), "occupancy query");
     if (per_cu < 1) per_cu = 1;
@@ -208,7 +209,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             if (j.params.filter != HIPDVR_FILTER_LINEAR || j.params.classification != HIPDVR_CLASS_TF ||
                 j.params.compositing != HIPDVR_COMPOSITE_DVR)
                 fail(HIPDVR_ERR_UNSUPPORTED, "batched jobs use the linear filter, TF classification and DVR");
-            const int ph = j.params.shading == HIPDVR_SHADE_PHONG ? 1 : 0;
+            const int ph = j.params.shading == HIPDVR_SHADE_PHONG ? (j.params.exact_shading ? 2 : 1) : 0;
             if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading mode");
             shade = ph;
             if (cell_record_bytes(j.volume) > 0xffffff00ull)
@@ -218,7 +219,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
         hipStream_t st = ctx_stream(ctx);
         hipdvr_batch* b = new hipdvr_batch();
         b->ctx = ctx;
-        b->phong = shade == 1;
+        b->shade = shade;
         try {
             int dev = 0;
             hip_check(hipGetDevice(&dev), "hipGetDevice");

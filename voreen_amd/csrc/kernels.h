@@ -25,8 +25,10 @@ void launch_march_ref(const MarchArgs& a, int dtype, int filter, int classif, bo
 void launch_march_modes(const MarchArgs& a, int dtype, int filter, bool phong, hipStream_t st);
 void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bool count, bool wide,
                        hipStream_t st);
-// batch: samples fetched together per ray (2, 4 or 8)
-void launch_march_cell(const MarchArgs& a, int dtype, bool phong, bool count, bool wide, int batch,
+// shade: 0 none, 1 Phong (interior gradient, approximate lighting), 2 Phong
+// bit-identical to the reference kernel; batch: samples fetched together per
+// ray (2 or 4)
+void launch_march_cell(const MarchArgs& a, int dtype, int shade, bool count, bool wide, int batch,
                        hipStream_t st);
 constexpr int kDefaultBatch = 4;
 

@@ -467,6 +467,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
             l->sparse_cap = need;
         }
         const bool phong = prm->shading == HIPDVR_SHADE_PHONG;
+        const int shade = phong ? (prm->exact_shading ? 2 : 1) : 0;
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(l->ctx);
@@ -483,7 +484,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         sp.grid_h = shc;
         sp.stride = (int32_t)stride;
         sp.pix_offset = (int32_t)offs;
-        launch_march_cell(sp, vol->desc.dtype, phong, false, vol->cells_wide, kDefaultBatch, st);
+        launch_march_cell(sp, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, st);
         // 3. judge: flag = PSNR below the floor
         const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);
         launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)offs,
@@ -492,7 +493,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         MarchArgs fb;
         fill_march_args(fb, vol, tf, prm, d_entry, d_exit, d_out, w, h, nullptr);
         fb.cond_flag = l->d_flag;
-        launch_march_cell(fb, vol->desc.dtype, phong, false, vol->cells_wide, kDefaultBatch, st);
+        launch_march_cell(fb, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, st);
         hip_check(hipGetLastError(), "learned render launch");
         if (timed) ctx_timing_stop(l->ctx);
         if (count) {

@@ -132,16 +132,79 @@ __device__ __forceinline__ void cell_gradient(const CellFetch<T, WIDE>& f, const
     for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
 }
 
+// Interior central differences from the cell records: when every axis has
+// u_i in [1, dim_i - 3) and a proper fraction, the six lookups at +-1 voxel
+// have the sample's own weights p, and their footprints are the sample's
+// corner planes plus the outer planes lo-1 and lo+2 (records of the cells
+// lo -+ e_i).  The same arithmetic as the bricked kernel's interior path;
+// equal to the six-lookup form up to float rounding.  Returns false (and
+// leaves gr) outside the interior.
+template <typename T, bool WIDE>
+__device__ __forceinline__ bool cell_gradient_interior(const CellFetch<T, WIDE>& f, const MarchArgs& a,
+                                                       const float s[3], const float v[8], const float p[3],
+                                                       float gr[3]) {
+    uint32_t lo[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float u = (a.fdims[i] * s[i]) + (-0.5f);
+        if (!(u >= 1.0f) || !(u + 3.0f < a.fdims[i]) || !(p[i] > 0.0f)) return false;
+        lo[i] = (uint32_t)u;
+    }
+    typename CellFetch<T, WIDE>::rec_t rec[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        uint32_t c[3] = {lo[0], lo[1], lo[2]};
+        c[k % 3] = k < 3 ? c[k % 3] - 1 : c[k % 3] + 1;
+        rec[k] = f.load(f.index(c));
+    }
+    float m[3][8], q[3][8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        CellRecord<T>::unpack(rec[i], m[i]);
+        CellRecord<T>::unpack(rec[3 + i], q[i]);
+    }
+    const float px = p[0], py = p[1], pz = p[2];
+    const float qx = 1.f - px, qy = 1.f - py, qz = 1.f - pz;
+    // corner order: 0 x0y0z0, 1 x1y0z0, 2 x1y1z0, 3 x0y1z0, 4 x0y0z1, 5 x1y0z1, 6 x1y1z1, 7 x0y1z1
+    {
+        const float w[4] = {qy * qz, py * qz, qy * pz, py * pz};
+        const float bm = m[0][0] * w[0] + m[0][3] * w[1] + m[0][4] * w[2] + m[0][7] * w[3];
+        const float b0 = v[0] * w[0] + v[3] * w[1] + v[4] * w[2] + v[7] * w[3];
+        const float b1 = v[1] * w[0] + v[2] * w[1] + v[5] * w[2] + v[6] * w[3];
+        const float bp = q[0][1] * w[0] + q[0][2] * w[1] + q[0][5] * w[2] + q[0][6] * w[3];
+        gr[0] = (((qx * bm + px * b0) - (qx * b1 + px * bp)) * a.grad_scale[0]) * a.rwm_scale;
+    }
+    {
+        const float w[4] = {qx * qz, px * qz, qx * pz, px * pz};
+        const float bm = m[1][0] * w[0] + m[1][1] * w[1] + m[1][4] * w[2] + m[1][5] * w[3];
+        const float b0 = v[0] * w[0] + v[1] * w[1] + v[4] * w[2] + v[5] * w[3];
+        const float b1 = v[3] * w[0] + v[2] * w[1] + v[7] * w[2] + v[6] * w[3];
+        const float bp = q[1][3] * w[0] + q[1][2] * w[1] + q[1][7] * w[2] + q[1][6] * w[3];
+        gr[1] = (((qy * bm + py * b0) - (qy * b1 + py * bp)) * a.grad_scale[1]) * a.rwm_scale;
+    }
+    {
+        const float w[4] = {qx * qy, px * qy, qx * py, px * py};
+        const float bm = m[2][0] * w[0] + m[2][1] * w[1] + m[2][3] * w[2] + m[2][2] * w[3];
+        const float b0 = v[0] * w[0] + v[1] * w[1] + v[3] * w[2] + v[2] * w[3];
+        const float b1 = v[4] * w[0] + v[5] * w[1] + v[7] * w[2] + v[6] * w[3];
+        const float bp = q[2][4] * w[0] + q[2][5] * w[1] + q[2][7] * w[2] + q[2][6] * w[3];
+        gr[2] = (((qz * bm + pz * b0) - (qz * b1 + pz * bp)) * a.grad_scale[2]) * a.rwm_scale;
+    }
+    return true;
+}
+
 // Per-ray counters of a march (COUNT builds).
 struct RayCounters {
     unsigned long long rays = 0, samples = 0, fetched = 0, shaded = 0, slots = 0, shade_slots = 0;
 };
 
 // One ray of the cell march from entry point e to exit point l (neither the
-// background): returns the composited RGBA.  s_tf / s_alpha are the TF and
+// background): returns the composited RGBA.  SHADE: 0 none, 1 Phong with the
+// interior gradient and the hardware-approximate lighting terms (within the
+// oracle's tolerance), 2 Phong bit-identical to the reference-layout kernel.  s_tf / s_alpha are the TF and
 // the opacity-correction table in LDS; a may live in kernel arguments or in
 // global memory (batched launches).
-template <typename T, bool WIDE, bool PHONG, bool COUNT, int K>
+template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
                                            const float* s_alpha, int px, int py, float4 e, float4 l,
                                            RayCounters& n) {
@@ -232,19 +295,21 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
             if (ca > 0.0f) {
                 float col[3] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu),
                                 byte_norm((c8 >> 16) & 0xffu)};
-                if (PHONG) {
+                if (SHADE) {
                     shade_now = true;
                     const float tt = tk[k];
                     const float ss[3] = {first[0] + tt * dir[0], first[1] + tt * dir[1],
                                          first[2] + tt * dir[2]};
                     float gr[3];
                     if (tt == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                    else cell_gradient<T, WIDE>(f, a, ss, gr);
+                    else if (SHADE == 2 || !cell_gradient_interior<T, WIDE>(f, a, ss, v, pk[k], gr))
+                        cell_gradient<T, WIDE>(f, a, ss, gr);
                     float pos[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
                         pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                    phong(a, gr, pos, col);
+                    if (SHADE == 2) phong(a, gr, pos, col);
+                    else phong_fast(a, gr, pos, col);
                     ++n.shaded;
                 }
                 const float wgt = (1.0f - A) * ca;

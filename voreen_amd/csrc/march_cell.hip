@@ -7,7 +7,7 @@
 
 namespace hipdvr {
 
-template <typename T, bool WIDE, bool PHONG, bool COUNT, int K>
+template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, const uint32_t tiles_x,
                                                          const uint32_t tiles_total,
                                                          const uint32_t tiles_per_xcd) {
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
         const float4 e = a.entry[pix];
         const float4 l = a.exit[pix];
         float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!is_background(e, l)) res = cell_ray<T, WIDE, PHONG, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+        if (!is_background(e, l)) res = cell_ray<T, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
         a.out[(size_t)gy * a.grid_w + gx] = res;
     }
     if (COUNT) {
@@ -49,41 +49,41 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
     }
 }
 
-template <typename T, bool WIDE, bool PHONG, int K>
+template <typename T, bool WIDE, int SHADE, int K>
 static void launch_k(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
     const uint32_t total = tx * ty;
     const uint32_t per_xcd = (total + 7) / 8;
     const dim3 grid(per_xcd * 8);
     if (count)
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, PHONG, true, K>), grid, dim3(256), 0, st, a, tx, total,
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), 0, st, a, tx, total,
                            per_xcd);
     else
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, PHONG, false, K>), grid, dim3(256), 0, st, a, tx, total,
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, false, K>), grid, dim3(256), 0, st, a, tx, total,
                            per_xcd);
+}
+
+template <typename T, bool WIDE, int SHADE>
+static void launch_ks(const MarchArgs& a, bool count, int batch, hipStream_t st) {
+    if (batch <= 2) launch_k<T, WIDE, SHADE, 2>(a, count, st);
+    else launch_k<T, WIDE, SHADE, 4>(a, count, st);
 }
 
 template <typename T, bool WIDE>
-static void launch_cw(const MarchArgs& a, bool phong, bool count, int batch, hipStream_t st) {
-    if (phong) {
-        if (batch <= 2) launch_k<T, WIDE, true, 2>(a, count, st);
-        else if (batch <= 4) launch_k<T, WIDE, true, 4>(a, count, st);
-        else launch_k<T, WIDE, true, 8>(a, count, st);
-    } else {
-        if (batch <= 2) launch_k<T, WIDE, false, 2>(a, count, st);
-        else if (batch <= 4) launch_k<T, WIDE, false, 4>(a, count, st);
-        else launch_k<T, WIDE, false, 8>(a, count, st);
-    }
+static void launch_cw(const MarchArgs& a, int shade, bool count, int batch, hipStream_t st) {
+    if (shade == 2) launch_ks<T, WIDE, 2>(a, count, batch, st);
+    else if (shade == 1) launch_ks<T, WIDE, 1>(a, count, batch, st);
+    else launch_ks<T, WIDE, 0>(a, count, batch, st);
 }
 
-void launch_march_cell(const MarchArgs& a, int dtype, bool phong, bool count, bool wide, int batch,
+void launch_march_cell(const MarchArgs& a, int dtype, int shade, bool count, bool wide, int batch,
                        hipStream_t st) {
     if (dtype == 0) {
-        if (wide) launch_cw<uint8_t, true>(a, phong, count, batch, st);
-        else launch_cw<uint8_t, false>(a, phong, count, batch, st);
+        if (wide) launch_cw<uint8_t, true>(a, shade, count, batch, st);
+        else launch_cw<uint8_t, false>(a, shade, count, batch, st);
     } else {
-        if (wide) launch_cw<uint16_t, true>(a, phong, count, batch, st);
-        else launch_cw<uint16_t, false>(a, phong, count, batch, st);
+        if (wide) launch_cw<uint16_t, true>(a, shade, count, batch, st);
+        else launch_cw<uint16_t, false>(a, shade, count, batch, st);
     }
 }
 

@@ -66,7 +66,7 @@ class Params(C.Structure):
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
                 ("loop_cap", C.c_int32), ("kernel", C.c_int32), ("batch", C.c_int32),
-                ("gamma", C.c_float), ("iso_value", C.c_float)]
+                ("gamma", C.c_float), ("iso_value", C.c_float), ("exact_shading", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -252,7 +252,7 @@ def synth_shells(seed, dims):
 
 def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading=SHADE_NONE,
                    kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5), batch=0,
-                   compositing=COMPOSITE_DVR, gamma=0.0, iso_value=0.5):
+                   compositing=COMPOSITE_DVR, gamma=0.0, iso_value=0.5, exact_shading=0):
     """Render parameters with Voreen's VolumeRenderer light defaults
     (src/core/processors/volumerenderer.cpp:41-54)."""
     p = Params()
@@ -262,6 +262,7 @@ def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading
     p.compositing = compositing
     p.gamma = gamma
     p.iso_value = iso_value
+    p.exact_shading = exact_shading
     p.shading = shading
     lt = dict(pos=(2.3, 1.5, 1.5), ambient=(0.4,) * 3, diffuse=(0.8,) * 3, specular=(0.6,) * 3,
               shininess=60.0)

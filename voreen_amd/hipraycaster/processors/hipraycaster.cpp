@@ -237,6 +237,7 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
              : k == "cell"      ? HIPDVR_KERNEL_CELL
                                 : HIPDVR_KERNEL_AUTO;
     p.batch = 0;
+    p.exact_shading = 0;
 }
 
 // Output exactly as CPURaycaster::process (cpuraycaster.cpp:157-159,197-207).
