@@ -141,6 +141,8 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.dom_lo = tf->domain_lo;
     a.dom_hi = tf->domain_hi;
     a.dom_unit = (tf->domain_hi - tf->domain_lo) == 1.0f ? 1 : 0;
+    a.tf_identity = (v.rwm_scale == 1.0f && v.rwm_offset == 0.0f && tf->domain_lo == 0.0f && tf->domain_hi == 1.0f)
+                        ? 1 : 0;
     a.tf_width = (int32_t)tf->width;
     a.tf = tf->d_rgba;
     a.preint = tf->d_preint;

@@ -94,14 +94,45 @@ template <typename T, bool WIDE>
 __device__ __forceinline__ typename CellFetch<T, WIDE>::off_t cell_of(const CellFetch<T, WIDE>& f,
                                                                       const MarchArgs& a, const float s[3],
                                                                       float p[3]) {
-    float vp[3];
+    // corners() of volumeram.cpp:79-86 on a clamped position (vp >= 0, so
+    // max(vp, 0) == vp): the record holds the clamped upper corners, so only
+    // the lower corner and the fraction are needed
+    uint32_t lo[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
-    Corners c;
-    corners(vp, a.dims, c);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) p[i] = c.p[i];
-    return f.index(c.lo);
+    for (int i = 0; i < 3; ++i) {
+        const float vp = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+        p[i] = vp - floorf(vp);
+        const uint32_t l = (uint32_t)vp, m = a.dims[i] - 1;
+        lo[i] = l < m ? l : m;
+    }
+    return f.index(lo);
+}
+
+// The reference loop's float sequence t_{k+1} = t_k + step, k steps at once:
+// inside one binade [2^e, 2^(e+1)) every addition of step rounds by the same
+// amount (t is on the binade's ulp grid, so (t + step) mod ulp is the same for
+// every t), hence t_k = t_0 + k*d with d = (t_0 + step) - t_0, exactly, while
+// each sum t_k + step stays below 2^(e+1).  The one exception, a tie resolved
+// to even, would alternate the increment; it is excluded by requiring the
+// second increment to equal the first.  Returns a number of steps j >= 0 that
+// may be taken as t + j*d: every t_k, k <= j, stays below bound and within
+// cap_left - 1 steps, with margins of a few steps for the float estimates.
+// The product and the sum are exact because t + j*d < 2^(e+1) lies on the
+// binade's ulp grid.
+__device__ __forceinline__ int closed_form_steps(float t, float step, float bound, int cap_left, float& d) {
+    if (!(t > 0.0f) || cap_left < 8) return 0;
+    const float B = __int_as_float((__float_as_int(t) & 0x7f800000) + 0x00800000);
+    d = (t + step) - t;
+    const float t1 = t + d;
+    if (((t1 + step) - t1) != d) return 0;
+    const float room = (B - step - t) / d;      // steps k with t_k + step < B: k < room
+    const float est = (bound - t) / d;          // steps k with t_k < bound: k < est
+    if (!(room > 6.0f) || !(est > 6.0f)) return 0;
+    const int kr = room > 1.0e6f ? 1000000 : (int)room - 3;
+    const int ke = est > 1.0e6f ? 1000000 : (int)est - 3;
+    int j = kr < ke ? kr : ke;
+    j = j < cap_left - 3 ? j : cap_left - 3;
+    return j > 0 ? j : 0;
 }
 
 // Central differences of six trilinear lookups at s +- one voxel per axis,
@@ -240,6 +271,18 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
             // the count and the float sequence t += step remain
             ++n.samples;
             if (COUNT) ++n.slots;
+            {
+                // most of the run in closed form; the last steps below take
+                // the reference's float sequence step by step
+                float d = 0.0f;
+                const int j = closed_form_steps(t, a.step, fminf(t_lim, tend), cap - loop, d);
+                if (j > 0) {
+                    t = t + (float)j * d;
+                    loop += j;
+                    n.samples += (unsigned long long)j;
+                    if (COUNT) n.slots += (unsigned long long)j;
+                }
+            }
             float tn = t + a.step;
             while (tn < t_lim && !(tn > tend) && loop + 1 < cap) {
                 t = tn;

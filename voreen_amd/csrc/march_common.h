@@ -39,6 +39,7 @@ struct MarchArgs {
     float dom_lo, dom_hi;
     int32_t dom_unit;           // dom_hi - dom_lo == 1: the division is exact, x = rw - lo
     int32_t dtype;              // 0 uint8, 1 uint16
+    int32_t tf_identity;        // rwm (1, 0) and TF domain [0, 1]: x = min(intensity, 1)
     int32_t tf_width;
     const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
     const float4* preint;       // N*N, [fe*N + fs]
@@ -154,6 +155,12 @@ __device__ __forceinline__ float rw_to_norm_unit(float rw, float lo, float hi) {
 
 // Intensity -> TF table index (cpuraycaster.cpp:254,266 and :328-332).
 __device__ __forceinline__ int tf_index(float inten, const MarchArgs& a) {
+    if (a.tf_identity) {
+        // rw = inten*1 + 0 == inten; intensities are >= 0, so the mapping
+        // reduces to the upper clamp, and x*(W-1) needs no index clamp
+        const float x = inten > 1.0f ? 1.0f : inten;
+        return iround_tgt(x * (float)(a.tf_width - 1));
+    }
     float rw = (inten * a.rwm_scale) + a.rwm_offset;
     float x = a.dom_unit ? rw_to_norm_unit(rw, a.dom_lo, a.dom_hi) : rw_to_norm(rw, a.dom_lo, a.dom_hi);
     int w1 = a.tf_width - 1;
