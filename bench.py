@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--jobs", type=int, default=64, help="c2: number of volumes")
     ap.add_argument("--no-autotune", action="store_true", help="c2: keep the default configuration")
     ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
+    ap.add_argument("--brick-log2", type=int, default=4, choices=[2, 3, 4, 5],
+                    help="empty-space brick size of the volume (2^n voxels)")
     ap.add_argument("--batch", type=int, default=0, choices=[0, 2, 4, 8],
                     help="cell kernel: samples fetched together per ray (0 = library default)")
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
@@ -227,7 +229,7 @@ def main():
     vsz = 1 if vol.dtype == np.uint8 else 2
 
     ctx = H.Context(0)
-    v = ctx.volume(vol, spacing=spacing, offset=offset)
+    v = ctx.volume(vol, spacing=spacing, offset=offset, brick_log2=args.brick_log2)
     t = ctx.transfunc(tf, (0.0, 1.0))
     img_bytes = wimg * himg * 16
     cams = [W.orbit_camera(i, nviews) for i in range(nviews)]
@@ -294,6 +296,7 @@ def main():
                          f"{'phong' if cfg['shading'] else 'no shading'}, {nviews} orbit views per step"),
             "kernel": args.kernel,
             "batch": args.batch,
+            "brick": 1 << args.brick_log2,
             "voxel": "u8" if vsz == 1 else "u16",
         },
         "gsamples_per_s": round(samples * args.steps / elapsed / 1e9, 4),
