@@ -57,6 +57,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(config, shading):
+    """HBM bytes per launch of the march kernel measured with rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) for this workload, from the
+    committed profile summary; None when there is none."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == config and f[1] == shading:
+            return {"bytes": int(f[2]), "source": f[3]}
+    return None
+
+
 def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     """Time the oracle's CPURaycaster restatement (no shading) on a band of
     rows at the image centre; scale by the plain sample count of a frame."""
@@ -277,6 +291,9 @@ def main():
     bytes_orbit = fetched * 8 * vsz + shaded * 24 * vsz + 48 * wimg * himg * nviews
     kernel_s = sum(launch_ms) / 1e3
     achieved = bytes_orbit * args.steps / kernel_s / 1e9
+    traffic = None
+    if args.kernel == "auto" and args.batch == 0 and args.brick_log2 == 4:
+        traffic = measured_traffic(args.config, "phong" if cfg["shading"] else "none")
     out = {
         "metric": "frames/s + Gsamples/s at 1024^2, 512^3 u16; learned vs plain at >=45 dB PSNR",
         "value": round(fps, 4),
@@ -313,10 +330,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
         },
         "setup_s": round(setup_s, 2),
     }
+    if traffic:
+        out["roofline"]["traffic_source"] = traffic["source"]
+        out["roofline"]["algorithmic_bytes_per_launch"] = round(bytes_orbit / nviews)
     if args.copy_calibration:
         nb = args.copy_calibration << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
