@@ -428,6 +428,10 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
             float t = 0.f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
             int guard = 0;
             bool finished = false;
+            uint32_t cur = 0xffffffffu;   // brick of the cached table entry
+            float t_exit = 0.f;
+            int mmax = 1;
+            bool skip = false;
             while (!finished && !(t > tend) && guard < a.loop_cap) {
                 float s[3], vp[3];
                 for (int i = 0; i < 3; ++i) {
@@ -437,9 +441,14 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                 uint32_t bc[3];
                 for (int i = 0; i < 3; ++i) bc[i] = (uint32_t)vp[i] >> kLBrickLog2;
                 const uint32_t brick = (bc[2] * la.nb16[1] + bc[1]) * la.nb16[0] + bc[0];
-                const uint8_t ent = la.table[brick * 8 + oct];
-                const float t_exit = box_exit(a, first, dir, bc, kLBrick, la.nb16);
-                if ((ent & 0x80) || !la.occ16[brick]) {
+                if (brick != cur) {
+                    cur = brick;
+                    const uint8_t ent = la.table[brick * 8 + oct];
+                    t_exit = box_exit(a, first, dir, bc, kLBrick, la.nb16);
+                    skip = (ent & 0x80) || !la.occ16[brick];
+                    mmax = 1 << (ent & 3);
+                }
+                if (skip) {
                     // skip to the last step-grid point inside the brick; the
                     // next pass finds the following brick one step on
                     const float k = floorf((t_exit - t) / a.step);
@@ -447,7 +456,6 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                     ++guard;
                     continue;
                 }
-                const int mmax = 1 << (ent & 3);
                 // up to K samples of this brick, their loads in flight together
                 float tk[K], pk[K][3];
                 int mk[K];
@@ -489,7 +497,8 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                                                  first[2] + t0 * dir[2]};
                             float gr[3];
                             if (t0 == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                            else cell_gradient<T, false>(f, a, ss, gr);
+                            else if (!cell_gradient_interior<T, false>(f, a, ss, v, pk[k], gr))
+                                cell_gradient<T, false>(f, a, ss, gr);
                             float pos[3];
                             for (int i = 0; i < 3; ++i)
                                 pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
