@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
             if (!is_background(e, l)) {
                 if (a.dtype == 0) {
                     const CellFetch<uint8_t, false> f = cell_fetch_of<uint8_t, false>(a);
-                    res = cell_ray<uint8_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                    res = cell_ray<uint8_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
                 } else {
                     const CellFetch<uint16_t, false> f = cell_fetch_of<uint16_t, false>(a);
-                    res = cell_ray<uint16_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+                    res = cell_ray<uint16_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
                 }
             }
             a.out[pix] = res;

@@ -41,6 +41,7 @@ void launch_features(const FeatureArgs& a, hipStream_t st);
 void launch_table(const float* y, uint32_t rows, uint8_t* table, float skip_logit, int max_log2,
                   hipStream_t st);
 void launch_fill_table(uint8_t* table, uint32_t rows, uint8_t v, hipStream_t st);
+void launch_merge_occ(uint8_t* table, const uint8_t* occ16, uint32_t rows, hipStream_t st);
 void launch_trace(const LearnedArgs& la, float* err, unsigned int* visits, hipStream_t st);
 void launch_targets(const float* err, const unsigned int* visits, const float* feat, uint32_t rows,
                     float tau_skip, float tau_step, float* out_x, float* out_t, unsigned int* count,

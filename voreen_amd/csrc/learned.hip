@@ -387,6 +387,7 @@ int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_
         } else {
             launch_fill_table(l->d_table, l->rows, 0, st);
         }
+        launch_merge_occ(l->d_table, l->d_occ16, l->rows, st);
         hip_check(hipGetLastError(), "learned prepare launch");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     });

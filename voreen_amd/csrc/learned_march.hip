@@ -156,6 +156,17 @@ void launch_fill_table(uint8_t* table, uint32_t rows, uint8_t v, hipStream_t st)
     hipLaunchKernelGGL(fill_table_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, table, rows, v);
 }
 
+// bricks whose value range maps to zero alpha are skipped whatever the table
+// says (exact), so the marcher reads one byte per brick
+__global__ void merge_occ_kernel(uint8_t* table, const uint8_t* __restrict__ occ16, uint32_t rows) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < rows && !occ16[row >> 3]) table[row] = (uint8_t)(table[row] | 0x80);
+}
+
+void launch_merge_occ(uint8_t* table, const uint8_t* occ16, uint32_t rows, hipStream_t st) {
+    hipLaunchKernelGGL(merge_occ_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, table, occ16, rows);
+}
+
 // ---------------------------------------------------------------------------
 // shared march helpers (linear filter, tiled volume, TF classification)
 // ---------------------------------------------------------------------------
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                     cur = brick;
                     const uint8_t ent = la.table[brick * 8 + oct];
                     t_exit = box_exit(a, first, dir, bc, kLBrick, la.nb16);
-                    skip = (ent & 0x80) || !la.occ16[brick];
+                    skip = (ent & 0x80) != 0;
                     mmax = 1 << (ent & 3);
                 }
                 if (skip) {

@@ -229,6 +229,10 @@ struct RayCounters {
     unsigned long long rays = 0, samples = 0, fetched = 0, shaded = 0, slots = 0, shade_slots = 0;
 };
 
+// Occupancy bit words staged in LDS by the per-frame kernel up to this count
+// (4 KiB: 32768 bricks, e.g. 512^3 at 16^3 bricks).
+constexpr uint32_t kMaxLdsOccWords = 1024;
+
 // One ray of the cell march from entry point e to exit point l (neither the
 // background): returns the composited RGBA.  SHADE: 0 none, 1 Phong with the
 // interior gradient and the hardware-approximate lighting terms (within the
@@ -237,8 +241,8 @@ struct RayCounters {
 // global memory (batched launches).
 template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
-                                           const float* s_alpha, int px, int py, float4 e, float4 l,
-                                           RayCounters& n) {
+                                           const float* s_alpha, const uint32_t* s_occ, int px, int py, float4 e,
+                                           float4 l, RayCounters& n) {
     typedef typename CellFetch<T, WIDE>::rec_t rec_t;
     const uint32_t bl = a.brick_log2;
     const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
@@ -263,7 +267,8 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
                                ((uint32_t)vp[0] >> bl);
         if (brick != cur_brick) {
             cur_brick = brick;
-            cur_occ = a.brick_occ[brick] != 0;
+            // occupancy bits staged in LDS when they fit, else the byte table
+            cur_occ = s_occ ? ((s_occ[brick >> 5] >> (brick & 31)) & 1u) != 0 : a.brick_occ[brick] != 0;
             if (!cur_occ) t_lim = empty_brick_limit(a, first, dir, vp, bl);
         }
         if (!cur_occ) {

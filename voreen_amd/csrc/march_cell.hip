@@ -19,9 +19,13 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
     if (a.cond_flag && *a.cond_flag == 0) return;
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[256];
+    extern __shared__ uint32_t s_occ[];
     const int tid = threadIdx.x;
     for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
     s_alpha[tid] = a.alpha_corr[tid];
+    const bool occ_lds = a.occ_words <= kMaxLdsOccWords;
+    if (occ_lds)
+        for (uint32_t i = tid; i < a.occ_words; i += 256) s_occ[i] = a.occ_bits[i];
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
@@ -36,7 +40,8 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
         const float4 e = a.entry[pix];
         const float4 l = a.exit[pix];
         float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!is_background(e, l)) res = cell_ray<T, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, px, py, e, l, n);
+        if (!is_background(e, l)) res = cell_ray<T, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, occ_lds ? s_occ : nullptr,
+                                                                          px, py, e, l, n);
         a.out[(size_t)gy * a.grid_w + gx] = res;
     }
     if (COUNT) {
@@ -55,11 +60,12 @@ static void launch_k(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t total = tx * ty;
     const uint32_t per_xcd = (total + 7) / 8;
     const dim3 grid(per_xcd * 8);
+    const size_t shm = a.occ_words <= kMaxLdsOccWords ? (size_t)a.occ_words * 4 : 0;
     if (count)
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), 0, st, a, tx, total,
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), shm, st, a, tx, total,
                            per_xcd);
     else
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, false, K>), grid, dim3(256), 0, st, a, tx, total,
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, false, K>), grid, dim3(256), shm, st, a, tx, total,
                            per_xcd);
 }
 
