@@ -216,6 +216,10 @@ struct hipdvr_learned {
     float* d_sse = nullptr;
     int* d_flag = nullptr;
     unsigned int* d_count = nullptr;
+    // the sparse plain re-march runs beside the learned march on a second
+    // stream (default priority): its few rays would leave the GPU idle alone
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -237,6 +241,9 @@ void free_learned(hipdvr_learned* l) {
     (void)hipFree(l->d_sse);
     (void)hipFree(l->d_flag);
     (void)hipFree(l->d_count);
+    if (l->fork) (void)hipEventDestroy(l->fork);
+    if (l->join) (void)hipEventDestroy(l->join);
+    if (l->side) (void)hipStreamDestroy(l->side);
     delete l;
 }
 
@@ -336,6 +343,9 @@ int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_lear
             hip_check(hipMalloc((void**)&l->d_sse, 64), "hipMalloc");
             hip_check(hipMalloc((void**)&l->d_flag, 64), "hipMalloc");
             hip_check(hipMalloc((void**)&l->d_count, 64), "hipMalloc");
+            hip_check(hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&l->fork, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&l->join, hipEventDisableTiming), "hipEventCreate");
             hipStream_t st = ctx_stream(ctx);
             launch_brick_stats(vol->d_linear, vol->desc.dtype, vol->desc.dims, l->nb, l->d_stats, st);
             launch_fill_table(l->d_table, l->rows, 0, st);
@@ -472,20 +482,24 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(l->ctx);
-        // 1. learned frame (on the cell records)
+        // 1. sparse plain re-march (the plain cell kernel on every stride-th
+        //    pixel) on the side stream, beside 2. the learned frame
         ensure_cells(vol, st);
-        LearnedArgs la;
-        fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
-        la.m.cells = vol->d_cells;
-        launch_learned(la, phong, count, st);
-        // 2. sparse plain re-march (the plain cell kernel on every stride-th pixel)
+        hip_check(hipEventRecord(l->fork, st), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(l->side, l->fork, 0), "hipStreamWaitEvent");
         MarchArgs sp;
         fill_march_args(sp, vol, tf, prm, d_entry, d_exit, (float*)l->d_sparse, w, h, nullptr);
         sp.grid_w = swc;
         sp.grid_h = shc;
         sp.stride = (int32_t)stride;
         sp.pix_offset = (int32_t)offs;
-        launch_march_cell(sp, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, st);
+        launch_march_cell(sp, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, l->side);
+        hip_check(hipEventRecord(l->join, l->side), "hipEventRecord");
+        LearnedArgs la;
+        fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
+        la.m.cells = vol->d_cells;
+        launch_learned(la, phong, count, st);
+        hip_check(hipStreamWaitEvent(st, l->join, 0), "hipStreamWaitEvent");
         // 3. judge: flag = PSNR below the floor
         const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);
         launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)offs,

@@ -53,7 +53,7 @@ def random_tf_keys(seed):
 
 
 def train(ctx, vol, dims, n_tfs=200, n_views=16, trace_size=192, tf_width=None, rate=2.0,
-          epochs=4, min_steps=2000, batch=8192, lr=2e-3, tau_skip=1e-3, tau_step=1.5e-3,
+          epochs=4, min_steps=2000, batch=8192, lr=2e-3, tau_skip=3e-3, tau_step=8e-3,
           seed=0x5EED, capacity=None, extra_tfs=(), diagnose=False, log=None):
     """Trace, collect and train; returns (mlp, info dict).
 
