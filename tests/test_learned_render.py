@@ -65,8 +65,10 @@ def test_learned_baseline_matches_plain(ctx, scene):
 def test_trained_network_gains_over_baseline(ctx, scene):
     from voreen_amd import hipdvr as H, stepper
     print("[case] train stepping network (24 TFs x 8 views) and render", flush=True)
+    # thresholds below the library's defaults: a small network trained on few
+    # traces of a 128^3 volume, checked against the 45 dB floor
     mlp, info = stepper.train(ctx, scene["v"], scene["dims"], n_tfs=24, n_views=8, trace_size=128,
-                              epochs=6)
+                              epochs=6, tau_step=4e-3, tau_skip=2e-3)
     print(f"training: {info}", flush=True)
     try:
         assert info["rows"] > 1000 and info["loss_last"] < info["loss_first"]
