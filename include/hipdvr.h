@@ -103,7 +103,18 @@ enum {
 };
 
 /* shading option keys (src/core/utils/glsl.cpp:397-409) */
-enum { HIPDVR_SHADE_NONE = 0, HIPDVR_SHADE_PHONG = 1 };
+enum {
+    HIPDVR_SHADE_NONE = 0,                   /* "none" */
+    HIPDVR_SHADE_PHONG = 1,                  /* "phong" (mod_shading.frag:134-151) */
+    HIPDVR_SHADE_PHONG_DIFFUSE = 2,          /* "phong-diffuse" */
+    HIPDVR_SHADE_PHONG_SPECULAR = 3,         /* "phong-specular" */
+    HIPDVR_SHADE_PHONG_DIFFUSE_AMBIENT = 4,  /* "phong-diffuse-ambient" */
+    HIPDVR_SHADE_PHONG_DIFFUSE_SPECULAR = 5, /* "phong-diffuse-specular" */
+    HIPDVR_SHADE_TOON = 6,                   /* "toon" (3 shades) */
+    HIPDVR_SHADE_COOK_TORRANCE = 7,          /* "cook-torrance" */
+    HIPDVR_SHADE_OREN_NAYAR = 8,             /* "oren-nayar" */
+    HIPDVR_SHADE_WARD = 9                    /* "ward" */
+};
 
 /* TransFuncBase::AlphaMode (transfuncbase.h:165-169) */
 enum { HIPDVR_ALPHA_ZERO = 0, HIPDVR_ALPHA_USE = 1, HIPDVR_ALPHA_ONE = 2 };

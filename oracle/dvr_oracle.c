@@ -520,6 +520,124 @@ static void phong_f(const dvr_params* P, const float gi[3], const float pos[3],
     }
 }
 
+/* GLSL normalize with zero -> zero (undefined in GLSL) */
+static void normalize3(float v[3]) {
+    float l2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (l2 > 0.f) { float r = 1.f / sqrtf(l2); v[0] *= r; v[1] *= r; v[2] *= r; }
+    else { v[0] = v[1] = v[2] = 0.f; }
+}
+static float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* The shading models of modules/core/glsl/modules/mod_shading.frag with the
+ * arguments APPLY_SHADING passes (src/core/utils/glsl.cpp:411-434; ka = kd =
+ * TF colour, ks = 1): phongShadingD :245-262, phongShadingS :188-210,
+ * phongShadingDA :222-239, phongShadingDS :160-177, toonShading :273-288
+ * (3 shades, unnormalized L), cookTorranceShading :303-361, orenNayarShading
+ * :374-405, wardShading :419-483.  acos arguments are clamped to [-1, 1]. */
+static void shade_f(const dvr_params* P, const float g[3], const float pos[3], const float c[3], float out[3]) {
+    const int m = P->shading;
+    if (m == DVR_SHADE_PHONG) { phong_f(P, g, pos, c, out); return; }
+    float N[3] = {g[0], g[1], g[2]};
+    normalize3(N);
+    float L[3], V[3];
+    for (int i = 0; i < 3; ++i) { L[i] = P->light_pos[i] - pos[i]; V[i] = P->camera_pos[i] - pos[i]; }
+    if (m == DVR_SHADE_TOON) {
+        float ndl = fmaxf(dot3(N, L), 0.f);
+        for (int i = 1; i <= 3; ++i) {
+            if (ndl <= (float)i / 3.0f) { ndl = (float)i / 3.0f; break; }
+        }
+        for (int i = 0; i < 3; ++i) out[i] = (c[i] * P->light_diffuse[i]) * ndl;
+        return;
+    }
+    if (m <= DVR_SHADE_PHONG_DS) {
+        normalize3(V);
+        float d = sqrtf(dot3(L, L));
+        for (int i = 0; i < 3; ++i) L[i] /= d;
+        float ndl = fmaxf(dot3(N, L), 0.f);
+        float H[3] = {V[0] + L[0], V[1] + L[1], V[2] + L[2]};
+        normalize3(H);
+        float sp = powf(fmaxf(dot3(N, H), 0.f), P->shininess);
+        int amb = m == DVR_SHADE_PHONG_DA;
+        int dif = m == DVR_SHADE_PHONG_D || m == DVR_SHADE_PHONG_DA || m == DVR_SHADE_PHONG_DS;
+        int spe = m == DVR_SHADE_PHONG_S || m == DVR_SHADE_PHONG_DS;
+        for (int i = 0; i < 3; ++i) {
+            float s = 0.0f;
+            if (amb) s = s + c[i] * P->light_ambient[i];
+            if (dif) s = s + (c[i] * P->light_diffuse[i]) * ndl;
+            if (spe) s = s + (1.0f * P->light_specular[i]) * sp;
+            out[i] = s;
+        }
+        return;
+    }
+    normalize3(L);
+    normalize3(V);
+    float H[3] = {V[0] + L[0], V[1] + L[1], V[2] + L[2]};
+    normalize3(H);
+    if (m == DVR_SHADE_COOK_TORRANCE) {
+        float rough = 1.0f - (P->shininess / 128.0f);
+        float s = P->shininess / 128.0f, dd = 1.0f - s;
+        float dw = sqrtf(dot3(L, L)) * 1.0f;
+        float NH = dot3(N, H), NV = dot3(N, V), NL = fmaxf(0.0000001f, dot3(N, L)), VH = dot3(V, H);
+        float F = 1.0f + (1.0f - 1.0f) * powf(1.0f - NV, 5.0f);
+        float G1 = (2.0f * NH * NL) / VH, G2 = (2.0f * NH * NV) / VH;
+        float G = fminf(1.0f, fmaxf(0.0f, fminf(G1, G2)));
+        float mq = rough * rough + 1e-5f, NHq = NH * NH + 1e-5f;
+        float D1 = 1.0f / (4.0f * mq * NHq * NHq);
+        float D2 = expf(-(1.0f - NHq) / (mq * NHq));
+        float D = D1 * D2;
+        float k = ((F * D) * G) / ((3.1415926f * NV) * NL);
+        for (int i = 0; i < 3; ++i) {
+            float Rd = ((c[i] * P->light_diffuse[i]) * 2.0f) * NL;
+            float Rs = (k * 1.0f) * P->light_specular[i];
+            out[i] = (0.0f + c[i] * P->light_ambient[i]) + ((NL * dw) * (s * Rs + dd * Rd));
+        }
+    } else if (m == DVR_SHADE_OREN_NAYAR) {
+        float rough = P->shininess / 128.0f;
+        float VdotN = dot3(V, N), LdotN = dot3(L, N), NdotL = fmaxf(0.0000001f, dot3(N, L));
+        float u[3], w[3];
+        for (int i = 0; i < 3; ++i) { u[i] = V[i] - N[i] * VdotN; w[i] = L[i] - N[i] * LdotN; }
+        float cosPhi = dot3(u, w);
+        float av = acosf(fminf(fmaxf(VdotN, -1.0f), 1.0f)), al = acosf(fminf(fmaxf(LdotN, -1.0f), 1.0f));
+        float alpha = fmaxf(av, al), beta = fminf(av, al);
+        float mq = rough * rough;
+        float A = 1.0f - 0.5f * (mq / (mq + 0.57f));
+        float B = 0.45f * (mq / (mq + 0.09f));
+        float r = 0.0f + (A + ((B * fmaxf(0.0f, cosPhi)) * sinf(alpha)) * tanf(beta));
+        for (int i = 0; i < 3; ++i)
+            out[i] = (0.0f + c[i] * P->light_ambient[i]) + ((c[i] * P->light_diffuse[i]) * NdotL) * r;
+    } else {
+        float ax = P->shininess / 128.0f;
+        float ay = ax * 2.0f;
+        float X[3] = {N[1] * 0.0f - N[2] * 0.0f, N[2] * 1.0f - N[0] * 0.0f, N[0] * 0.0f - N[1] * 1.0f};
+        normalize3(X);
+        float Y[3] = {N[1] * X[2] - N[2] * X[1], N[2] * X[0] - N[0] * X[2], N[0] * X[1] - N[1] * X[0]};
+        normalize3(Y);
+        float VdotN = dot3(V, N), LdotN = dot3(L, N), HdotN = dot3(H, N);
+        float HdotX = dot3(H, X), HdotY = dot3(H, Y), NdotL = fmaxf(0.0000001f, dot3(N, L));
+        ax += 0.0000001f;
+        ay += 0.0000001f;
+        float A = HdotX / ax; A *= A;
+        float B = HdotY / ay; B *= B;
+        float expo = -2.0f * ((A + B) / (1.0f + HdotN));
+        float nenner = 4.0f * 3.14159f;
+        nenner *= ax;
+        nenner *= ay;
+        nenner *= sqrtf(LdotN * VdotN);
+        if (nenner < 0.0f) nenner = fminf(0.0000001f, nenner);
+        else nenner = fmaxf(0.0000001f, nenner);
+        float e = expf(expo) / nenner;
+        for (int i = 0; i < 3; ++i) {
+            float Rd = c[i] * P->light_diffuse[i];
+            float Rs = (1.0f * P->light_specular[i]) * e;
+            out[i] = (0.0f + c[i] * P->light_ambient[i]) + NdotL * (Rs + Rd);
+        }
+    }
+}
+
+void dvr_shade(const dvr_params* P, const float g[3], const float pos[3], const float c[3], float out[3]) {
+    shade_f(P, g, pos, c, out);
+}
+
 static void phong_d(const dvr_params* P, const double gi[3], const double pos[3],
                     const double c[3], double out[3]) {
     double gl = gi[0] * gi[0] + gi[1] * gi[1] + gi[2] * gi[2];
@@ -593,15 +711,15 @@ static int64_t march_modes_f(const dvr_volume* v, const dvr_tf* tf, const dvr_pa
                         (float)c8[3] / 255.f};
         if (col[3] > 0.0f) {
             float gr[3] = {0.f, 0.f, 0.f};
-            const int need_grad = P->shading == DVR_SHADE_PHONG || mode == DVR_COMP_FHN;
+            const int need_grad = P->shading != DVR_SHADE_NONE || mode == DVR_COMP_FHN;
             if (need_grad) {
                 if (t == 0.0f) clip_border_gradient(entry, w, h, px, py, s, dir, gr);
                 else gradient_f(v, P->filter, s, gr);
             }
-            if (P->shading == DVR_SHADE_PHONG) {
+            if (P->shading != DVR_SHADE_NONE) {
                 float pos[3], c[3] = {col[0], col[1], col[2]}, sh[3];
                 tex_to_phys_f(v, s, pos);
-                phong_f(P, gr, pos, c, sh);
+                shade_f(P, gr, pos, c, sh);
                 col[0] = sh[0]; col[1] = sh[1]; col[2] = sh[2];
             }
             switch (mode) {
@@ -725,12 +843,12 @@ static int64_t march_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
             ca = dvr_opacity_correct(ca, step);
         }
         if (ca > 0.0f) {
-            if (P->shading == DVR_SHADE_PHONG) {
+            if (P->shading != DVR_SHADE_NONE) {
                 float gr[3], pos[3], c[3] = {cr, cg, cb}, sh[3];
                 if (t == 0.0f) clip_border_gradient(entry, w, h, px, py, s, dir, gr);
                 else gradient_f(v, P->filter, s, gr);
                 tex_to_phys_f(v, s, pos);
-                phong_f(P, gr, pos, c, sh);
+                shade_f(P, gr, pos, c, sh);
                 cr = sh[0]; cg = sh[1]; cb = sh[2];
             }
             float wgt = (1.0f - A) * ca;
@@ -775,6 +893,13 @@ static int64_t march_d(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
         /* the FP64 envelope is defined for DVR only; other modes are the FP32 result */
         float rf[4];
         const int64_t k = march_modes_f(v, tf, P, first, dir, tend, entry, w, h, px, py, rf);
+        for (int i = 0; i < 4; ++i) res[i] = rf[i];
+        return k;
+    }
+    if (P->shading != DVR_SHADE_NONE && P->shading != DVR_SHADE_PHONG) {
+        /* and for Phong among the shading models; the others are the FP32 result */
+        float rf[4];
+        const int64_t k = march_f(v, tf, P, first, last, entry, w, h, px, py, rf);
         for (int i = 0; i < 4; ++i) res[i] = rf[i];
         return k;
     }

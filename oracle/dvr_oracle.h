@@ -45,7 +45,10 @@ extern "C" {
 enum { DVR_U8 = 0, DVR_U16 = 1 };
 enum { DVR_FILTER_NEAREST = 0, DVR_FILTER_LINEAR = 1 };
 enum { DVR_CLASS_TF = 0, DVR_CLASS_PREINT = 1 };
-enum { DVR_SHADE_NONE = 0, DVR_SHADE_PHONG = 1 };
+/* shading models, glsl.cpp:397-434 option keys in this order */
+enum { DVR_SHADE_NONE = 0, DVR_SHADE_PHONG = 1, DVR_SHADE_PHONG_D = 2, DVR_SHADE_PHONG_S = 3,
+       DVR_SHADE_PHONG_DA = 4, DVR_SHADE_PHONG_DS = 5, DVR_SHADE_TOON = 6, DVR_SHADE_COOK_TORRANCE = 7,
+       DVR_SHADE_OREN_NAYAR = 8, DVR_SHADE_WARD = 9 };
 /* compositing modes, mod_compositing.frag:47-230 / volumeraycaster.cpp:250-271 */
 enum { DVR_COMP_DVR = 0, DVR_COMP_MIP = 1, DVR_COMP_MOP = 2, DVR_COMP_MIDA = 3, DVR_COMP_ISO = 4,
        DVR_COMP_FHP = 5, DVR_COMP_FHN = 6 };
@@ -129,6 +132,10 @@ int64_t dvr_cpu_render(const dvr_volume* vol, const dvr_tf* tf, const dvr_params
 int64_t dvr_fp64_render(const dvr_volume* vol, const dvr_tf* tf, const dvr_params* prm,
                         const float* entry, const float* exit_, int32_t w, int32_t h,
                         double* out_rgba, int32_t nthreads);
+
+/* one shading evaluation (params' model, light, camera, shininess) of TF
+ * colour c with gradient g at physical position pos (for unit tests) */
+void dvr_shade(const dvr_params* prm, const float g[3], const float pos[3], const float c[3], float out[3]);
 
 /* single trilinear reconstruction, volumeram.cpp:78-97 (for unit tests) */
 float dvr_sample_linear(const dvr_volume* vol, float vx, float vy, float vz);

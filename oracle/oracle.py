@@ -21,7 +21,8 @@ _LIB_PATH = os.path.join(_HERE, "libdvr_oracle.so")
 U8, U16 = 0, 1
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 CLASS_TF, CLASS_PREINT = 0, 1
-SHADE_NONE, SHADE_PHONG = 0, 1
+(SHADE_NONE, SHADE_PHONG, SHADE_PHONG_D, SHADE_PHONG_S, SHADE_PHONG_DA, SHADE_PHONG_DS, SHADE_TOON,
+ SHADE_COOK_TORRANCE, SHADE_OREN_NAYAR, SHADE_WARD) = range(10)
 COMP_DVR, COMP_MIP, COMP_MOP, COMP_MIDA, COMP_ISO, COMP_FHP, COMP_FHN = range(7)
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
 
@@ -85,6 +86,8 @@ def _load():
         fn.argtypes = [P(Volume), P(TF), P(Params), C.c_void_p, C.c_void_p, C.c_int32,
                        C.c_int32, C.c_void_p, C.c_int32]
         fn.restype = C.c_int64
+    lib.dvr_shade.argtypes = [P(Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.dvr_shade.restype = None
     lib.dvr_sample_linear.argtypes = [P(Volume), C.c_float, C.c_float, C.c_float]
     lib.dvr_sample_linear.restype = C.c_float
     lib.dvr_mlp_forward.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
@@ -226,6 +229,18 @@ def render(vol, tf_rgba8, domain, entry, exit_, step, filter_=FILTER_LINEAR,
                                  w, h, _ptr(out), nthreads)
     del vol_keep
     return out, int(n)
+
+
+def shade(model, g, pos, color, light=None, shininess=60.0):
+    """One shading evaluation of the oracle (FP32)."""
+    p = _params(1.0, FILTER_LINEAR, CLASS_TF, None, model, light)
+    p.shininess = shininess
+    g = np.ascontiguousarray(g, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    color = np.ascontiguousarray(color, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().dvr_shade(C.byref(p), _ptr(g), _ptr(pos), _ptr(color), _ptr(out))
+    return out
 
 
 def sample_linear(vol, pos):

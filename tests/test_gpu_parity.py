@@ -201,6 +201,60 @@ def test_phong_on_shells(ctx, oracle, kernel):
     assert r["stats"]["shaded"] > 0
 
 
+SHADING_MODELS = list(range(2, 10))
+
+
+@pytest.mark.parametrize("model", SHADING_MODELS)
+def test_shading_models(ctx, oracle, model):
+    """The shading models of mod_shading.frag (phong parts, toon, Cook-Torrance,
+    Oren-Nayar, Ward) on the reference kernel (both filters) and on the cell
+    kernel / automatic choice, against the FP32 oracle, same tolerance as Phong."""
+    from voreen_amd import hipdvr as H, workloads as W
+    dims = (64, 64, 64)
+    vol = W.make_volume("c1", dims=dims)
+    keys, _ = W.make_tf("c1")
+    tf = oracle.tf_build(keys, 4096)
+    cam = W.orbit_camera(9, 36)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 96, 96, llf=llf, urb=urb)
+    sp = (1.0 / 64,) * 3
+    off = tuple(float(llf[i] + 0.5 / 64) for i in range(3))
+    name = H.SHADING_KEYS[model]
+    for kernel, filt in ((H.KERNEL_REFERENCE, 0), (H.KERNEL_REFERENCE, 1), (H.KERNEL_CELL, 1),
+                         (H.KERNEL_AUTO, 1), (H.KERNEL_AUTO, 0)):
+        say(f"shading {name} kernel {kernel} filter {filt} 64^3 u16 96^2")
+        r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, W.step_for(dims, 2.0), shading=model,
+                        filter_=filt, spacing=sp, offset=off, camera_pos=tuple(cam.eye), kernel=kernel)
+        # relative beyond 1: Ward's specular term grows without bound at grazing
+        # light or view (its denominator sqrt(L.N * V.N) tends to 0)
+        d = (report(f"{name} k{kernel} f{filt}", r, oracle) / np.maximum(1.0, np.abs(r["cpu"]))).max(axis=2)
+        assert np.mean(d > TOL) < 0.002
+        assert r["stats"]["samples"] == r["n"] and r["stats"]["shaded"] > 0
+        assert np.mean(r["gpu"][..., 3] > 0) > 0.05
+    # the bricked kernel serves Phong only
+    v = ctx.volume(vol)
+    t = ctx.transfunc(tf)
+    try:
+        with pytest.raises(H.HipDvrError) as e:
+            ctx.render(v, t, en, ex, H.default_params(0.01, kernel=H.KERNEL_BRICKED, shading=model))
+        assert e.value.code == H.ERR_UNSUPPORTED
+    finally:
+        v.close()
+        t.close()
+
+
+def test_shading_model_with_isosurface_compositing(ctx, oracle):
+    from voreen_amd import hipdvr as H
+    vol, tf, en, ex, step = c0_inputs(oracle, size=96, view=3, nviews=36)
+    for model in (H.SHADE_TOON, H.SHADE_WARD):
+        say(f"iso compositing with {H.SHADING_KEYS[model]}")
+        r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, step, shading=model,
+                        compositing=H.COMPOSITE_ISO, iso_value=0.3)
+        d = report(f"iso {model}", r, oracle).max(axis=2)
+        assert np.mean(d > TOL) < 0.002
+        assert np.mean(r["gpu"][..., 3] > 0) > 0.02
+
+
 @pytest.mark.parametrize("integral", [False, True])
 def test_preint_table_on_the_device(ctx, oracle, integral):
     """hipdvr_tf_preint_device against the oracle's table (the device's pow/exp

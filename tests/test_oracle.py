@@ -489,3 +489,45 @@ def test_mida_gamma_le_zero_converges_to_its_fixed_point(oracle):
         A = a / (1 - beta * (1 - a))
         R = (1 - beta * A) * a * np.array([200, 100, 50]) / 255.0 / (1 - beta)
         assert abs(img[0, 0, 3] - A) < 1e-5 and np.allclose(img[0, 0, :3], R, atol=1e-5)
+
+
+# --------------------------------------------------------------------------
+# shading models (mod_shading.frag) — closed forms where N = L = V
+# --------------------------------------------------------------------------
+
+def test_shading_models_closed_forms(oracle):
+    c = np.array([0.5, 0.25, 1.0])
+    amb, dif, spe = 0.4, 0.8, 0.6
+    light = dict(pos=(0.0, 0.0, 5.0), camera=(0.0, 0.0, 5.0))
+    n, pos = (0.0, 0.0, 2.0), (0.0, 0.0, 0.0)
+    sh = 60.0
+
+    def shade(model, light=light, g=n, shininess=sh):
+        return oracle.shade(model, g, pos, c, light=light, shininess=shininess).astype(np.float64)
+
+    assert np.allclose(shade(oracle.SHADE_PHONG_D), dif * c, atol=1e-6)
+    assert np.allclose(shade(oracle.SHADE_PHONG_S), np.full(3, spe), atol=1e-6)
+    assert np.allclose(shade(oracle.SHADE_PHONG_DA), (amb + dif) * c, atol=1e-6)
+    assert np.allclose(shade(oracle.SHADE_PHONG_DS), dif * c + spe, atol=1e-6)
+    assert np.allclose(shade(oracle.SHADE_PHONG), amb * c + dif * c + spe, atol=1e-6)
+    # diffuse falls off as cos(theta) with the light at 60 degrees
+    off = dict(pos=(0.0, 5.0 * math.sin(math.pi / 3), 5.0 * math.cos(math.pi / 3)), camera=(0.0, 0.0, 5.0))
+    assert np.allclose(shade(oracle.SHADE_PHONG_D, light=off), dif * c * 0.5, atol=1e-6)
+    # toon: N.L with the unnormalized light vector, quantized up to i/3 when <= 1
+    assert np.allclose(shade(oracle.SHADE_TOON), dif * c * 5.0, atol=1e-5)
+    near = dict(pos=(0.0, 0.0, 0.5), camera=(0.0, 0.0, 5.0))
+    assert np.allclose(shade(oracle.SHADE_TOON, light=near), dif * c * (2.0 / 3.0), atol=1e-6)
+    # Cook-Torrance at N = H = L = V: F = 1, G = 1, Beckmann D at NH = 1
+    m, s = 1 - sh / 128, sh / 128
+    mq, nhq = m * m + 1e-5, 1 + 1e-5
+    D = 1 / (4 * mq * nhq * nhq) * math.exp(-(1 - nhq) / (mq * nhq))
+    rs, rd = D / 3.1415926 * spe, dif * 2 * c
+    assert np.allclose(shade(oracle.SHADE_COOK_TORRANCE), amb * c + (s * rs + (1 - s) * rd), rtol=1e-5)
+    # Oren-Nayar at N = L = V: alpha = beta = 0, so only A remains
+    mq = (sh / 128) ** 2
+    A = 1 - 0.5 * (mq / (mq + 0.57))
+    assert np.allclose(shade(oracle.SHADE_OREN_NAYAR), amb * c + dif * c * A, rtol=1e-6)
+    # Ward at N = H: tangent frame X = (0,1,0), Y = (-1,0,0), exponent 0
+    ax, ay = sh / 128 + 1e-7, 2 * sh / 128 + 1e-7
+    nenner = 4 * 3.14159 * ax * ay
+    assert np.allclose(shade(oracle.SHADE_WARD), amb * c + (spe / nenner + dif * c), rtol=1e-6)

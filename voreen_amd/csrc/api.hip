@@ -99,7 +99,7 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR need transfer-function classification");
     if (!std::isfinite(p->gamma) || !std::isfinite(p->iso_value))
         fail(HIPDVR_ERR_INVALID, "gamma and iso_value must be finite");
-    if (p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
+    if (p->shading < HIPDVR_SHADE_NONE || p->shading > HIPDVR_SHADE_WARD)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
     if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
@@ -141,6 +141,7 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.dom_lo = tf->domain_lo;
     a.dom_hi = tf->domain_hi;
     a.dom_unit = (tf->domain_hi - tf->domain_lo) == 1.0f ? 1 : 0;
+    a.shade_model = p->shading;
     a.tf_identity = (v.rwm_scale == 1.0f && v.rwm_offset == 0.0f && tf->domain_lo == 0.0f && tf->domain_hi == 1.0f)
                         ? 1 : 0;
     a.tf_width = (int32_t)tf->width;
@@ -182,7 +183,9 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     if (count) launch_zero_u64(ctx->d_counters, kNumCounters, ctx->stream);
     MarchArgs a;
     fill_march_args(a, vol, tf, p, d_entry, d_exit, d_out, w, h, ctx->d_counters);
-    const bool phong = p->shading == HIPDVR_SHADE_PHONG;
+    // any shading model: the gradient plus lighting; the fast cell variant
+    // (interior gradient, hardware lighting terms) only for Phong
+    const bool phong = p->shading != HIPDVR_SHADE_NONE;
     int kernel = p->kernel;
     // the skipping kernels drop samples, which the pre-integrated mode cannot
     // (its lookup pairs every sample with the previous one); the cell records
@@ -193,13 +196,16 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     if (kernel == HIPDVR_KERNEL_AUTO) {
         if (p->classification != HIPDVR_CLASS_TF || modes) kernel = HIPDVR_KERNEL_REFERENCE;
         else if (p->filter == HIPDVR_FILTER_LINEAR && cells_allowed(vol)) kernel = HIPDVR_KERNEL_CELL;
-        else kernel = HIPDVR_KERNEL_BRICKED;
+        else if (p->shading == HIPDVR_SHADE_NONE || p->shading == HIPDVR_SHADE_PHONG) kernel = HIPDVR_KERNEL_BRICKED;
+        else kernel = HIPDVR_KERNEL_REFERENCE;
     }
     const bool cell = kernel == HIPDVR_KERNEL_CELL;
     if ((kernel == HIPDVR_KERNEL_BRICKED || cell) && p->classification != HIPDVR_CLASS_TF)
         fail(HIPDVR_ERR_UNSUPPORTED, "the skipping kernels support transfer-function classification only");
     if (cell && p->filter != HIPDVR_FILTER_LINEAR)
         fail(HIPDVR_ERR_UNSUPPORTED, "the cell kernel supports the linear filter only");
+    if (kernel == HIPDVR_KERNEL_BRICKED && p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
+        fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel shades with Phong only");
     hipdvr_volume* mvol = const_cast<hipdvr_volume*>(vol);
     if (kernel == HIPDVR_KERNEL_BRICKED || cell) ensure_occupancy(ctx, mvol, tf);
     if (cell) {
@@ -215,7 +221,9 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     } else if (kernel == HIPDVR_KERNEL_CELL) {
         // measured on the 512^3 workload: two samples in flight per ray when
         // shading (gradient loads add their own), four otherwise
-        launch_march_cell(a, vol->desc.dtype, phong ? (p->exact_shading ? 2 : 1) : 0, count, mvol->cells_wide,
+        launch_march_cell(a, vol->desc.dtype,
+                          phong ? ((p->exact_shading || p->shading != HIPDVR_SHADE_PHONG) ? 2 : 1) : 0, count,
+                          mvol->cells_wide,
                           p->batch ? p->batch : (phong ? 2 : kDefaultBatch),
                           ctx->stream);
 

@@ -209,8 +209,9 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             if (j.params.filter != HIPDVR_FILTER_LINEAR || j.params.classification != HIPDVR_CLASS_TF ||
                 j.params.compositing != HIPDVR_COMPOSITE_DVR)
                 fail(HIPDVR_ERR_UNSUPPORTED, "batched jobs use the linear filter, TF classification and DVR");
-            const int ph = j.params.shading == HIPDVR_SHADE_PHONG ? (j.params.exact_shading ? 2 : 1) : 0;
-            if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading mode");
+            const int ph = j.params.shading == HIPDVR_SHADE_NONE ? 0
+                         : (j.params.shading == HIPDVR_SHADE_PHONG && !j.params.exact_shading) ? 1 : 2;
+            if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading path");
             shade = ph;
             if (cell_record_bytes(j.volume) > 0xffffff00ull)
                 fail(HIPDVR_ERR_UNSUPPORTED, "batched volumes must have cell records below 4 GiB");

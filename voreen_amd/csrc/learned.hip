@@ -457,6 +457,10 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         validate_params(prm, tf);
         if (prm->filter != HIPDVR_FILTER_LINEAR || prm->classification != HIPDVR_CLASS_TF)
             fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
+        if (prm->shading != HIPDVR_SHADE_NONE && prm->shading != HIPDVR_SHADE_PHONG)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping shades with Phong or not at all");
+        if (prm->compositing != HIPDVR_COMPOSITE_DVR)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping composites DVR");
         const hipdvr_learned_opts opts = default_opts(o);
         ctx_set_device(l->ctx);
         hipStream_t st = ctx_stream(l->ctx);

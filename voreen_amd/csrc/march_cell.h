@@ -356,7 +356,7 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
                         pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                    if (SHADE == 2) phong(a, gr, pos, col);
+                    if (SHADE == 2) shade_model(a, gr, pos, col);
                     else phong_fast(a, gr, pos, col);
                     ++n.shaded;
                 }

@@ -18,6 +18,9 @@ constexpr float kErtOpacity = 0.95f;        // mod_raysetup.frag:36, cpuraycaste
 constexpr int kMaxTfWidth = 4096;           // transfunc1dkeysproperty.cpp:95
 constexpr int kDefaultLoopCap = 255 * 255;  // cpuraycaster.cpp:239
 constexpr int kNumCounters = 6;
+// shading models (values of hipdvr_params::shading)
+constexpr int kShadeNone = 0, kShadePhong = 1, kShadePhongD = 2, kShadePhongS = 3, kShadePhongDA = 4,
+              kShadePhongDS = 5, kShadeToon = 6, kShadeCookTorrance = 7, kShadeOrenNayar = 8, kShadeWard = 9;
 // compositing modes (values of hipdvr_params::compositing)
 constexpr int kCompDVR = 0, kCompMIP = 1, kCompMOP = 2, kCompMIDA = 3, kCompISO = 4, kCompFHP = 5,
               kCompFHN = 6;
@@ -40,6 +43,7 @@ struct MarchArgs {
     int32_t dom_unit;           // dom_hi - dom_lo == 1: the division is exact, x = rw - lo
     int32_t dtype;              // 0 uint8, 1 uint16
     int32_t tf_identity;        // rwm (1, 0) and TF domain [0, 1]: x = min(intensity, 1)
+    int32_t shade_model;        // kShade*
     int32_t tf_width;
     const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
     const float4* preint;       // N*N, [fe*N + fs]
@@ -218,6 +222,156 @@ __device__ __forceinline__ void phong(const MarchArgs& a, const float g[3], cons
         float spe = (1.0f * a.light_spe[i]) * sp;
         c[i] = ((0.0f + amb) + dif) + spe;
     }
+}
+
+// GLSL normalize() with a zero vector mapped to zero (undefined in GLSL).
+__device__ __forceinline__ void normalize3(float v[3]) {
+    const float l2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (l2 > 0.f) {
+        const float r = 1.f / sqrtf(l2);
+        v[0] *= r;
+        v[1] *= r;
+        v[2] *= r;
+    } else {
+        v[0] = v[1] = v[2] = 0.f;
+    }
+}
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// The shading models of mod_shading.frag as rc_singlevolume.frag:229 calls
+// them through APPLY_SHADING (glsl.cpp:411-434): ka = kd = TF colour, ks = 1.
+//   2 phongShadingD  :245-262   3 phongShadingS  :188-210   4 phongShadingDA :222-239
+//   5 phongShadingDS :160-177   6 toonShading    :273-288 (3 shades)
+//   7 cookTorranceShading :303-361   8 orenNayarShading :374-405   9 wardShading :419-483
+// Phong itself (1) is phong() above.  acos arguments are clamped to [-1, 1]
+// and a zero vector normalizes to zero, where GLSL leaves the result undefined.
+__device__ __forceinline__ void shade_model(const MarchArgs& a, const float g[3], const float pos[3], float c[3]) {
+    const int m = a.shade_model;
+    if (m == kShadePhong) {
+        phong(a, g, pos, c);
+        return;
+    }
+    float N[3] = {g[0], g[1], g[2]};
+    normalize3(N);
+    float L[3], V[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        L[i] = a.light_pos[i] - pos[i];
+        V[i] = a.cam_pos[i] - pos[i];
+    }
+    float out[3] = {0.f, 0.f, 0.f};
+    if (m == kShadeToon) {
+        // L is not normalized here (mod_shading.frag:276)
+        float ndl = fmaxf(dot3(N, L), 0.f);
+        for (int i = 1; i <= 3; ++i) {
+            if (ndl <= (float)i / 3.0f) {
+                ndl = (float)i / 3.0f;
+                break;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[i] = (c[i] * a.light_dif[i]) * ndl;
+    } else if (m <= kShadePhongDS) {
+        normalize3(V);
+        const float d = sqrtf(dot3(L, L));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) L[i] /= d;
+        const float ndl = fmaxf(dot3(N, L), 0.f);
+        float H[3] = {V[0] + L[0], V[1] + L[1], V[2] + L[2]};
+        normalize3(H);
+        const float sp = powf(fmaxf(dot3(N, H), 0.f), a.shininess);
+        const bool amb = m == kShadePhongDA;
+        const bool dif = m == kShadePhongD || m == kShadePhongDA || m == kShadePhongDS;
+        const bool spe = m == kShadePhongS || m == kShadePhongDS;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float s = 0.0f;
+            if (amb) s = s + c[i] * a.light_amb[i];
+            if (dif) s = s + (c[i] * a.light_dif[i]) * ndl;
+            if (spe) s = s + (1.0f * a.light_spe[i]) * sp;
+            out[i] = s;
+        }
+    } else {
+        normalize3(L);
+        normalize3(V);
+        float H[3] = {V[0] + L[0], V[1] + L[1], V[2] + L[2]};
+        normalize3(H);
+        if (m == kShadeCookTorrance) {
+            const float rough = 1.0f - (a.shininess / 128.0f);
+            const float s = a.shininess / 128.0f, dd = 1.0f - s;
+            const float dw = sqrtf(dot3(L, L)) * 1.0f;
+            const float NH = dot3(N, H), NV = dot3(N, V), NL = fmaxf(0.0000001f, dot3(N, L)), VH = dot3(V, H);
+            const float F = 1.0f + (1.0f - 1.0f) * powf(1.0f - NV, 5.0f);
+            const float G1 = (2.0f * NH * NL) / VH, G2 = (2.0f * NH * NV) / VH;
+            const float G = fminf(1.0f, fmaxf(0.0f, fminf(G1, G2)));
+            const float mq = rough * rough + 1e-5f, NHq = NH * NH + 1e-5f;
+            const float D1 = 1.0f / (4.0f * mq * NHq * NHq);
+            const float D2 = expf(-(1.0f - NHq) / (mq * NHq));
+            const float D = D1 * D2;
+            const float k = ((F * D) * G) / ((3.1415926f * NV) * NL);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float Rd = ((c[i] * a.light_dif[i]) * 2.0f) * NL;
+                const float Rs = (k * 1.0f) * a.light_spe[i];
+                out[i] = (0.0f + c[i] * a.light_amb[i]) + ((NL * dw) * (s * Rs + dd * Rd));
+            }
+        } else if (m == kShadeOrenNayar) {
+            const float rough = a.shininess / 128.0f;
+            const float VdotN = dot3(V, N), LdotN = dot3(L, N), NdotL = fmaxf(0.0000001f, dot3(N, L));
+            float u[3], w[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                u[i] = V[i] - N[i] * VdotN;
+                w[i] = L[i] - N[i] * LdotN;
+            }
+            const float cosPhi = dot3(u, w);
+            const float av = acosf(fminf(fmaxf(VdotN, -1.0f), 1.0f)), al = acosf(fminf(fmaxf(LdotN, -1.0f), 1.0f));
+            const float alpha = fmaxf(av, al), beta = fminf(av, al);
+            const float mq = rough * rough;
+            const float A = 1.0f - 0.5f * (mq / (mq + 0.57f));
+            const float B = 0.45f * (mq / (mq + 0.09f));
+            const float r = 0.0f + (A + ((B * fmaxf(0.0f, cosPhi)) * sinf(alpha)) * tanf(beta));
+#pragma unroll
+            for (int i = 0; i < 3; ++i) out[i] = (0.0f + c[i] * a.light_amb[i]) + ((c[i] * a.light_dif[i]) * NdotL) * r;
+        } else {   // Ward
+            float ax = a.shininess / 128.0f;
+            float ay = ax * 2.0f;
+            const float eps[3] = {1.0f, 0.0f, 0.0f};
+            float X[3] = {N[1] * eps[2] - N[2] * eps[1], N[2] * eps[0] - N[0] * eps[2], N[0] * eps[1] - N[1] * eps[0]};
+            normalize3(X);
+            float Y[3] = {N[1] * X[2] - N[2] * X[1], N[2] * X[0] - N[0] * X[2], N[0] * X[1] - N[1] * X[0]};
+            normalize3(Y);
+            const float VdotN = dot3(V, N), LdotN = dot3(L, N), HdotN = dot3(H, N);
+            const float HdotX = dot3(H, X), HdotY = dot3(H, Y), NdotL = fmaxf(0.0000001f, dot3(N, L));
+            ax += 0.0000001f;
+            ay += 0.0000001f;
+            float A = HdotX / ax;
+            A *= A;
+            float B = HdotY / ay;
+            B *= B;
+            const float expo = -2.0f * ((A + B) / (1.0f + HdotN));
+            float nenner = 4.0f * 3.14159f;
+            nenner *= ax;
+            nenner *= ay;
+            nenner *= sqrtf(LdotN * VdotN);
+            // a NaN (negative radicand) takes the max branch, which keeps 1e-7
+            if (nenner < 0.0f) nenner = fminf(0.0000001f, nenner);
+            else nenner = fmaxf(0.0000001f, nenner);
+            const float e = expf(expo) / nenner;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float Rd = c[i] * a.light_dif[i];
+                const float Rs = (1.0f * a.light_spe[i]) * e;
+                out[i] = (0.0f + c[i] * a.light_amb[i]) + NdotL * (Rs + Rd);
+            }
+        }
+    }
+    c[0] = out[0];
+    c[1] = out[1];
+    c[2] = out[2];
 }
 
 // The same Phong terms with hardware reciprocal, reciprocal square root and

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
 #pragma unroll
                         for (int i = 0; i < 3; ++i)
                             pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                        phong(a, gr, pos, c);
+                        shade_model(a, gr, pos, c);
                         cr = c[0]; cg = c[1]; cb = c[2];
                         ++n_shaded;
                     }
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void march_modes_kernel(const MarchArgs a) {
                     float pos[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) pos[i] = (((s[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
-                    phong(a, gr, pos, col);
+                    shade_model(a, gr, pos, col);
                     ++n_shaded;
                 }
                 if (mode == kCompMIP) {

@@ -16,7 +16,10 @@ U8, U16 = 0, 1
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 CLASS_TF, CLASS_PREINT = 0, 1
 COMPOSITE_DVR, COMPOSITE_MIP, COMPOSITE_MOP, COMPOSITE_MIDA, COMPOSITE_ISO, COMPOSITE_FHP, COMPOSITE_FHN = range(7)
-SHADE_NONE, SHADE_PHONG = 0, 1
+(SHADE_NONE, SHADE_PHONG, SHADE_PHONG_DIFFUSE, SHADE_PHONG_SPECULAR, SHADE_PHONG_DIFFUSE_AMBIENT,
+ SHADE_PHONG_DIFFUSE_SPECULAR, SHADE_TOON, SHADE_COOK_TORRANCE, SHADE_OREN_NAYAR, SHADE_WARD) = range(10)
+SHADING_KEYS = ("none", "phong", "phong-diffuse", "phong-specular", "phong-diffuse-ambient",
+                "phong-diffuse-specular", "toon", "cook-torrance", "oren-nayar", "ward")
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL = 0, 1, 2, 3
 

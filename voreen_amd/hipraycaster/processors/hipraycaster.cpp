@@ -217,7 +217,18 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
                                      : HIPDVR_COMPOSITE_DVR;
     p.gamma = gammaValue1_.get();
     p.iso_value = isoValue_.get();
-    p.shading = shadeMode_.getKey() == "phong" ? HIPDVR_SHADE_PHONG : HIPDVR_SHADE_NONE;
+    // shading keys of glsl.cpp:397-409
+    const std::string sh = shadeMode_.getKey();
+    p.shading = sh == "phong"                  ? HIPDVR_SHADE_PHONG
+              : sh == "phong-diffuse"          ? HIPDVR_SHADE_PHONG_DIFFUSE
+              : sh == "phong-specular"         ? HIPDVR_SHADE_PHONG_SPECULAR
+              : sh == "phong-diffuse-ambient"  ? HIPDVR_SHADE_PHONG_DIFFUSE_AMBIENT
+              : sh == "phong-diffuse-specular" ? HIPDVR_SHADE_PHONG_DIFFUSE_SPECULAR
+              : sh == "toon"                   ? HIPDVR_SHADE_TOON
+              : sh == "cook-torrance"          ? HIPDVR_SHADE_COOK_TORRANCE
+              : sh == "oren-nayar"             ? HIPDVR_SHADE_OREN_NAYAR
+              : sh == "ward"                   ? HIPDVR_SHADE_WARD
+                                               : HIPDVR_SHADE_NONE;
     // light and camera to the volume's physical space (src/core/utils/glsl.cpp:61-72)
     const tgt::mat4 worldToPhysical = vb->getWorldToPhysicalMatrix();
     const tgt::vec3 lp = (worldToPhysical * lightPosition_.get()).xyz();
@@ -278,12 +289,7 @@ void HIPRaycaster::process() {
         return;
     }
     const std::string shade = shadeMode_.getKey();
-    if (shade != "none" && shade != "phong") {
-        fail("shading mode '" + shade + "' is not implemented by HIPRaycaster (none, phong)");
-        writeOutput(std::vector<float>(), size.x, size.y);
-        return;
-    }
-    if (shade == "phong" && gradientMode_.getKey() != "central-differences")
+    if (shade != "none" && gradientMode_.getKey() != "central-differences")
         LWARNING("HIPRaycaster computes gradients by central differences");
 
     if (!ensureContext() || !ensureVolume(vb, vram)) {
