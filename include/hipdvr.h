@@ -40,6 +40,13 @@
  *       PreIntegrationTable::computeTable's GPU path preintegrationtable.cpp:
  *       206-254 in the reference; here the exact/integral CPU algorithm runs
  *       one lane per table entry).
+ *   hipdvr_tf2d_create
+ *       the float RGBA texture and domains of tFunc2DPrimitives_ for the
+ *       TFT_2DPRIMITIVES path (cpuraycaster.cpp:133-148,334-338;
+ *       src/core/datastructures/transfunc/2d/transfunc2d.cpp:229-231).
+ *   hipdvr_volume_set_gradient
+ *       the RGB(A) VolumeRAM of gradientVolumePort_ that the 2D TF path samples
+ *       (cpuraycaster.cpp:46,134-141,270-284).
  *   hipdvr_render
  *       the pixel loop and ray march of CPURaycaster::process/directRendering
  *       (cpuraycaster.cpp:157-208 with :210-326), host entry/exit/output buffers
@@ -77,8 +84,9 @@ enum {
     HIPDVR_ERR_UNSUPPORTED = 4  /* option combination not implemented */
 };
 
-/* voxel types: VolumeElement<uint8_t>/<uint16_t> */
-enum { HIPDVR_U8 = 0, HIPDVR_U16 = 1 };
+/* voxel types: VolumeElement<uint8_t>/<uint16_t>; float channels for gradient
+ * volumes only (getTypeAsFloat(float), volumeelement.h:1243-1245) */
+enum { HIPDVR_U8 = 0, HIPDVR_U16 = 1, HIPDVR_F32 = 2 };
 
 /* textureFilterMode_ option keys "nearest"/"linear" (cpuraycaster.cpp:62-64) */
 enum { HIPDVR_FILTER_NEAREST = 0, HIPDVR_FILTER_LINEAR = 1 };
@@ -86,7 +94,20 @@ enum { HIPDVR_FILTER_NEAREST = 0, HIPDVR_FILTER_LINEAR = 1 };
 /* classification option keys (src/core/utils/classificationmodes.cpp:84-90) */
 enum {
     HIPDVR_CLASS_TF = 0,        /* "transfer-function" */
-    HIPDVR_CLASS_PREINT = 1     /* "pre-integrated-cpu" / "-approximation" (table given) */
+    HIPDVR_CLASS_PREINT = 1,    /* "pre-integrated-cpu" / "-approximation" (table given) */
+    HIPDVR_CLASS_TF2D = 2       /* "transfer-function" with tFuncType tFunc2DPrimitives: a 2D
+                                   TF over intensity and gradient-volume magnitude
+                                   (cpuraycaster.cpp:133-148,270-290) */
+};
+
+/* gradient option keys (volumeraycaster.cpp:126-131, CALC_GRADIENT :197-207;
+ * mod_gradients.frag): the gradient used by shading and FHN compositing */
+enum {
+    HIPDVR_GRADIENT_CENTRAL = 0,   /* "central-differences" (the default) */
+    HIPDVR_GRADIENT_FORWARD = 1,   /* "forward-differences" */
+    HIPDVR_GRADIENT_NONE = 2,      /* "none": (voxel.xyz - 0.5) * 2 */
+    HIPDVR_GRADIENT_FILTERED = 3,  /* "filtered" */
+    HIPDVR_GRADIENT_SOBEL = 4      /* "sobel" */
 };
 
 /* compositing option keys (volumeraycaster.cpp:379-388); the modes other
@@ -165,6 +186,8 @@ typedef struct hipdvr_params {
     int32_t exact_shading;  /* Phong in the cell kernel: 1 = bit-identical to the reference
                                kernel; 0 = interior gradient from the sample's own records and
                                hardware rsq/exp2/log2 lighting (within the oracle tolerance) */
+    int32_t gradient;       /* HIPDVR_GRADIENT_*; other than central differences the shaded
+                               (or FHN) render runs on the reference kernel */
 } hipdvr_params;
 
 typedef struct hipdvr_stats {
@@ -210,6 +233,11 @@ int hipdvr_volume_create_device(hipdvr_ctx* ctx, const hipdvr_volume_desc* desc,
 int hipdvr_volume_destroy(hipdvr_volume* vol);
 /* device pointer of the linear (x-fastest) copy */
 int hipdvr_volume_device_ptr(const hipdvr_volume* vol, const void** out);
+/* Attach the gradient volume of the 2D TF path: the volume's dims, `channels`
+ * (3 or 4) interleaved channels of dtype (HIPDVR_U8 / U16 / F32) per voxel,
+ * x-fastest; channels 0..2 are read (cpuraycaster.cpp:46 gradientVolumePort_,
+ * checks :134-141, sampling :270-284). */
+int hipdvr_volume_set_gradient(hipdvr_volume* vol, const void* host_data, int32_t dtype, uint32_t channels);
 
 /* ---- transfer function ---- */
 int hipdvr_tf_from_keys(const hipdvr_tf_key* keys, int32_t nkeys, int32_t width, float gamma,
@@ -217,6 +245,14 @@ int hipdvr_tf_from_keys(const hipdvr_tf_key* keys, int32_t nkeys, int32_t width,
 int hipdvr_tf_create(hipdvr_ctx* ctx, const uint8_t* rgba8, uint32_t width, float domain_lo,
                      float domain_hi, hipdvr_tf** out);
 int hipdvr_tf_destroy(hipdvr_tf* tf);
+/* A 2D transfer function: the float RGBA texture of TransFunc2DPrimitives
+ * (texel (x, y) at rgba[4*(y*width + x)], x = intensity, y = gradient
+ * magnitude) with domain = {intensity lo, hi, magnitude lo, hi}
+ * (transfunc2d.cpp:229-231).  Rendered with HIPDVR_CLASS_TF2D; the opacity
+ * correction is applied per step on the host, as cpuraycaster.cpp:289 does
+ * per sample. */
+int hipdvr_tf2d_create(hipdvr_ctx* ctx, const float* rgba, uint32_t width, uint32_t height,
+                       const float domain[4], hipdvr_tf** out);
 /* N x N RGBA float table, element [fe_index*N + fs_index] */
 int hipdvr_tf_set_preint(hipdvr_tf* tf, const float* table_rgba, uint32_t n);
 int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, float gamma, float thr_lo,

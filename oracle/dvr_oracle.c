@@ -390,6 +390,48 @@ static double sample_linear_d(const dvr_volume* v, float px_, float py_, float p
          + voxel_norm_d(v, lo[0], hi[1], hi[2]) * ix * p[1] * p[2];
 }
 
+/* Channel c of the gradient volume, normalized (getVoxelNormalized(pos,
+ * channel), volumeatomic.h:531-536: float channels are returned as stored). */
+static inline float grad_voxel_norm(const dvr_volume* v, size_t x, size_t y, size_t z, int c) {
+    size_t idx = (z * (size_t)v->dims[0] * (size_t)v->dims[1] + y * (size_t)v->dims[0] + x) *
+                 (size_t)v->grad_channels + (size_t)c;
+    if (v->grad_dtype == DVR_U8) return (float)((const uint8_t*)v->grad_data)[idx] / 255.0f;
+    if (v->grad_dtype == DVR_U16) return (float)((const uint16_t*)v->grad_data)[idx] / 65535.0f;
+    return ((const float*)v->grad_data)[idx];
+}
+
+/* The gradient-volume sample of the 2D TF path, cpuraycaster.cpp:270-284:
+ * channels 0..2 at the rounded voxel position (nearest) or trilinear per
+ * channel (getVoxelNormalizedLinear(pos, channel), volumeram.cpp:78-97). */
+static void grad_volume_sample(const dvr_volume* v, int filter, const float vp[3], float out[3]) {
+    if (filter == DVR_FILTER_NEAREST) {
+        size_t x = (size_t)iroundf_tgt(vp[0]), y = (size_t)iroundf_tgt(vp[1]), z = (size_t)iroundf_tgt(vp[2]);
+        for (int c = 0; c < 3; ++c) out[c] = grad_voxel_norm(v, x, y, z, c);
+        return;
+    }
+    float pa[3] = { smaxf(vp[0], 0.0f), smaxf(vp[1], 0.0f), smaxf(vp[2], 0.0f) };
+    float p[3];
+    size_t lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        p[i] = pa[i] - floorf(pa[i]);
+        lo[i] = (size_t)pa[i];
+        hi[i] = (size_t)ceilf(pa[i]);
+        size_t m = (size_t)v->dims[i] - 1;
+        hi[i] = hi[i] < m ? hi[i] : m;
+        lo[i] = lo[i] < m ? lo[i] : m;
+    }
+    float ix = 1.f - p[0], iy = 1.f - p[1], iz = 1.f - p[2];
+    for (int c = 0; c < 3; ++c)
+        out[c] = grad_voxel_norm(v, lo[0], lo[1], lo[2], c) * ix * iy * iz
+               + grad_voxel_norm(v, hi[0], lo[1], lo[2], c) * p[0] * iy * iz
+               + grad_voxel_norm(v, hi[0], hi[1], lo[2], c) * p[0] * p[1] * iz
+               + grad_voxel_norm(v, lo[0], hi[1], lo[2], c) * ix * p[1] * iz
+               + grad_voxel_norm(v, lo[0], lo[1], hi[2], c) * ix * iy * p[2]
+               + grad_voxel_norm(v, hi[0], lo[1], hi[2], c) * p[0] * iy * p[2]
+               + grad_voxel_norm(v, hi[0], hi[1], hi[2], c) * p[0] * p[1] * p[2]
+               + grad_voxel_norm(v, lo[0], hi[1], hi[2], c) * ix * p[1] * p[2];
+}
+
 /* Texture space -> clamped voxel position: VolumeBase::getTextureToVoxelMatrix
  * (volumebase.cpp:221-224) applied as mat4*vec4 (matrix.h:766-771: row dot
  * product starting from 0), then tgt::clamp to [0, dim-1]
@@ -440,7 +482,7 @@ static inline double rw_to_norm_d(double rw, double lo, double hi) {
 
 /* calcGradientA, mod_gradients.frag:202-214: central differences at
  * +-1/dim in texture space, down-gradient, scaled by spacingRCP*0.5*rwmScale. */
-static void gradient_f(const dvr_volume* v, int filter, const float s[3], float g[3]) {
+static void grad_central(const dvr_volume* v, int filter, const float s[3], float g[3]) {
     float off[3], q[3], val[6];
     for (int i = 0; i < 3; ++i) off[i] = 1.f / (float)v->dims[i];
     for (int k = 0; k < 6; ++k) {
@@ -452,6 +494,96 @@ static void gradient_f(const dvr_volume* v, int filter, const float s[3], float 
     for (int i = 0; i < 3; ++i) {
         float srcp = 1.f / v->spacing[i];
         g[i] = ((val[3 + i] - val[i]) * (srcp * 0.5f)) * v->rwm_scale;
+    }
+}
+
+/* calcGradientAFD, mod_gradients.frag:116-126: v(s) - v(s + 1/dim) per axis,
+ * scaled by spacingRCP*rwmScale. */
+static void grad_forward(const dvr_volume* v, int filter, const float s[3], float g[3]) {
+    float q[3];
+    const float c = reconstruct(v, filter, s);
+    for (int axis = 0; axis < 3; ++axis) {
+        for (int i = 0; i < 3; ++i) q[i] = s[i] + ((i == axis) ? 1.f / (float)v->dims[i] : 0.0f);
+        const float n = reconstruct(v, filter, q);
+        g[axis] = ((c - n) * (1.f / v->spacing[axis])) * v->rwm_scale;
+    }
+}
+
+/* gradient "none": (voxel.xyz - 0.5)*2 (volumeraycaster.cpp:198-199) on the
+ * sample as the shader sees it: a one-channel volume texture swizzled to
+ * (R, 0, 0, R) (volumegl.cpp:367) with the real-world mapping applied to all
+ * four components (mod_sampler3d.frag:90-101). */
+static void grad_none(const dvr_volume* v, float inten, float g[3]) {
+    const float x = (inten * v->rwm_scale) + v->rwm_offset;
+    const float yz = (0.0f * v->rwm_scale) + v->rwm_offset;
+    g[0] = (x - 0.5f) * 2.0f;
+    g[1] = (yz - 0.5f) * 2.0f;
+    g[2] = (yz - 0.5f) * 2.0f;
+}
+
+/* calcGradientSobel, mod_gradients.frag:224-286.  V[n] is the shader's vNN:
+ * n-1 = xg*9 + zg*3 + yg with xg = 0, 1, 2 for x = +1, 0, -1 (in units of
+ * 1/dim), zg likewise for z = +1, 0, -1 and yg = 0, 1, 2 for y = -1, 0, +1.
+ * The three sums are evaluated left to right as written; the factors 2 and 4
+ * scale exactly. */
+static void grad_sobel(const dvr_volume* v, int filter, const float s[3], float g[3]) {
+    float V[28], q[3];
+    const float o[3] = {1.f / (float)v->dims[0], 1.f / (float)v->dims[1], 1.f / (float)v->dims[2]};
+    for (int n = 1; n <= 27; ++n) {
+        if (n == 14) { V[n] = 0.f; continue; }
+        const int k = n - 1, xg = k / 9, zg = (k / 3) % 3, yg = k % 3;
+        const int sx = 1 - xg, sz = 1 - zg, sy = yg - 1;
+        q[0] = s[0] + (sx > 0 ? o[0] : (sx < 0 ? -o[0] : 0.0f));
+        q[1] = s[1] + (sy > 0 ? o[1] : (sy < 0 ? -o[1] : 0.0f));
+        q[2] = s[2] + (sz > 0 ? o[2] : (sz < 0 ? -o[2] : 0.0f));
+        V[n] = reconstruct(v, filter, q);
+    }
+    float gx = V[1] + 2.0f * V[2] + V[3] - V[19] - 2.0f * V[20] - V[21]
+             + 2.0f * V[4] + 4.0f * V[5] + 2.0f * V[6] - 2.0f * V[22] - 4.0f * V[23] - 2.0f * V[24]
+             + V[7] + 2.0f * V[8] + V[9] - V[25] - 2.0f * V[26] - V[27];
+    float gy = -V[1] + V[3] - 2.0f * V[10] + 2.0f * V[12] - V[19] + V[21]
+             - 2.0f * V[4] + 2.0f * V[6] - 4.0f * V[13] + 4.0f * V[15] - 2.0f * V[22] + 2.0f * V[24]
+             - V[7] + V[9] - 2.0f * V[16] + 2.0f * V[18] - V[25] + V[27];
+    float gz = V[1] + 2.0f * V[2] + V[3] + 2.0f * V[10] + 4.0f * V[11] + 2.0f * V[12] + V[19] + 2.0f * V[20] + V[21]
+             - V[7] - 2.0f * V[8] - V[9] - 2.0f * V[16] - 4.0f * V[17] - 2.0f * V[18] - V[25] - 2.0f * V[26] - V[27];
+    const float gs[3] = {-gx, -gy, -gz};
+    for (int i = 0; i < 3; ++i) {
+        float srcp = 1.f / v->spacing[i];
+        g[i] = ((gs[i] * (srcp * 0.5f)) / 16.0f) * v->rwm_scale;
+    }
+}
+
+/* calcGradientFiltered, mod_gradients.frag:298-314: the central gradient at
+ * the sample and at its eight diagonal neighbours, blended with GLSL mix
+ * (x*(1-a) + y*a). */
+static void grad_filtered(const dvr_volume* v, int filter, const float s[3], float g[3]) {
+    static const int diag[8][3] = {{-1, -1, -1}, {1, 1, 1}, {-1, 1, -1}, {1, -1, 1},
+                                   {-1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {1, -1, -1}};
+    const float d[3] = {1.f / (float)v->dims[0], 1.f / (float)v->dims[1], 1.f / (float)v->dims[2]};
+    float G[9][3], q[3];
+    grad_central(v, filter, s, G[0]);
+    for (int k = 0; k < 8; ++k) {
+        for (int i = 0; i < 3; ++i) q[i] = s[i] + (diag[k][i] > 0 ? d[i] : -d[i]);
+        grad_central(v, filter, q, G[k + 1]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        const float m12 = G[1][i] * 0.5f + G[2][i] * 0.5f, m34 = G[3][i] * 0.5f + G[4][i] * 0.5f;
+        const float m56 = G[5][i] * 0.5f + G[6][i] * 0.5f, m78 = G[7][i] * 0.5f + G[8][i] * 0.5f;
+        const float mix0 = m12 * 0.5f + m34 * 0.5f, mix1 = m56 * 0.5f + m78 * 0.5f;
+        const float m = mix0 * 0.5f + mix1 * 0.5f;
+        g[i] = G[0][i] * 0.25f + m * 0.75f;
+    }
+}
+
+/* CALC_GRADIENT as configured by the "gradient" property
+ * (volumeraycaster.cpp:197-207). */
+static void gradient_f(const dvr_volume* v, const dvr_params* P, const float s[3], float inten, float g[3]) {
+    switch (P->gradient) {
+    case DVR_GRAD_FORWARD: grad_forward(v, P->filter, s, g); break;
+    case DVR_GRAD_NONE: grad_none(v, inten, g); break;
+    case DVR_GRAD_FILTERED: grad_filtered(v, P->filter, s, g); break;
+    case DVR_GRAD_SOBEL: grad_sobel(v, P->filter, s, g); break;
+    default: grad_central(v, P->filter, s, g); break;
     }
 }
 
@@ -634,6 +766,10 @@ static void shade_f(const dvr_params* P, const float g[3], const float pos[3], c
     }
 }
 
+void dvr_gradient(const dvr_volume* v, const dvr_params* P, const float s[3], float g[3]) {
+    gradient_f(v, P, s, reconstruct(v, P->filter, s), g);
+}
+
 void dvr_shade(const dvr_params* P, const float g[3], const float pos[3], const float c[3], float out[3]) {
     shade_f(P, g, pos, c, out);
 }
@@ -714,7 +850,7 @@ static int64_t march_modes_f(const dvr_volume* v, const dvr_tf* tf, const dvr_pa
             const int need_grad = P->shading != DVR_SHADE_NONE || mode == DVR_COMP_FHN;
             if (need_grad) {
                 if (t == 0.0f) clip_border_gradient(entry, w, h, px, py, s, dir, gr);
-                else gradient_f(v, P->filter, s, gr);
+                else gradient_f(v, P, s, inten, gr);
             }
             if (P->shading != DVR_SHADE_NONE) {
                 float pos[3], c[3] = {col[0], col[1], col[2]}, sh[3];
@@ -831,6 +967,24 @@ static int64_t march_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
             const float* c = P->preint_table + 4 * ((size_t)i2 * N + i1);
             cr = c[0]; cg = c[1]; cb = c[2]; ca = c[3];
             last_int = x;
+        } else if (P->classification == DVR_CLASS_TF2D) {
+            /* cpuraycaster.cpp:270-290 with apply2DTF :334-338: intensity and
+             * gradient magnitude (tgt::length) each mapped by its domain, the
+             * texel at the truncated positions, then opacity correction */
+            float vp[3], gv[3];
+            tex_to_voxel(v, s, vp);
+            grad_volume_sample(v, P->filter, vp, gv);
+            float lsq = 0.0f;
+            lsq += gv[0] * gv[0];
+            lsq += gv[1] * gv[1];
+            lsq += gv[2] * gv[2];
+            const float xi = rw_to_norm(rw, tf->domain2d[0], tf->domain2d[1]);
+            const float xg = rw_to_norm(sqrtf(lsq), tf->domain2d[2], tf->domain2d[3]);
+            const size_t ix = (size_t)(xi * (float)(tf->width2d - 1));
+            const size_t iy = (size_t)(xg * (float)(tf->height2d - 1));
+            const float* c = tf->rgba2d + 4 * (iy * (size_t)tf->width2d + ix);
+            cr = c[0]; cg = c[1]; cb = c[2];
+            ca = dvr_opacity_correct(c[3], step);
         } else {
             float x = rw_to_norm(rw, tf->domain_lo, tf->domain_hi);
             int idx = iroundf_tgt(x * (float)W1);
@@ -846,7 +1000,7 @@ static int64_t march_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
             if (P->shading != DVR_SHADE_NONE) {
                 float gr[3], pos[3], c[3] = {cr, cg, cb}, sh[3];
                 if (t == 0.0f) clip_border_gradient(entry, w, h, px, py, s, dir, gr);
-                else gradient_f(v, P->filter, s, gr);
+                else gradient_f(v, P, s, inten, gr);
                 tex_to_phys_f(v, s, pos);
                 shade_f(P, gr, pos, c, sh);
                 cr = sh[0]; cg = sh[1]; cb = sh[2];
@@ -896,8 +1050,11 @@ static int64_t march_d(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
         for (int i = 0; i < 4; ++i) res[i] = rf[i];
         return k;
     }
-    if (P->shading != DVR_SHADE_NONE && P->shading != DVR_SHADE_PHONG) {
-        /* and for Phong among the shading models; the others are the FP32 result */
+    if ((P->shading != DVR_SHADE_NONE && P->shading != DVR_SHADE_PHONG) ||
+        (P->shading != DVR_SHADE_NONE && P->gradient != DVR_GRAD_CENTRAL) ||
+        P->classification == DVR_CLASS_TF2D) {
+        /* and for Phong on central differences with 1D classification; the
+         * other shading models, gradients and the 2D TF are the FP32 result */
         float rf[4];
         const int64_t k = march_f(v, tf, P, first, last, entry, w, h, px, py, rf);
         for (int i = 0; i < 4; ++i) res[i] = rf[i];

@@ -42,9 +42,12 @@
 extern "C" {
 #endif
 
-enum { DVR_U8 = 0, DVR_U16 = 1 };
+enum { DVR_U8 = 0, DVR_U16 = 1, DVR_F32 = 2 };  /* DVR_F32: gradient volumes only */
 enum { DVR_FILTER_NEAREST = 0, DVR_FILTER_LINEAR = 1 };
-enum { DVR_CLASS_TF = 0, DVR_CLASS_PREINT = 1 };
+enum { DVR_CLASS_TF = 0, DVR_CLASS_PREINT = 1, DVR_CLASS_TF2D = 2 };
+/* VolumeRaycaster "gradient" option keys (volumeraycaster.cpp:126-131,197-207) */
+enum { DVR_GRAD_CENTRAL = 0, DVR_GRAD_FORWARD = 1, DVR_GRAD_NONE = 2, DVR_GRAD_FILTERED = 3,
+       DVR_GRAD_SOBEL = 4 };
 /* shading models, glsl.cpp:397-434 option keys in this order */
 enum { DVR_SHADE_NONE = 0, DVR_SHADE_PHONG = 1, DVR_SHADE_PHONG_D = 2, DVR_SHADE_PHONG_S = 3,
        DVR_SHADE_PHONG_DA = 4, DVR_SHADE_PHONG_DS = 5, DVR_SHADE_TOON = 6, DVR_SHADE_COOK_TORRANCE = 7,
@@ -63,6 +66,11 @@ typedef struct {
     float rwm_offset;
     float spacing[3];
     float offset[3];
+    /* gradient volume of the 2D TF path (cpuraycaster.cpp:134-141,270-288): same
+     * dims, grad_channels (3 or 4) interleaved channels of grad_dtype per voxel */
+    const void* grad_data;
+    int32_t grad_dtype;     /* DVR_U8 / DVR_U16 / DVR_F32 */
+    int32_t grad_channels;
 } dvr_volume;
 
 typedef struct {
@@ -70,6 +78,13 @@ typedef struct {
     int32_t width;
     float domain_lo;
     float domain_hi;
+    /* 2D TF texture (TransFunc2D::getTexture, float RGBA, texel (x, y) at
+     * y*width2d + x) and its per-dimension domains {intensity lo, hi, gradient
+     * magnitude lo, hi} (transfunc2d.cpp:229-231) */
+    const float* rgba2d;
+    int32_t width2d;
+    int32_t height2d;
+    float domain2d[4];
 } dvr_tf;
 
 typedef struct {
@@ -89,6 +104,7 @@ typedef struct {
     int32_t compositing;    /* DVR_COMP_*; modes other than DVR need TF classification */
     float gamma;            /* MIDA gammaValue, [-1, 1] (volumeraycaster.cpp:67) */
     float iso_value;        /* ISO isoValue (volumeraycaster.cpp:57) */
+    int32_t gradient;       /* DVR_GRAD_*: gradient of shading and FHN */
 } dvr_params;
 
 typedef struct {
@@ -136,6 +152,8 @@ int64_t dvr_fp64_render(const dvr_volume* vol, const dvr_tf* tf, const dvr_param
 /* one shading evaluation (params' model, light, camera, shininess) of TF
  * colour c with gradient g at physical position pos (for unit tests) */
 void dvr_shade(const dvr_params* prm, const float g[3], const float pos[3], const float c[3], float out[3]);
+/* CALC_GRADIENT of prm->gradient at texture-space position s (FP32). */
+void dvr_gradient(const dvr_volume* vol, const dvr_params* prm, const float s[3], float g[3]);
 
 /* single trilinear reconstruction, volumeram.cpp:78-97 (for unit tests) */
 float dvr_sample_linear(const dvr_volume* vol, float vx, float vy, float vz);

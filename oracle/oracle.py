@@ -18,9 +18,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libdvr_oracle.so")
 
-U8, U16 = 0, 1
+U8, U16, F32 = 0, 1, 2
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
-CLASS_TF, CLASS_PREINT = 0, 1
+CLASS_TF, CLASS_PREINT, CLASS_TF2D = 0, 1, 2
+GRAD_CENTRAL, GRAD_FORWARD, GRAD_NONE, GRAD_FILTERED, GRAD_SOBEL = range(5)
 (SHADE_NONE, SHADE_PHONG, SHADE_PHONG_D, SHADE_PHONG_S, SHADE_PHONG_DA, SHADE_PHONG_DS, SHADE_TOON,
  SHADE_COOK_TORRANCE, SHADE_OREN_NAYAR, SHADE_WARD) = range(10)
 COMP_DVR, COMP_MIP, COMP_MOP, COMP_MIDA, COMP_ISO, COMP_FHP, COMP_FHN = range(7)
@@ -30,12 +31,15 @@ ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
 class Volume(C.Structure):
     _fields_ = [("dims", C.c_int32 * 3), ("dtype", C.c_int32), ("data", C.c_void_p),
                 ("rwm_scale", C.c_float), ("rwm_offset", C.c_float),
-                ("spacing", C.c_float * 3), ("offset", C.c_float * 3)]
+                ("spacing", C.c_float * 3), ("offset", C.c_float * 3),
+                ("grad_data", C.c_void_p), ("grad_dtype", C.c_int32), ("grad_channels", C.c_int32)]
 
 
 class TF(C.Structure):
     _fields_ = [("rgba8", C.c_void_p), ("width", C.c_int32),
-                ("domain_lo", C.c_float), ("domain_hi", C.c_float)]
+                ("domain_lo", C.c_float), ("domain_hi", C.c_float),
+                ("rgba2d", C.c_void_p), ("width2d", C.c_int32), ("height2d", C.c_int32),
+                ("domain2d", C.c_float * 4)]
 
 
 class Params(C.Structure):
@@ -45,7 +49,7 @@ class Params(C.Structure):
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
                 ("loop_cap", C.c_int32), ("compositing", C.c_int32), ("gamma", C.c_float),
-                ("iso_value", C.c_float)]
+                ("iso_value", C.c_float), ("gradient", C.c_int32)]
 
 
 class TFKey(C.Structure):
@@ -88,6 +92,8 @@ def _load():
         fn.restype = C.c_int64
     lib.dvr_shade.argtypes = [P(Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.dvr_shade.restype = None
+    lib.dvr_gradient.argtypes = [P(Volume), P(Params), C.c_void_p, C.c_void_p]
+    lib.dvr_gradient.restype = None
     lib.dvr_sample_linear.argtypes = [P(Volume), C.c_float, C.c_float, C.c_float]
     lib.dvr_sample_linear.restype = C.c_float
     lib.dvr_mlp_forward.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
@@ -204,18 +210,37 @@ def _params(step, filter_, classification, preint, shading, light, compositing=0
 def render(vol, tf_rgba8, domain, entry, exit_, step, filter_=FILTER_LINEAR,
            classification=CLASS_TF, preint=None, shading=SHADE_NONE, light=None,
            rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0),
-           fp64=False, nthreads=0, compositing=COMP_DVR, gamma=0.0, iso_value=0.5):
+           fp64=False, nthreads=0, compositing=COMP_DVR, gamma=0.0, iso_value=0.5,
+           gradient=GRAD_CENTRAL, grad_volume=None, tf2d=None, tf2d_domain=((0.0, 1.0), (0.0, 1.0))):
     """Render one frame; vol is a (Z, Y, X) uint8/uint16 array, entry/exit
-    (H, W, 4) float32.  Returns (rgba (H, W, 4), samples)."""
+    (H, W, 4) float32.  The 2D TF path (classification CLASS_TF2D) takes
+    grad_volume (Z, Y, X, 3 or 4) uint8/uint16/float32 and tf2d (H2, W2, 4)
+    float32 with its (intensity, gradient-magnitude) domains.  Returns
+    (rgba (H, W, 4), samples)."""
     v, vol_keep = _volume(vol, rwm, spacing, offset)
-    tf_rgba8 = np.ascontiguousarray(tf_rgba8, np.uint8)
+    if grad_volume is not None:
+        grad_volume = np.ascontiguousarray(grad_volume)
+        assert grad_volume.shape[:3] == vol_keep.shape and grad_volume.shape[3] in (3, 4)
+        v.grad_data = grad_volume.ctypes.data
+        v.grad_dtype = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
+                        np.dtype(np.float32): F32}[grad_volume.dtype]
+        v.grad_channels = grad_volume.shape[3]
+    tf_rgba8 = np.ascontiguousarray(np.zeros((2, 4), np.uint8) if tf_rgba8 is None else tf_rgba8, np.uint8)
     t = TF()
     t.rgba8 = tf_rgba8.ctypes.data
     t.width = tf_rgba8.shape[0]
     t.domain_lo, t.domain_hi = domain
+    if tf2d is not None:
+        tf2d = np.ascontiguousarray(tf2d, np.float32)
+        t.rgba2d = tf2d.ctypes.data
+        t.height2d, t.width2d = tf2d.shape[:2]
+        t.domain2d[:] = [tf2d_domain[0][0], tf2d_domain[0][1], tf2d_domain[1][0], tf2d_domain[1][1]]
+    if classification == CLASS_TF2D:
+        assert grad_volume is not None and tf2d is not None
     if preint is not None:
         preint = np.ascontiguousarray(preint, np.float32)
     p = _params(step, filter_, classification, preint, shading, light, compositing, gamma, iso_value)
+    p.gradient = gradient
     entry = np.ascontiguousarray(entry, np.float32)
     exit_ = np.ascontiguousarray(exit_, np.float32)
     h, w = entry.shape[:2]
@@ -240,6 +265,18 @@ def shade(model, g, pos, color, light=None, shininess=60.0):
     color = np.ascontiguousarray(color, np.float32)
     out = np.zeros(3, np.float32)
     lib().dvr_shade(C.byref(p), _ptr(g), _ptr(pos), _ptr(color), _ptr(out))
+    return out
+
+
+def gradient(vol, s, mode, filter_=FILTER_LINEAR, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0)):
+    """The oracle's gradient (CALC_GRADIENT of the given mode) at texture-space s."""
+    v, keep = _volume(vol, rwm, spacing)
+    p = _params(1.0, filter_, CLASS_TF, None, SHADE_PHONG, None)
+    p.gradient = mode
+    s = np.ascontiguousarray(s, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().dvr_gradient(C.byref(v), C.byref(p), _ptr(s), _ptr(out))
+    del keep
     return out
 
 

@@ -543,3 +543,158 @@ def test_vvd_load_and_volume_minmax(ctx, oracle, name):
         v.close()
         vm.close()
         t.close()
+
+
+GRADIENTS = [(1, "forward"), (2, "none"), (3, "filtered"), (4, "sobel")]
+
+
+@pytest.mark.parametrize("mode,name", GRADIENTS, ids=[n for _, n in GRADIENTS])
+def test_gradient_modes(ctx, oracle, mode, name):
+    """The "gradient" property's modes (mod_gradients.frag) under Phong and
+    FHN on the reference kernel, both filters, against the oracle; the
+    skipping kernels refuse them."""
+    from voreen_amd import hipdvr as H, workloads as W
+    dims = (48, 48, 48)
+    vol = W.make_volume("c1", dims=dims)
+    keys, _ = W.make_tf("c1")
+    tf = oracle.tf_build(keys, 4096)
+    cam = W.orbit_camera(11, 36)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 80, 80, llf=llf, urb=urb)
+    sp = (1.0 / 48,) * 3
+    off = tuple(float(llf[i] + 0.5 / 48) for i in range(3))
+    step = W.step_for(dims, 2.0)
+    for kernel, filt, shading, comp in ((H.KERNEL_REFERENCE, 0, 1, 0), (H.KERNEL_REFERENCE, 1, 1, 0),
+                                        (H.KERNEL_AUTO, 1, 8, 0), (H.KERNEL_AUTO, 1, 0, H.COMPOSITE_FHN)):
+        say(f"gradient {name} kernel {kernel} filter {filt} shading {shading} compositing {comp}")
+        v = ctx.volume(vol, spacing=sp, offset=off)
+        t = ctx.transfunc(tf)
+        try:
+            gpu, st = ctx.render(v, t, en, ex, H.default_params(step, filter_=filt, shading=shading, kernel=kernel,
+                                                               camera_pos=tuple(cam.eye), compositing=comp,
+                                                               gradient=mode))
+        finally:
+            v.close()
+            t.close()
+        cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step, filter_=filt, shading=shading,
+                               light=dict(camera=tuple(cam.eye)), spacing=sp, offset=off, compositing=comp,
+                               gradient=mode)
+        d = np.abs(gpu - cpu).max(axis=2)
+        print(f"{name} k{kernel} f{filt} s{shading} c{comp}: max_abs={d.max():.3g} "
+              f"exact={np.mean(np.all(gpu == cpu, axis=2)):.4f}", flush=True)
+        assert np.mean(d > TOL) < 0.002
+        assert st["samples"] == n
+        assert np.mean(gpu[..., 3] > 0) > 0.05
+    v = ctx.volume(vol)
+    t = ctx.transfunc(tf)
+    try:
+        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL):
+            with pytest.raises(H.HipDvrError) as e:
+                ctx.render(v, t, en, ex, H.default_params(step, kernel=k, shading=1, gradient=mode))
+            assert e.value.code == H.ERR_UNSUPPORTED
+        # unshaded DVR never evaluates a gradient: every kernel accepts any mode
+        img, _ = ctx.render(v, t, en, ex, H.default_params(step, kernel=H.KERNEL_CELL, gradient=mode))
+        cpu, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+        assert np.abs(img - cpu).max() <= TOL
+    finally:
+        v.close()
+        t.close()
+
+
+def tf2d_inputs(oracle, gdtype=np.float32, channels=3):
+    """Shells volume, a central-difference gradient volume of it and a 2D TF
+    that makes boundaries (large gradient magnitude) opaque."""
+    from voreen_amd import workloads as W
+    dims = (48, 40, 36)
+    vol = W.make_volume("c1", dims=dims)
+    f = vol.astype(np.float64) / 65535.0
+    g = np.stack([np.gradient(f, axis=2), np.gradient(f, axis=1), np.gradient(f, axis=0)], axis=3)
+    g = np.abs(g) * 4.0
+    if channels == 4:
+        g = np.concatenate([g, np.full(g.shape[:3] + (1,), 0.7)], axis=3)
+    gmax = {np.float32: 1.0, np.uint8: 255.0, np.uint16: 65535.0}[gdtype]
+    grad = g.astype(np.float32) if gdtype == np.float32 else np.clip(np.round(g * gmax), 0, gmax).astype(gdtype)
+    W2, H2 = 64, 32
+    xi = np.linspace(0, 1, W2)[None, :]
+    yg = np.linspace(0, 1, H2)[:, None]
+    tf2d = np.zeros((H2, W2, 4), np.float32)
+    tf2d[..., 0] = xi
+    tf2d[..., 1] = 1.0 - xi * 0.5
+    tf2d[..., 2] = yg
+    tf2d[..., 3] = np.clip(yg * 1.5 - 0.1, 0, 1) * (xi > 0.15)
+    cam = W.orbit_camera(2, 36)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, 72, 64, llf=llf, urb=urb)
+    sp = tuple(1.0 / max(dims) for _ in range(3))
+    off = tuple(float(llf[i] + 0.5 * sp[i]) for i in range(3))
+    return vol, grad, tf2d, ((0.0, 1.0), (0.0, 0.4)), en, ex, W.step_for(dims, 2.0), sp, off, tuple(cam.eye)
+
+
+@pytest.mark.parametrize("gdtype,channels", [(np.float32, 3), (np.uint8, 4), (np.uint16, 3)],
+                         ids=["f32x3", "u8x4", "u16x3"])
+def test_tf2d_classification(ctx, oracle, gdtype, channels):
+    """The CPURaycaster 2D TF path (intensity x gradient-volume magnitude)
+    against the oracle: bit-exact unshaded, Phong tolerance shaded."""
+    from voreen_amd import hipdvr as H
+    vol, grad, tf2d, dom, en, ex, step, sp, off, eye = tf2d_inputs(oracle, gdtype, channels)
+    for filt in (0, 1):
+        for shading in (0, 1):
+            say(f"tf2d grad {np.dtype(gdtype).name}x{channels} filter {filt} shading {shading}")
+            v = ctx.volume(vol, spacing=sp, offset=off)
+            v.set_gradient(grad)
+            t = ctx.transfunc2d(tf2d, dom)
+            try:
+                gpu, st = ctx.render(v, t, en, ex, H.default_params(step, filter_=filt, shading=shading,
+                                                                   classification=H.CLASS_TF2D,
+                                                                   camera_pos=eye))
+                # a second step size rebuilds the opacity-corrected table
+                gpu2, _ = ctx.render(v, t, en, ex, H.default_params(step * 0.5, filter_=filt,
+                                                                    classification=H.CLASS_TF2D))
+            finally:
+                v.close()
+                t.close()
+            cpu, n = oracle.render(vol, None, (0.0, 1.0), en, ex, step, filter_=filt, shading=shading,
+                                   classification=oracle.CLASS_TF2D, grad_volume=grad, tf2d=tf2d,
+                                   tf2d_domain=dom, light=dict(camera=eye), spacing=sp, offset=off)
+            cpu2, _ = oracle.render(vol, None, (0.0, 1.0), en, ex, step * 0.5, filter_=filt,
+                                    classification=oracle.CLASS_TF2D, grad_volume=grad, tf2d=tf2d,
+                                    tf2d_domain=dom, spacing=sp, offset=off)
+            d = np.abs(gpu - cpu).max(axis=2)
+            print(f"tf2d f{filt} s{shading}: max_abs={d.max():.3g} "
+                  f"exact={np.mean(np.all(gpu == cpu, axis=2)):.4f}", flush=True)
+            if shading:
+                assert np.mean(d > TOL) < 0.002
+            else:
+                assert d.max() <= TOL
+            assert np.abs(gpu2 - cpu2).max() <= TOL
+            assert st["samples"] == n
+            assert np.mean(gpu[..., 3] > 0) > 0.05
+
+
+def test_tf2d_rejections(ctx, oracle):
+    from voreen_amd import hipdvr as H
+    vol, grad, tf2d, dom, en, ex, step, sp, off, eye = tf2d_inputs(oracle)
+    say("tf2d: invalid combinations")
+    v = ctx.volume(vol)
+    t2 = ctx.transfunc2d(tf2d, dom)
+    t1 = ctx.transfunc(np.zeros((256, 4), np.uint8))
+    try:
+        cases = [(t2, dict(), H.ERR_INVALID),                                   # 2D TF, 1D classification
+                 (t1, dict(classification=H.CLASS_TF2D), H.ERR_INVALID),        # 1D TF, 2D classification
+                 (t2, dict(classification=H.CLASS_TF2D), H.ERR_INVALID)]        # no gradient volume yet
+        for t, kw, code in cases:
+            with pytest.raises(H.HipDvrError) as e:
+                ctx.render(v, t, en, ex, H.default_params(step, **kw))
+            assert e.value.code == code
+        v.set_gradient(grad)
+        for kw in (dict(kernel=H.KERNEL_CELL), dict(kernel=H.KERNEL_BRICKED),
+                   dict(compositing=H.COMPOSITE_MIP)):
+            with pytest.raises(H.HipDvrError) as e:
+                ctx.render(v, t2, en, ex, H.default_params(step, classification=H.CLASS_TF2D, **kw))
+            assert e.value.code == H.ERR_UNSUPPORTED
+        with pytest.raises(ValueError):
+            v.set_gradient(grad[:, :, :-1])
+    finally:
+        v.close()
+        t1.close()
+        t2.close()

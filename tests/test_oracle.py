@@ -531,3 +531,123 @@ def test_shading_models_closed_forms(oracle):
     ax, ay = sh / 128 + 1e-7, 2 * sh / 128 + 1e-7
     nenner = 4 * 3.14159 * ax * ay
     assert np.allclose(shade(oracle.SHADE_WARD), amb * c + (spe / nenner + dif * c), rtol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# gradient modes (mod_gradients.frag) — closed forms on polynomial fields
+# --------------------------------------------------------------------------
+
+_G8 = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+
+
+@pytest.mark.parametrize("name,field,mode,expect", [
+    # every mode returns the down-gradient of an affine field
+    ("affine-central", lambda x, y, z: 300 * x + 500 * y + 700 * z, 0, lambda X, Y, Z: (-300, -500, -700)),
+    ("affine-forward", lambda x, y, z: 300 * x + 500 * y + 700 * z, 1, lambda X, Y, Z: (-300, -500, -700)),
+    ("affine-filtered", lambda x, y, z: 300 * x + 500 * y + 700 * z, 3, lambda X, Y, Z: (-300, -500, -700)),
+    ("affine-sobel", lambda x, y, z: 300 * x + 500 * y + 700 * z, 4, lambda X, Y, Z: (-300, -500, -700)),
+    # the modes differ on higher orders: forward differences of x^2 are
+    # -(2x+1); Sobel smooths y^2 across the x-planes with weights 1,2,1 (adds
+    # 8/16 * 1); the filtered gradient of x^3 blends central differences
+    # -(3x^2+1) at x and its diagonal neighbours 1/4 : 3/4
+    ("x2-forward", lambda x, y, z: 900 * x ** 2, 1, lambda X, Y, Z: (-900 * (2 * X + 1), 0, 0)),
+    ("xy2-central", lambda x, y, z: 100 * x * y * y, 0, lambda X, Y, Z: (-100 * Y * Y, -200 * X * Y, 0)),
+    ("xy2-sobel", lambda x, y, z: 100 * x * y * y, 4, lambda X, Y, Z: (-100 * (Y * Y + 0.5), -200 * X * Y, 0)),
+    ("x3-filtered", lambda x, y, z: 190 * x ** 3, 3, lambda X, Y, Z: (-190 * (3 * X * X + 3.25), 0, 0)),
+])
+def test_gradient_modes_closed_forms(oracle, name, field, mode, expect):
+    z, y, x = _G8
+    vol = field(x, y, z).astype(np.uint16)
+    for X in range(2, 6):
+        for Y in range(2, 6):
+            for Z in range(2, 6):
+                s = [(X + 0.5) / 8, (Y + 0.5) / 8, (Z + 0.5) / 8]
+                g = oracle.gradient(vol, s, mode).astype(np.float64) * 65535.0
+                assert np.allclose(g, expect(X, Y, Z), atol=2e-3), (name, X, Y, Z, g)
+
+
+def test_gradient_spacing_rwm_and_none(oracle):
+    z, y, x = _G8
+    vol = (300 * x + 500 * y + 700 * z).astype(np.uint16)
+    s = [4.5 / 8, 3.5 / 8, 2.5 / 8]
+    sp, scale = (0.5, 2.0, 4.0), 3.0
+    for mode in (0, 1, 3, 4):
+        g = oracle.gradient(vol, s, mode, rwm=(scale, 0.25), spacing=sp).astype(np.float64) * 65535.0
+        assert np.allclose(g, [-300 * scale / 0.5, -500 * scale / 2.0, -700 * scale / 4.0], rtol=1e-5)
+    # "none": ((rw - 0.5) * 2, (offset - 0.5) * 2, (offset - 0.5) * 2)
+    v = (300 * 4 + 500 * 3 + 700 * 2) / 65535.0
+    g = oracle.gradient(vol, s, oracle.GRAD_NONE, rwm=(scale, 0.25))
+    assert np.allclose(g, [(v * scale + 0.25 - 0.5) * 2, -0.5, -0.5], atol=1e-5)
+
+
+# --------------------------------------------------------------------------
+# 2D TF (CPURaycaster TFT_2DPRIMITIVES path) — single-sample rays
+# --------------------------------------------------------------------------
+
+def _single_sample_rays(rng, n):
+    pts = rng.uniform(0.02, 0.98, (1, n, 3)).astype(np.float32)
+    en = np.concatenate([pts, np.ones((1, n, 1), np.float32)], axis=2)
+    return en, en.copy()
+
+
+@pytest.mark.parametrize("filter_,gdtype,channels", [(1, np.float32, 3), (0, np.float32, 3),
+                                                     (1, np.uint8, 4), (1, np.uint16, 3)])
+def test_tf2d_single_samples(oracle, filter_, gdtype, channels):
+    """Degenerate rays take one sample, so the output is (a'*rgb, a') of the
+    texel at (trunc(xi*(W-1)), trunc(xg*(H-1))) with xi, xg the intensity and
+    the gradient-volume magnitude mapped by the two domains."""
+    dims = (12, 10, 9)                       # x, y, z
+    z, y, x = np.meshgrid(*(np.arange(d) for d in dims[::-1]), indexing="ij")
+    vol = (5000 * x).astype(np.uint16)
+    gmax = {np.float32: 1.0, np.uint8: 255.0, np.uint16: 65535.0}[gdtype]
+    gx = 0.08 * y / 9.0                      # gradient volume channels in [0, 0.08]
+    grad = np.zeros(vol.shape + (channels,), np.float64)
+    grad[..., 0] = gx
+    grad[..., 1] = 0.03
+    grad[..., 2] = 0.0
+    if channels == 4:
+        grad[..., 3] = 0.9                   # ignored
+    graw = (grad * gmax).astype(gdtype) if gdtype != np.float32 else grad.astype(np.float32)
+    W, H = 17, 9
+    tf2d = np.zeros((H, W, 4), np.float32)
+    tf2d[..., 0] = np.arange(W)[None, :] / (W - 1)
+    tf2d[..., 1] = np.arange(H)[:, None] / (H - 1)
+    tf2d[..., 2] = 0.5
+    tf2d[..., 3] = 0.1 + 0.8 * (np.arange(H)[:, None] * W + np.arange(W)[None, :]) / (W * H)
+    dom = ((0.0, 1.0), (0.02, 0.1))
+    rng = np.random.default_rng(3)
+    en, ex = _single_sample_rays(rng, 400)
+    step = 1.0 / 400
+    img, n = oracle.render(vol, None, (0.0, 1.0), en, ex, step, filter_=filter_,
+                           classification=oracle.CLASS_TF2D, grad_volume=graw, tf2d=tf2d, tf2d_domain=dom)
+    assert n == 400
+    checked = 0
+    for i in range(400):
+        vp = np.clip(en[0, i, :3].astype(np.float64) * dims - 0.5, 0, np.array(dims) - 1)
+        if filter_ == 0:
+            vp = np.floor(vp + 0.5)
+        gxs = (np.floor(vp[1]) if filter_ == 0 else vp[1])
+        # channel values as stored (quantized for integer types), then trilinear = affine in y
+        if gdtype == np.float32:
+            c0 = 0.08 * gxs / 9.0
+            c1 = np.float64(np.float32(0.03))
+        else:
+            # the stored integers are affine only per voxel: interpolate them
+            y0, y1 = int(np.floor(gxs)), min(int(np.floor(gxs)) + 1, dims[1] - 1)
+            f = gxs - np.floor(gxs)
+            q0 = graw[0, y0, 0, 0] / gmax
+            q1 = graw[0, y1, 0, 0] / gmax
+            c0 = q0 * (1 - f) + q1 * f
+            c1 = graw[0, 0, 0, 1] / gmax
+        mag = np.sqrt(c0 * c0 + c1 * c1)
+        xi = min(max(5000 * vp[0] / 65535.0, 0.0), 1.0)
+        xg = min(max((mag - 0.02) / 0.08, 0.0), 1.0)
+        fi, fg = xi * (W - 1), xg * (H - 1)
+        if min(abs(fi - round(fi)), abs(fg - round(fg))) < 1e-3:
+            continue                         # a truncation boundary: float rounding decides
+        tex = tf2d[int(fg), int(fi)]
+        a = oracle.opacity_correct(float(tex[3]), step)
+        want = np.array([a * tex[0], a * tex[1], a * tex[2], 1.0 if a >= 0.95 else a], np.float32)
+        assert np.allclose(img[0, i], want, atol=2e-6), (i, img[0, i], want)
+        checked += 1
+    assert checked > 300

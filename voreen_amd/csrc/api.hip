@@ -89,14 +89,20 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
     if (!(p->step > 0.f) || !std::isfinite(p->step)) fail(HIPDVR_ERR_INVALID, "step must be > 0");
     if (p->filter != HIPDVR_FILTER_NEAREST && p->filter != HIPDVR_FILTER_LINEAR)
         fail(HIPDVR_ERR_INVALID, "unknown filter mode");
-    if (p->classification != HIPDVR_CLASS_TF && p->classification != HIPDVR_CLASS_PREINT)
+    if (p->classification != HIPDVR_CLASS_TF && p->classification != HIPDVR_CLASS_PREINT &&
+        p->classification != HIPDVR_CLASS_TF2D)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported classification mode");
+    if ((p->classification == HIPDVR_CLASS_TF2D) != tf->is2d)
+        fail(HIPDVR_ERR_INVALID, tf->is2d ? "a 2D transfer function needs HIPDVR_CLASS_TF2D"
+                                          : "HIPDVR_CLASS_TF2D needs a 2D transfer function (hipdvr_tf2d_create)");
     if (p->classification == HIPDVR_CLASS_PREINT && (!tf->d_preint || tf->preint_n < 2))
         fail(HIPDVR_ERR_INVALID, "pre-integrated classification needs a table (hipdvr_tf_set_preint)");
+    if (p->gradient < HIPDVR_GRADIENT_CENTRAL || p->gradient > HIPDVR_GRADIENT_SOBEL)
+        fail(HIPDVR_ERR_UNSUPPORTED, "unknown gradient mode");
     if (p->compositing < HIPDVR_COMPOSITE_DVR || p->compositing > HIPDVR_COMPOSITE_FHN)
         fail(HIPDVR_ERR_UNSUPPORTED, "unknown compositing mode");
     if (p->compositing != HIPDVR_COMPOSITE_DVR && p->classification != HIPDVR_CLASS_TF)
-        fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR need transfer-function classification");
+        fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR need 1D transfer-function classification");
     if (!std::isfinite(p->gamma) || !std::isfinite(p->iso_value))
         fail(HIPDVR_ERR_INVALID, "gamma and iso_value must be finite");
     if (p->shading < HIPDVR_SHADE_NONE || p->shading > HIPDVR_SHADE_WARD)
@@ -129,6 +135,7 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
         a.offset[i] = v.offset[i];
         a.grad_off[i] = 1.f / (float)v.dims[i];
         a.grad_scale[i] = (1.f / v.spacing[i]) * 0.5f;
+        a.spacing_rcp[i] = 1.f / v.spacing[i];
         a.light_pos[i] = p->light_pos[i];
         a.light_amb[i] = p->light_ambient[i];
         a.light_dif[i] = p->light_diffuse[i];
@@ -164,7 +171,33 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     a.compositing = p->compositing;
     a.gamma = p->gamma;
     a.iso_value = p->iso_value;
+    a.gradient = p->gradient;
+    a.gvol = vol->d_grad;
+    a.gvol_dtype = vol->grad_dtype;
+    a.gvol_channels = (int32_t)vol->grad_channels;
+    a.tf2d = tf->d_tf2d;
+    a.tf2d_w = (int32_t)tf->w2d;
+    a.tf2d_h = (int32_t)tf->h2d;
+    for (int i = 0; i < 4; ++i) a.tf2d_dom[i] = tf->dom2d[i];
     fill_alpha_table(p->step, a.alpha_corr);
+}
+
+void ensure_tf2d(hipdvr_tf* tf, float step, hipStream_t st) {
+    if (tf->d_tf2d && tf->tf2d_step == step) return;
+    const size_t n = (size_t)tf->w2d * tf->h2d;
+    // kernels of earlier renders may still read the device table and the
+    // upload below reads the host copy asynchronously
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!tf->d_tf2d) hip_check(hipMalloc((void**)&tf->d_tf2d, n * sizeof(float4)), "hipMalloc(2D TF)");
+    tf->host2d_corr = tf->host2d;
+    const float e = step * 200.0f;   // samplingStepSize * SAMPLING_BASE_INTERVAL_RCP
+    for (size_t i = 0; i < n; ++i) {
+        float& a = tf->host2d_corr[4 * i + 3];
+        a = 1.f - std::pow(1.f - a, e);
+    }
+    hip_check(hipMemcpyAsync(tf->d_tf2d, tf->host2d_corr.data(), n * sizeof(float4), hipMemcpyHostToDevice, st),
+              "2D TF upload");
+    tf->tf2d_step = step;
 }
 
 }  // namespace hipdvr
@@ -186,6 +219,9 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     // any shading model: the gradient plus lighting; the fast cell variant
     // (interior gradient, hardware lighting terms) only for Phong
     const bool phong = p->shading != HIPDVR_SHADE_NONE;
+    // a gradient other than central differences (shading or FHN) runs on the reference kernels
+    const bool other_gradient = p->gradient != HIPDVR_GRADIENT_CENTRAL &&
+                                (phong || p->compositing == HIPDVR_COMPOSITE_FHN);
     int kernel = p->kernel;
     // the skipping kernels drop samples, which the pre-integrated mode cannot
     // (its lookup pairs every sample with the previous one); the cell records
@@ -193,8 +229,13 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     const bool modes = p->compositing != HIPDVR_COMPOSITE_DVR;
     if (modes && kernel != HIPDVR_KERNEL_AUTO && kernel != HIPDVR_KERNEL_REFERENCE)
         fail(HIPDVR_ERR_UNSUPPORTED, "compositing modes other than DVR run on the reference kernel");
+    if (other_gradient && kernel != HIPDVR_KERNEL_AUTO && kernel != HIPDVR_KERNEL_REFERENCE)
+        fail(HIPDVR_ERR_UNSUPPORTED, "gradient modes other than central differences run on the reference kernel");
+    if (p->classification == HIPDVR_CLASS_TF2D && (!vol->d_grad || vol->grad_channels < 3))
+        fail(HIPDVR_ERR_INVALID, "2D transfer functions need an RGB or RGBA gradient volume "
+                                 "(hipdvr_volume_set_gradient)");
     if (kernel == HIPDVR_KERNEL_AUTO) {
-        if (p->classification != HIPDVR_CLASS_TF || modes) kernel = HIPDVR_KERNEL_REFERENCE;
+        if (p->classification != HIPDVR_CLASS_TF || modes || other_gradient) kernel = HIPDVR_KERNEL_REFERENCE;
         else if (p->filter == HIPDVR_FILTER_LINEAR && cells_allowed(vol)) kernel = HIPDVR_KERNEL_CELL;
         else if (p->shading == HIPDVR_SHADE_NONE || p->shading == HIPDVR_SHADE_PHONG) kernel = HIPDVR_KERNEL_BRICKED;
         else kernel = HIPDVR_KERNEL_REFERENCE;
@@ -207,6 +248,11 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     if (kernel == HIPDVR_KERNEL_BRICKED && p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
         fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel shades with Phong only");
     hipdvr_volume* mvol = const_cast<hipdvr_volume*>(vol);
+    if (p->classification == HIPDVR_CLASS_TF2D) {
+        hipdvr_tf* mtf = const_cast<hipdvr_tf*>(tf);
+        ensure_tf2d(mtf, p->step, ctx->stream);
+        a.tf2d = mtf->d_tf2d;
+    }
     if (kernel == HIPDVR_KERNEL_BRICKED || cell) ensure_occupancy(ctx, mvol, tf);
     if (cell) {
         ensure_cells(mvol, ctx->stream);
@@ -424,8 +470,56 @@ int hipdvr_tf_destroy(hipdvr_tf* tf) {
     if (tf->d_rgba) (void)hipFree(tf->d_rgba);
     if (tf->d_alpha_prefix) (void)hipFree(tf->d_alpha_prefix);
     if (tf->d_preint) (void)hipFree(tf->d_preint);
+    if (tf->d_tf2d) (void)hipFree(tf->d_tf2d);
     delete tf;
     return HIPDVR_OK;
+}
+
+int hipdvr_tf2d_create(hipdvr_ctx* ctx, const float* rgba, uint32_t width, uint32_t height,
+                       const float domain[4], hipdvr_tf** out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (!out || !rgba || !domain) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (width < 2 || height < 2 || width > 4096 || height > 4096)
+            fail(HIPDVR_ERR_INVALID, "2D TF width and height must be in [2, 4096]");
+        if (!(domain[0] <= domain[1]) || !(domain[2] <= domain[3]))
+            fail(HIPDVR_ERR_INVALID, "invalid 2D transfer function domain");
+        hipdvr_tf* tf = new hipdvr_tf();
+        tf->ctx = ctx;
+        tf->serial = next_serial();
+        tf->is2d = true;
+        tf->w2d = width;
+        tf->h2d = height;
+        for (int i = 0; i < 4; ++i) tf->dom2d[i] = domain[i];
+        tf->host2d.assign(rgba, rgba + 4 * (size_t)width * height);
+        *out = tf;
+    });
+}
+
+int hipdvr_volume_set_gradient(hipdvr_volume* vol, const void* host_data, int32_t dtype, uint32_t channels) {
+    if (!vol) return HIPDVR_ERR_INVALID;
+    return guarded(vol->ctx, [&] {
+        if (!host_data) fail(HIPDVR_ERR_INVALID, "NULL argument");
+        if (dtype != HIPDVR_U8 && dtype != HIPDVR_U16 && dtype != HIPDVR_F32)
+            fail(HIPDVR_ERR_INVALID, "gradient volume dtype must be HIPDVR_U8, HIPDVR_U16 or HIPDVR_F32");
+        if (channels != 3 && channels != 4)
+            fail(HIPDVR_ERR_INVALID, "gradient volumes have 3 (RGB) or 4 (RGBA) channels");
+        const size_t esz = dtype == HIPDVR_U8 ? 1 : (dtype == HIPDVR_U16 ? 2 : 4);
+        const size_t bytes = (size_t)vol->desc.dims[0] * vol->desc.dims[1] * vol->desc.dims[2] * channels * esz;
+        set_device(vol->ctx);
+        hip_check(hipStreamSynchronize(vol->ctx->stream), "hipStreamSynchronize");
+        void* d = nullptr;
+        hip_check(hipMalloc(&d, bytes), "hipMalloc(gradient volume)");
+        const hipError_t e = hipMemcpy(d, host_data, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            hip_check(e, "gradient volume upload");
+        }
+        if (vol->d_grad) (void)hipFree(vol->d_grad);
+        vol->d_grad = d;
+        vol->grad_dtype = dtype;
+        vol->grad_channels = channels;
+    });
 }
 
 int hipdvr_tf_set_preint(hipdvr_tf* tf, const float* table, uint32_t n) {

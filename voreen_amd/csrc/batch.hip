@@ -209,6 +209,8 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             if (j.params.filter != HIPDVR_FILTER_LINEAR || j.params.classification != HIPDVR_CLASS_TF ||
                 j.params.compositing != HIPDVR_COMPOSITE_DVR)
                 fail(HIPDVR_ERR_UNSUPPORTED, "batched jobs use the linear filter, TF classification and DVR");
+            if (j.params.shading != HIPDVR_SHADE_NONE && j.params.gradient != HIPDVR_GRADIENT_CENTRAL)
+                fail(HIPDVR_ERR_UNSUPPORTED, "batched jobs shade with central-difference gradients");
             const int ph = j.params.shading == HIPDVR_SHADE_NONE ? 0
                          : (j.params.shading == HIPDVR_SHADE_PHONG && !j.params.exact_shading) ? 1 : 2;
             if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading path");

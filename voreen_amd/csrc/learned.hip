@@ -373,6 +373,8 @@ int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_
         if (!tf || !prm) fail(HIPDVR_ERR_INVALID, "NULL argument");
         if (tf->ctx != l->ctx || (mlp && mlp->ctx != l->ctx)) fail(HIPDVR_ERR_INVALID, "handle from another context");
         validate_params(prm, tf);
+        if (prm->classification != HIPDVR_CLASS_TF)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
         const hipdvr_learned_opts opts = default_opts(o);
         ctx_set_device(l->ctx);
         hipStream_t st = ctx_stream(l->ctx);
@@ -459,6 +461,8 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
             fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
         if (prm->shading != HIPDVR_SHADE_NONE && prm->shading != HIPDVR_SHADE_PHONG)
             fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping shades with Phong or not at all");
+        if (prm->shading != HIPDVR_SHADE_NONE && prm->gradient != HIPDVR_GRADIENT_CENTRAL)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping shades with central-difference gradients");
         if (prm->compositing != HIPDVR_COMPOSITE_DVR)
             fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping composites DVR");
         const hipdvr_learned_opts opts = default_opts(o);

@@ -24,6 +24,8 @@ constexpr int kShadeNone = 0, kShadePhong = 1, kShadePhongD = 2, kShadePhongS = 
 // compositing modes (values of hipdvr_params::compositing)
 constexpr int kCompDVR = 0, kCompMIP = 1, kCompMOP = 2, kCompMIDA = 3, kCompISO = 4, kCompFHP = 5,
               kCompFHN = 6;
+// gradient modes (VolumeRaycaster "gradient" property, volumeraycaster.cpp:126-131,197-207)
+constexpr int kGradCentral = 0, kGradForward = 1, kGradNone = 2, kGradFiltered = 3, kGradSobel = 4;
 
 // Arguments of one march launch (plain data; passed by value as kernarg).
 struct MarchArgs {
@@ -48,12 +50,21 @@ struct MarchArgs {
     const uint32_t* tf;         // RGBA8 packed little-endian (r | g<<8 | b<<16 | a<<24)
     const float4* preint;       // N*N, [fe*N + fs]
     int32_t preint_n;
+    // 2D TF classification: gradient volume (same dims, gvol_channels interleaved
+    // channels of u8 / u16 / f32 per voxel) and the opacity-corrected table
+    const void* gvol;
+    int32_t gvol_dtype, gvol_channels;
+    const float4* tf2d;         // [y*tf2d_w + x]
+    int32_t tf2d_w, tf2d_h;
+    float tf2d_dom[4];          // intensity lo, hi; gradient magnitude lo, hi
     float step;
     int32_t loop_cap;
     // Phong
     float spacing[3], offset[3];
     float grad_off[3];          // 1/dims (datasetDimensionsRCP_)
     float grad_scale[3];        // (1/spacing) * 0.5
+    float spacing_rcp[3];       // 1/spacing (forward differences)
+    int32_t gradient;           // kGrad*
     float light_pos[3], light_amb[3], light_dif[3], light_spe[3], shininess;
     float cam_pos[3];
     // images

@@ -58,6 +58,154 @@ __device__ __forceinline__ float reconstruct(const LinearFetch<T>& f, const Marc
     return trilerp(v, c.p);
 }
 
+// calcGradientA, mod_gradients.frag:202-214: central differences at +-1/dim
+// in texture space (down-gradient), times spacingRCP*0.5 and rwmScale.
+template <typename T, int FILTER>
+__device__ __forceinline__ void grad_central(const LinearFetch<T>& f, const MarchArgs& a, const float s[3],
+                                             float gr[3]) {
+    float vals[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int axis = k % 3;
+        const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
+        float q[3] = {s[0], s[1], s[2]};
+        q[axis] = s[axis] + off;
+        q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
+        q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
+        vals[k] = reconstruct<T, FILTER>(f, a, q);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+}
+
+// calcGradientSobel, mod_gradients.frag:224-286.  V[n] is the shader's vNN:
+// n-1 = xg*9 + zg*3 + yg, xg = 0, 1, 2 for x = +1, 0, -1 (in units of 1/dim),
+// zg likewise for z, yg = 0, 1, 2 for y = -1, 0, +1.  The sums run left to
+// right as written; the factors 2 and 4 scale exactly.
+template <typename T, int FILTER>
+__device__ __noinline__ void grad_sobel(const LinearFetch<T>& f, const MarchArgs& a, const float s[3], float gr[3]) {
+    float V[28];
+    V[0] = V[14] = 0.f;
+#pragma unroll
+    for (int n = 1; n <= 27; ++n) {
+        if (n == 14) continue;
+        const int k = n - 1, sx = 1 - k / 9, sz = 1 - (k / 3) % 3, sy = k % 3 - 1;
+        float q[3];
+        q[0] = s[0] + (sx > 0 ? a.grad_off[0] : (sx < 0 ? -a.grad_off[0] : 0.0f));
+        q[1] = s[1] + (sy > 0 ? a.grad_off[1] : (sy < 0 ? -a.grad_off[1] : 0.0f));
+        q[2] = s[2] + (sz > 0 ? a.grad_off[2] : (sz < 0 ? -a.grad_off[2] : 0.0f));
+        V[n] = reconstruct<T, FILTER>(f, a, q);
+    }
+    const float gx = V[1] + 2.0f * V[2] + V[3] - V[19] - 2.0f * V[20] - V[21]
+                   + 2.0f * V[4] + 4.0f * V[5] + 2.0f * V[6] - 2.0f * V[22] - 4.0f * V[23] - 2.0f * V[24]
+                   + V[7] + 2.0f * V[8] + V[9] - V[25] - 2.0f * V[26] - V[27];
+    const float gy = -V[1] + V[3] - 2.0f * V[10] + 2.0f * V[12] - V[19] + V[21]
+                   - 2.0f * V[4] + 2.0f * V[6] - 4.0f * V[13] + 4.0f * V[15] - 2.0f * V[22] + 2.0f * V[24]
+                   - V[7] + V[9] - 2.0f * V[16] + 2.0f * V[18] - V[25] + V[27];
+    const float gz = V[1] + 2.0f * V[2] + V[3] + 2.0f * V[10] + 4.0f * V[11] + 2.0f * V[12] + V[19]
+                   + 2.0f * V[20] + V[21] - V[7] - 2.0f * V[8] - V[9] - 2.0f * V[16] - 4.0f * V[17]
+                   - 2.0f * V[18] - V[25] - 2.0f * V[26] - V[27];
+    const float g[3] = {-gx, -gy, -gz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gr[i] = ((g[i] * a.grad_scale[i]) / 16.0f) * a.rwm_scale;
+}
+
+// calcGradientFiltered, mod_gradients.frag:298-314: central gradients at the
+// sample and its eight diagonal neighbours blended by GLSL mix
+// (x*(1-a) + y*a).
+template <typename T, int FILTER>
+__device__ __noinline__ void grad_filtered(const LinearFetch<T>& f, const MarchArgs& a, const float s[3],
+                                           float gr[3]) {
+    float G[9][3];
+    grad_central<T, FILTER>(f, a, s, G[0]);
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+        // diagonals in the shader's order: ---, +++, -+-, +-+, --+, ++-, -++, +--
+        const uint32_t neg = 0xC63547u >> (3 * k);   // bit i set: axis i steps by -1/dim
+        float q[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q[i] = s[i] + (((neg >> i) & 1u) ? -a.grad_off[i] : a.grad_off[i]);
+        float g[3];
+        grad_central<T, FILTER>(f, a, q, g);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) G[k + 1][i] = g[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float m12 = G[1][i] * 0.5f + G[2][i] * 0.5f, m34 = G[3][i] * 0.5f + G[4][i] * 0.5f;
+        const float m56 = G[5][i] * 0.5f + G[6][i] * 0.5f, m78 = G[7][i] * 0.5f + G[8][i] * 0.5f;
+        const float mix0 = m12 * 0.5f + m34 * 0.5f, mix1 = m56 * 0.5f + m78 * 0.5f;
+        gr[i] = G[0][i] * 0.25f + (mix0 * 0.5f + mix1 * 0.5f) * 0.75f;
+    }
+}
+
+// CALC_GRADIENT as the "gradient" property configures it (volumeraycaster.cpp:197-207).
+template <typename T, int FILTER>
+__device__ __forceinline__ void sample_gradient(const LinearFetch<T>& f, const MarchArgs& a, const float s[3],
+                                                float inten, float gr[3]) {
+    if (a.gradient == kGradCentral) {
+        grad_central<T, FILTER>(f, a, s, gr);
+    } else if (a.gradient == kGradForward) {
+        // calcGradientAFD, mod_gradients.frag:116-126: (v(s) - v(s + 1/dim)) * spacingRCP * rwmScale
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            float q[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) q[i] = s[i] + (i == axis ? a.grad_off[i] : 0.0f);
+            gr[axis] = ((inten - reconstruct<T, FILTER>(f, a, q)) * a.spacing_rcp[axis]) * a.rwm_scale;
+        }
+    } else if (a.gradient == kGradNone) {
+        // (voxel.xyz - 0.5)*2 on the sample of a one-channel texture swizzled to
+        // (R, 0, 0, R) (volumegl.cpp:367), real-world mapping on every component
+        // (mod_sampler3d.frag:90-101)
+        const float yz = (0.0f * a.rwm_scale) + a.rwm_offset;
+        gr[0] = (((inten * a.rwm_scale) + a.rwm_offset) - 0.5f) * 2.0f;
+        gr[1] = (yz - 0.5f) * 2.0f;
+        gr[2] = (yz - 0.5f) * 2.0f;
+    } else if (a.gradient == kGradFiltered) {
+        grad_filtered<T, FILTER>(f, a, s, gr);
+    } else {
+        grad_sobel<T, FILTER>(f, a, s, gr);
+    }
+}
+
+// Channel c of gradient-volume voxel `vox`, normalized (getVoxelNormalized(pos,
+// channel), volumeatomic.h:531-536; float channels as stored).
+__device__ __forceinline__ float gvol_norm(const MarchArgs& a, uint64_t vox, int c) {
+    const uint64_t i = vox * (uint32_t)a.gvol_channels + (uint32_t)c;
+    if (a.gvol_dtype == 0) return VoxNorm<uint8_t>::apply(((const uint8_t*)a.gvol)[i]);
+    if (a.gvol_dtype == 1) return VoxNorm<uint16_t>::apply(((const uint16_t*)a.gvol)[i]);
+    return ((const float*)a.gvol)[i];
+}
+
+// The gradient-volume sample of the 2D TF path (cpuraycaster.cpp:270-284):
+// channels 0..2 at the rounded voxel position, or trilinear per channel.
+template <int FILTER>
+__device__ __forceinline__ void gvol_sample(const MarchArgs& a, const float vp[3], float out[3]) {
+    const uint32_t X = a.dims[0];
+    const uint64_t XY = (uint64_t)a.dims[0] * a.dims[1];
+    if (FILTER == 0) {
+        const uint64_t v = (uint64_t)(uint32_t)iround_tgt(vp[2]) * XY + (uint64_t)(uint32_t)iround_tgt(vp[1]) * X +
+                           (uint32_t)iround_tgt(vp[0]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = gvol_norm(a, v, c);
+        return;
+    }
+    Corners cr;
+    corners(vp, a.dims, cr);
+    const uint64_t base = (uint64_t)cr.lo[2] * XY + (uint64_t)cr.lo[1] * X + cr.lo[0];
+    const uint64_t dx = cr.d[0], dy = (uint64_t)cr.d[1] * X, dz = (uint64_t)cr.d[2] * XY;
+    const uint64_t at[8] = {base, base + dx, base + dx + dy, base + dy,
+                            base + dz, base + dx + dz, base + dx + dy + dz, base + dy + dz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = gvol_norm(a, at[k], c);
+        out[c] = trilerp(v, cr.p);
+    }
+}
+
 template <typename T, int FILTER, int CLASSIF, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
     __shared__ uint32_t s_tf[kMaxTfWidth];
@@ -107,6 +255,27 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
                     const float4 c = a.preint[(size_t)preint_index(x, n) * n + preint_index(last_x, n)];
                     cr = c.x; cg = c.y; cb = c.z; ca = c.w;
                     last_x = x;
+                } else if (CLASSIF == 2) {
+                    // cpuraycaster.cpp:270-290 with apply2DTF :334-338; the table
+                    // holds the opacity-corrected alpha
+                    float vp[3], gv[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) vp[i] = tex_to_voxel(s[i], a.fdims[i], a.fdm1[i]);
+                    gvol_sample<FILTER>(a, vp, gv);
+                    float lsq = 0.0f;
+                    lsq = lsq + gv[0] * gv[0];
+                    lsq = lsq + gv[1] * gv[1];
+                    lsq = lsq + gv[2] * gv[2];
+                    const float rw = (inten * a.rwm_scale) + a.rwm_offset;
+                    const float xi = rw_to_norm(rw, a.tf2d_dom[0], a.tf2d_dom[1]);
+                    const float xg = rw_to_norm(sqrtf(lsq), a.tf2d_dom[2], a.tf2d_dom[3]);
+                    // truncation as size_t(x*(W-1)); x is in [0, 1] (a NaN
+                    // magnitude from float gradient data selects texel 0)
+                    const float fx = xi * (float)(a.tf2d_w - 1), fy = xg * (float)(a.tf2d_h - 1);
+                    const uint32_t ix = fx > 0.0f ? min((uint32_t)fx, (uint32_t)a.tf2d_w - 1u) : 0u;
+                    const uint32_t iy = fy > 0.0f ? min((uint32_t)fy, (uint32_t)a.tf2d_h - 1u) : 0u;
+                    const float4 c = a.tf2d[(size_t)iy * (uint32_t)a.tf2d_w + ix];
+                    cr = c.x; cg = c.y; cb = c.z; ca = c.w;
                 } else {
                     const uint32_t c8 = s_tf[tf_index(inten, a)];
                     ca = s_alpha[c8 >> 24];
@@ -118,24 +287,8 @@ __global__ __launch_bounds__(256) void march_ref_kernel(const MarchArgs a) {
                     shade_now = PHONG;
                     if (PHONG) {
                         float gr[3];
-                        if (t == 0.0f) {
-                            clip_border_gradient(a, px, py, s, dir, gr);
-                        } else {
-                            float vals[6];
-#pragma unroll
-                            for (int k = 0; k < 6; ++k) {
-                                const int axis = k % 3;
-                                const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
-                                float q[3] = {s[0], s[1], s[2]};
-                                q[axis] = s[axis] + off;
-                                q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
-                                q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
-                                vals[k] = reconstruct<T, FILTER>(f, a, q);
-                            }
-#pragma unroll
-                            for (int i = 0; i < 3; ++i)
-                                gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
-                        }
+                        if (t == 0.0f) clip_border_gradient(a, px, py, s, dir, gr);
+                        else sample_gradient<T, FILTER>(f, a, s, inten, gr);
                         float pos[3], c[3] = {cr, cg, cb};
 #pragma unroll
                         for (int i = 0; i < 3; ++i)
@@ -234,23 +387,8 @@ __global__ __launch_bounds__(256) void march_modes_kernel(const MarchArgs a) {
             if (col[3] > 0.0f) {
                 float gr[3] = {0.f, 0.f, 0.f};
                 if (PHONG || mode == kCompFHN) {
-                    if (t == 0.0f) {
-                        clip_border_gradient(a, px, py, s, dir, gr);
-                    } else {
-                        float vals[6];
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            const int axis = k % 3;
-                            const float off = (k < 3) ? a.grad_off[axis] : -a.grad_off[axis];
-                            float q[3] = {s[0], s[1], s[2]};
-                            q[axis] = s[axis] + off;
-                            q[(axis + 1) % 3] = s[(axis + 1) % 3] + 0.0f;
-                            q[(axis + 2) % 3] = s[(axis + 2) % 3] + 0.0f;
-                            vals[k] = reconstruct<T, FILTER>(f, a, q);
-                        }
-#pragma unroll
-                        for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
-                    }
+                    if (t == 0.0f) clip_border_gradient(a, px, py, s, dir, gr);
+                    else sample_gradient<T, FILTER>(f, a, s, inten, gr);
                 }
                 if (PHONG) {
                     float pos[3];
@@ -354,6 +492,14 @@ static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
 template <typename T>
 static void launch_dtype(const MarchArgs& a, int filter, int classif, bool phong, bool count,
                          hipStream_t st) {
+    if (classif == 2) {
+        if (filter == 0) {
+            if (phong) launch_t<T, 0, 2, true>(a, count, st); else launch_t<T, 0, 2, false>(a, count, st);
+        } else {
+            if (phong) launch_t<T, 1, 2, true>(a, count, st); else launch_t<T, 1, 2, false>(a, count, st);
+        }
+        return;
+    }
     if (filter == 0) {
         if (classif == 0) {
             if (phong) launch_t<T, 0, 0, true>(a, count, st); else launch_t<T, 0, 0, false>(a, count, st);

@@ -64,6 +64,9 @@ struct hipdvr_volume {
     void* d_cells = nullptr;         // cell records (null until first use)
     uint64_t cell_bytes = 0;
     bool cells_wide = false;         // more than 2^32 - 256 bytes: 64-bit addressing
+    void* d_grad = nullptr;          // gradient volume of the 2D TF path (same dims)
+    int32_t grad_dtype = 0;          // HIPDVR_U8 / HIPDVR_U16 / HIPDVR_F32
+    uint32_t grad_channels = 0;      // 3 or 4 interleaved channels per voxel
 };
 
 namespace hipdvr {

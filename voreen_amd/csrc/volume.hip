@@ -322,6 +322,7 @@ void volume_destroy(hipdvr_volume* v) {
     if (v->d_minmax) (void)hipFree(v->d_minmax);
     if (v->d_occ) (void)hipFree(v->d_occ);
     if (v->d_occ_bits) (void)hipFree(v->d_occ_bits);
+    if (v->d_grad) (void)hipFree(v->d_grad);
     delete v;
 }
 

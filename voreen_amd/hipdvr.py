@@ -12,9 +12,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhipdvr.so")
 
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
-U8, U16 = 0, 1
+U8, U16, F32 = 0, 1, 2
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
-CLASS_TF, CLASS_PREINT = 0, 1
+CLASS_TF, CLASS_PREINT, CLASS_TF2D = 0, 1, 2
+GRADIENT_CENTRAL, GRADIENT_FORWARD, GRADIENT_NONE, GRADIENT_FILTERED, GRADIENT_SOBEL = range(5)
+# VolumeRaycaster "gradient" option keys (volumeraycaster.cpp:126-131) by GRADIENT_* value
+GRADIENT_KEYS = ("central-differences", "forward-differences", "none", "filtered", "sobel")
 COMPOSITE_DVR, COMPOSITE_MIP, COMPOSITE_MOP, COMPOSITE_MIDA, COMPOSITE_ISO, COMPOSITE_FHP, COMPOSITE_FHN = range(7)
 (SHADE_NONE, SHADE_PHONG, SHADE_PHONG_DIFFUSE, SHADE_PHONG_SPECULAR, SHADE_PHONG_DIFFUSE_AMBIENT,
  SHADE_PHONG_DIFFUSE_SPECULAR, SHADE_TOON, SHADE_COOK_TORRANCE, SHADE_OREN_NAYAR, SHADE_WARD) = range(10)
@@ -28,8 +31,9 @@ EXPORTS = (
     "hipdvr_create", "hipdvr_destroy", "hipdvr_last_error", "hipdvr_abi_version",
     "hipdvr_set_stream", "hipdvr_synchronize",
     "hipdvr_volume_create", "hipdvr_volume_create_device", "hipdvr_volume_destroy",
-    "hipdvr_volume_device_ptr",
+    "hipdvr_volume_device_ptr", "hipdvr_volume_set_gradient",
     "hipdvr_tf_from_keys", "hipdvr_tf_create", "hipdvr_tf_destroy", "hipdvr_tf_set_preint",
+    "hipdvr_tf2d_create",
     "hipdvr_preint_build",
     "hipdvr_render", "hipdvr_render_device", "hipdvr_entry_exit_device",
     "hipdvr_buffer_alloc", "hipdvr_buffer_free", "hipdvr_memcpy_h2d", "hipdvr_memcpy_d2h",
@@ -69,7 +73,8 @@ class Params(C.Structure):
                 ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
                 ("shininess", C.c_float), ("camera_pos", C.c_float * 3),
                 ("loop_cap", C.c_int32), ("kernel", C.c_int32), ("batch", C.c_int32),
-                ("gamma", C.c_float), ("iso_value", C.c_float), ("exact_shading", C.c_int32)]
+                ("gamma", C.c_float), ("iso_value", C.c_float), ("exact_shading", C.c_int32),
+                ("gradient", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -142,6 +147,8 @@ def load():
         "hipdvr_tf_from_keys": ([P(TFKey), i32, i32, C.c_float, C.c_float, C.c_float, i32, vp], C.c_int),
         "hipdvr_tf_create": ([vp, vp, u32, C.c_float, C.c_float, P(vp)], C.c_int),
         "hipdvr_tf_destroy": ([vp], C.c_int),
+        "hipdvr_tf2d_create": ([vp, vp, u32, u32, vp, P(vp)], C.c_int),
+        "hipdvr_volume_set_gradient": ([vp, vp, i32, u32], C.c_int),
         "hipdvr_tf_set_preint": ([vp, vp, u32], C.c_int),
         "hipdvr_preint_build": ([P(TFKey), i32, C.c_float, C.c_float, C.c_float, i32, u32,
                                  C.c_float, i32, vp], C.c_int),
@@ -255,7 +262,8 @@ def synth_shells(seed, dims):
 
 def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading=SHADE_NONE,
                    kernel=KERNEL_AUTO, light=None, camera_pos=(0.0, 0.0, 3.5), batch=0,
-                   compositing=COMPOSITE_DVR, gamma=0.0, iso_value=0.5, exact_shading=0):
+                   compositing=COMPOSITE_DVR, gamma=0.0, iso_value=0.5, exact_shading=0,
+                   gradient=GRADIENT_CENTRAL):
     """Render parameters with Voreen's VolumeRenderer light defaults
     (src/core/processors/volumerenderer.cpp:41-54)."""
     p = Params()
@@ -266,6 +274,7 @@ def default_params(step, filter_=FILTER_LINEAR, classification=CLASS_TF, shading
     p.gamma = gamma
     p.iso_value = iso_value
     p.exact_shading = exact_shading
+    p.gradient = gradient
     p.shading = shading
     lt = dict(pos=(2.3, 1.5, 1.5), ambient=(0.4,) * 3, diffuse=(0.8,) * 3, specular=(0.6,) * 3,
               shininess=60.0)
@@ -330,6 +339,19 @@ class Context:
 
     def transfunc(self, rgba8, domain=(0.0, 1.0)):
         return TransFunc(self, rgba8, domain)
+
+    def transfunc2d(self, rgba, domain=((0.0, 1.0), (0.0, 1.0))):
+        """2D TF (hipdvr_tf2d_create) from a (H, W, 4) float32 texture and the
+        (intensity, gradient-magnitude) domains."""
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        if rgba.ndim != 3 or rgba.shape[2] != 4:
+            raise ValueError("2D TF texture must be (H, W, 4)")
+        t = TransFunc.__new__(TransFunc)
+        t.ctx, t.width, t.h = self, rgba.shape[1], C.c_void_p()
+        dom = (C.c_float * 4)(domain[0][0], domain[0][1], domain[1][0], domain[1][1])
+        self.check(self.lib.hipdvr_tf2d_create(self.h, _ptr(rgba), rgba.shape[1], rgba.shape[0], dom,
+                                               C.byref(t.h)), "hipdvr_tf2d_create")
+        return t
 
     def load_vvd(self, path, brick_log2=4):
         h, d = C.c_void_p(), VolumeDesc()
@@ -449,6 +471,18 @@ class Volume:
         v = cls.__new__(cls)
         v.ctx, v.h, v.desc = ctx, h, desc
         return v
+
+    def set_gradient(self, grad):
+        """Attach the gradient volume of the 2D TF path: (Z, Y, X, 3 or 4)
+        uint8 / uint16 / float32, the volume's dims."""
+        grad = np.ascontiguousarray(grad)
+        dt = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float32): F32}.get(grad.dtype)
+        if dt is None or grad.ndim != 4:
+            raise ValueError("gradient volume must be a (Z, Y, X, C) uint8, uint16 or float32 array")
+        if tuple(grad.shape[:3]) != tuple(self.desc.dims[::-1]):
+            raise ValueError("gradient volume dimensions differ from intensity volume dimensions")
+        self.ctx.check(self.ctx.lib.hipdvr_volume_set_gradient(self.h, _ptr(grad), dt, grad.shape[3]),
+                       "hipdvr_volume_set_gradient")
 
     def minmax(self):
         """(minNorm, maxNorm, min, max) of the whole volume (VolumeMinMax)."""
