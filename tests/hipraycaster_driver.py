@@ -26,6 +26,8 @@ def lib():
         l.hrc_set_property.argtypes = [vp, C.c_char_p, C.c_char_p]
         l.hrc_set_volume.argtypes = [vp, vp, vp, C.c_int, C.c_float, C.c_float, vp, vp]
         l.hrc_set_tf.argtypes = [vp, vp, C.c_int, C.c_float, C.c_float]
+        l.hrc_set_gradient_volume.argtypes = [vp, vp, vp, C.c_int, C.c_int]
+        l.hrc_set_tf2d.argtypes = [vp, vp, C.c_int, C.c_int, vp]
         l.hrc_set_images.argtypes = [vp, vp, vp, C.c_int, C.c_int]
         l.hrc_connect.argtypes = [vp, C.c_int, C.c_int]
         l.hrc_is_ready.argtypes = [vp]
@@ -76,6 +78,21 @@ class Raycaster:
         arr, n = H.make_keys(keys)
         self._keep.append(arr)
         lib().hrc_set_tf(self.h, C.cast(arr, C.c_void_p), n, domain[0], domain[1])
+
+    def set_gradient_volume(self, grad):
+        """(Z, Y, X, 3 or 4) uint8 / uint16 / float32 on volumehandle.gradientvolumehandle."""
+        grad = np.ascontiguousarray(grad)
+        self._keep.append(grad)
+        nz, ny, nx, ch = grad.shape
+        dims = np.array([nx, ny, nz], np.uint32)
+        self._keep.append(dims)
+        dt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[grad.dtype]
+        return lib().hrc_set_gradient_volume(self.h, _p(grad), _p(dims), dt, ch)
+
+    def set_tf2d(self, rgba, domain=((0.0, 1.0), (0.0, 1.0))):
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        dom = np.array([domain[0][0], domain[0][1], domain[1][0], domain[1][1]], np.float32)
+        return lib().hrc_set_tf2d(self.h, _p(rgba), rgba.shape[1], rgba.shape[0], _p(dom))
 
     def set_images(self, entry, exit_):
         entry = np.ascontiguousarray(entry, np.float32)

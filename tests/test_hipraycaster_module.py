@@ -50,7 +50,11 @@ def test_ports_and_properties_match_the_reference(rc):
     assert props["shading"]["value"] == "none"
     assert "phong" in props["shading"]["options"]
     assert props["preIntegrationTableSize"]["options"] == ["deriveFromBitDepth", "256", "512", "1024"]
-    assert props["tFuncType"]["options"] == ["1DKeys", "2DPrimitives"]
+    # TransFuncTypeProperty option keys (transfunctypeproperty.cpp:113-128, spelling included)
+    assert props["tFuncType"]["options"] == ["1D Keys Color Map", "2D Primitves Color Map"]
+    assert props["gradient"]["options"] == ["none", "forward-differences", "central-differences", "sobel",
+                                            "filtered"]
+    assert props["gradient"]["value"] == "central-differences"
     assert props["lightPosition"]["value"] == "2.3 1.5 1.5 1"
     assert props["materialShininess"]["value"] == "60"
     hip = [k for k in props if k.startswith("hip")]
@@ -198,3 +202,61 @@ def test_process_compositing_modes(rc, oracle, mode, code):
     ref, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, rc.step(), compositing=code, gamma=0.25,
                            iso_value=0.3)
     assert np.abs(out - ref).max() <= 1e-3
+
+
+def test_tf2d_checks_of_the_reference(rc):
+    """The 2D TF checks of cpuraycaster.cpp:134-146 log an error and clear the
+    output before any device work."""
+    rc.set_volume(np.full((8, 8, 8), 100, np.uint8))
+    en = np.full((4, 4, 4), 0.3, np.float32)
+    rc.set_images(en, en)
+    rc.set_tf2d(np.ones((4, 4, 4), np.float32))
+    assert rc.set("tFuncType", "2D Primitves Color Map") == 0
+    out, nerr = rc.process()
+    assert nerr == 1 and "gradient volume is needed" in rc.last_error() and np.all(out == 0)
+    rc.set_gradient_volume(np.zeros((8, 8, 8, 1), np.float32))
+    out, nerr = rc.process()
+    assert nerr == 1 and "gradient volume is needed" in rc.last_error()
+    rc.set_gradient_volume(np.zeros((8, 8, 7, 3), np.float32))
+    out, nerr = rc.process()
+    assert nerr == 1 and "dimensions differ" in rc.last_error()
+    rc.set_gradient_volume(np.zeros((8, 8, 8, 4), np.uint8))
+    rc.set("classification", "pre-integrated-cpu")
+    out, nerr = rc.process()
+    assert nerr == 1 and "Pre-integration cannot be used with 2D tfs" in rc.last_error()
+    rc.set("classification", "transfer-function")
+    rc.set("compositing", "mip")
+    out, nerr = rc.process()
+    assert nerr == 1 and "compositing" in rc.last_error() and np.all(out == 0)
+
+
+@pytest.mark.gpu
+def test_process_tf2d_and_gradient_modes(rc, oracle):
+    import test_gpu_parity as G
+    vol, grad, tf2d, dom, en, ex, step, sp, off, eye = G.tf2d_inputs(oracle, np.uint16, 3)
+    print("[case] HIPRaycaster.process 2D TF", flush=True)
+    rc.set_volume(vol, spacing=sp, offset=off)
+    rc.set_gradient_volume(grad)
+    rc.set_tf2d(tf2d, dom)
+    rc.set_images(en, ex)
+    rc.set("tFuncType", "2D Primitves Color Map")
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    ref, n = oracle.render(vol, None, (0.0, 1.0), en, ex, rc.step(), classification=oracle.CLASS_TF2D,
+                           grad_volume=grad, tf2d=tf2d, tf2d_domain=dom, spacing=sp, offset=off)
+    assert np.abs(out - ref).max() <= 1e-3 and np.mean(out[..., 3] > 0) > 0.05
+    assert rc.stats()["samples"] == n
+    print("[case] HIPRaycaster.process 1D TF, sobel gradient, phong", flush=True)
+    from voreen_amd import workloads as W
+    keys, _ = W.make_tf("c1")
+    rc.set("tFuncType", "1D Keys Color Map")
+    rc.set_tf(keys)
+    rc.set("shading", "phong")
+    rc.set("gradient", "sobel")
+    rc.set("camera", " ".join(str(v) for v in eye))
+    out, nerr = rc.process()
+    assert nerr == 0, rc.last_error()
+    tf = oracle.tf_build(keys, 4096)
+    ref, _ = oracle.render(vol, tf, (0.0, 1.0), en, ex, rc.step(), shading=1, gradient=oracle.GRAD_SOBEL,
+                           light=dict(camera=eye), spacing=sp, offset=off)
+    assert np.mean(np.abs(out - ref).max(axis=2) > 1e-3) < 0.002 and np.mean(out[..., 3] > 0) > 0.05

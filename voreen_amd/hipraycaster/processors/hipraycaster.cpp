@@ -27,7 +27,11 @@ HIPRaycaster::HIPRaycaster()
     , ctx_(0)
     , dvrVolume_(0)
     , dvrTF_(0)
+    , dvrTF2D_(0)
     , uploadedVolume_(0)
+    , volumeGeneration_(0)
+    , uploadedGradient_(0)
+    , gradientGeneration_(0)
     , uploadedDomain_(0.f, 0.f)
     , uploadedBrickLog2_(0)
     , uploadedPreintStep_(-1.f)
@@ -111,9 +115,13 @@ void HIPRaycaster::deinitialize() {
 
 void HIPRaycaster::releaseDevice() {
     if (dvrTF_) hipdvr_tf_destroy(dvrTF_);
+    if (dvrTF2D_) hipdvr_tf_destroy(dvrTF2D_);
     if (dvrVolume_) hipdvr_volume_destroy(dvrVolume_);
     if (ctx_) hipdvr_destroy(ctx_);
     dvrTF_ = 0;
+    dvrTF2D_ = 0;
+    uploadedGradient_ = 0;
+    uploadedTF2D_.clear();
     dvrVolume_ = 0;
     ctx_ = 0;
     uploadedVolume_ = 0;
@@ -173,6 +181,61 @@ bool HIPRaycaster::ensureVolume(const VolumeBase* vb, const VolumeRAM* vram) {
     }
     uploadedVolume_ = vb;
     uploadedBrickLog2_ = brickLog2;
+    ++volumeGeneration_;
+    return true;
+}
+
+// Attach the gradient volume of the 2D TF path to the uploaded volume
+// (checks of cpuraycaster.cpp:134-141 are made by the caller).
+bool HIPRaycaster::ensureGradientVolume(const VolumeBase* gvb, const VolumeRAM* gram) {
+    if (uploadedGradient_ == gvb && gradientGeneration_ == volumeGeneration_ && !gradientVolumePort_.hasChanged())
+        return true;
+    const std::string type = gram->getBaseType();
+    const int32_t dtype = type == "uint8" ? HIPDVR_U8 : type == "uint16" ? HIPDVR_U16 : type == "float" ? HIPDVR_F32 : -1;
+    if (dtype < 0) return fail("HIPRaycaster supports uint8, uint16 and float gradient volumes, got " + type);
+    const int rc = hipdvr_volume_set_gradient(dvrVolume_, gram->getData(), dtype,
+                                              static_cast<uint32_t>(gram->getNumChannels()));
+    if (rc != HIPDVR_OK) {
+        uploadedGradient_ = 0;
+        return fail(std::string("gradient volume upload failed: ") + hipdvr_last_error(ctx_));
+    }
+    uploadedGradient_ = gvb;
+    gradientGeneration_ = volumeGeneration_;
+    return true;
+}
+
+// Upload the texture and domains of tFunc2DPrimitives_ when they changed.
+// CPURaycaster reads its texels as float RGBA (cpuraycaster.cpp:334-338); a
+// texture held as bytes is converted by /255.
+bool HIPRaycaster::ensureTransFunc2D(const TransFunc2DPrimitives* tf) {
+    const tgt::Texture* tex = tf ? tf->getTexture() : 0;
+    if (!tex || !tex->getCpuTextureData() || tex->getWidth() < 2 || tex->getHeight() < 2)
+        return fail("2D transfer function has no texture data");
+    const size_t n = 4 * static_cast<size_t>(tex->getWidth()) * tex->getHeight();
+    std::vector<float> texels(n);
+    if (tex->getDataType() == GL_FLOAT) {
+        std::memcpy(texels.data(), tex->getCpuTextureData(), n * sizeof(float));
+    } else {
+        const GLubyte* b = tex->getCpuTextureData();
+        for (size_t i = 0; i < n; ++i) texels[i] = static_cast<float>(b[i]) / 255.f;
+    }
+    const tgt::vec2 d0 = tf->getDomain(0), d1 = tf->getDomain(1);
+    const tgt::vec4 dom(d0.x, d0.y, d1.x, d1.y);
+    if (dvrTF2D_ && texels == uploadedTF2D_ && uploadedDomain2D_ == dom) return true;
+    if (dvrTF2D_) {
+        hipdvr_tf_destroy(dvrTF2D_);
+        dvrTF2D_ = 0;
+    }
+    const float domain[4] = {dom.x, dom.y, dom.z, dom.w};
+    const int rc = hipdvr_tf2d_create(ctx_, texels.data(), static_cast<uint32_t>(tex->getWidth()),
+                                      static_cast<uint32_t>(tex->getHeight()), domain, &dvrTF2D_);
+    if (rc != HIPDVR_OK) {
+        dvrTF2D_ = 0;
+        uploadedTF2D_.clear();
+        return fail(std::string("2D transfer function upload failed: ") + hipdvr_last_error(ctx_));
+    }
+    uploadedTF2D_.swap(texels);
+    uploadedDomain2D_ = dom;
     return true;
 }
 
@@ -249,6 +312,13 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
                                 : HIPDVR_KERNEL_AUTO;
     p.batch = 0;
     p.exact_shading = 0;
+    // gradient keys of volumeraycaster.cpp:126-131
+    const std::string g = gradientMode_.getKey();
+    p.gradient = g == "forward-differences" ? HIPDVR_GRADIENT_FORWARD
+               : g == "none"                ? HIPDVR_GRADIENT_NONE
+               : g == "filtered"            ? HIPDVR_GRADIENT_FILTERED
+               : g == "sobel"               ? HIPDVR_GRADIENT_SOBEL
+                                            : HIPDVR_GRADIENT_CENTRAL;
 }
 
 // Output exactly as CPURaycaster::process (cpuraycaster.cpp:157-159,197-207).
@@ -277,20 +347,39 @@ void HIPRaycaster::process() {
     tFunc1DKeys_.setVolume(vb);
     tFunc2DPrimitives_.setVolume(vb);
 
-    if (tFuncType_.getValue() != TFT_1DKEYS) {
-        fail("HIPRaycaster supports 1D key transfer functions only");
+    const bool tf2d = tFuncType_.getValue() == TFT_2DPRIMITIVES;
+    if (tFuncType_.getValue() != TFT_1DKEYS && !tf2d) {
+        fail("HIPRaycaster supports 1D key and 2D primitive transfer functions");
         writeOutput(std::vector<float>(), size.x, size.y);
         return;
     }
     const std::string comp = compositingMode1_.getKey();
-    if (comp != "dvr" && classificationMode_.getKey() != "transfer-function") {
-        fail("compositing mode '" + comp + "' needs transfer-function classification");
+    const std::string cls = classificationMode_.getKey();
+    if (comp != "dvr" && (cls != "transfer-function" || tf2d)) {
+        fail("compositing mode '" + comp + "' needs 1D transfer-function classification");
         writeOutput(std::vector<float>(), size.x, size.y);
         return;
     }
-    const std::string shade = shadeMode_.getKey();
-    if (shade != "none" && gradientMode_.getKey() != "central-differences")
-        LWARNING("HIPRaycaster computes gradients by central differences");
+    const VolumeBase* gvb = gradientVolumePort_.getData();
+    const VolumeRAM* gram = gvb ? gvb->getRepresentation<VolumeRAM>() : 0;
+    if (tf2d) {
+        // the checks of cpuraycaster.cpp:134-146
+        if (!gram || gram->getNumChannels() < 3) {
+            fail("To use 2D tfs a RGB or RGBA gradient volume is needed");
+            writeOutput(std::vector<float>(), size.x, size.y);
+            return;
+        }
+        if (gram->getDimensions() != vram->getDimensions()) {
+            fail("Gradient volume dimensions differ from intensity volume dimensions");
+            writeOutput(std::vector<float>(), size.x, size.y);
+            return;
+        }
+        if (cls.compare(0, 14, "pre-integrated") == 0) {
+            fail("Pre-integration cannot be used with 2D tfs.");
+            writeOutput(std::vector<float>(), size.x, size.y);
+            return;
+        }
+    }
 
     if (!ensureContext() || !ensureVolume(vb, vram)) {
         writeOutput(std::vector<float>(), size.x, size.y);
@@ -300,18 +389,30 @@ void HIPRaycaster::process() {
     // sampling step, volumeraycaster.cpp:312-322 (CPURaycaster: cpuraycaster.cpp:104)
     const float step = getSamplingStepSize(vb);
 
-    tgt::Texture* tfTexture = tFunc1DKeys_.get()->getTexture();
-    tfTexture->downloadTexture();
-    if (!ensureTransFunc(tfTexture, tFunc1DKeys_.get()->getDomain())) {
-        writeOutput(std::vector<float>(), size.x, size.y);
-        return;
-    }
-
     hipdvr_params params;
     fillParams(params, vb, step);
+    hipdvr_tf* renderTF = 0;
 
-    const std::string cls = classificationMode_.getKey();
-    if (cls == "pre-integrated-cpu" || cls == "pre-integrated-approximation") {
+    if (tf2d) {
+        if (!ensureGradientVolume(gvb, gram) || !ensureTransFunc2D(tFunc2DPrimitives_.get())) {
+            writeOutput(std::vector<float>(), size.x, size.y);
+            return;
+        }
+        params.classification = HIPDVR_CLASS_TF2D;
+        renderTF = dvrTF2D_;
+    } else {
+        tgt::Texture* tfTexture = tFunc1DKeys_.get()->getTexture();
+        tfTexture->downloadTexture();
+        if (!ensureTransFunc(tfTexture, tFunc1DKeys_.get()->getDomain())) {
+            writeOutput(std::vector<float>(), size.x, size.y);
+            return;
+        }
+        renderTF = dvrTF_;
+    }
+
+    if (tf2d) {
+        // classification set above
+    } else if (cls == "pre-integrated-cpu" || cls == "pre-integrated-approximation") {
         const bool approx = cls == "pre-integrated-approximation";
         const std::string key = cls + ":" + std::to_string(preIntegrationTableSize_.getValue());
         if (uploadedPreintKey_ != key || uploadedPreintStep_ != step) {
@@ -375,7 +476,7 @@ void HIPRaycaster::process() {
         exitPort_.getColorTexture()->downloadTextureToBuffer(GL_RGBA, GL_FLOAT));
 
     std::vector<float> output(4 * static_cast<size_t>(size.x) * size.y, 0.f);
-    const int rc = hipdvr_render(ctx_, dvrVolume_, dvrTF_, reinterpret_cast<const float*>(entryBuffer),
+    const int rc = hipdvr_render(ctx_, dvrVolume_, renderTF, reinterpret_cast<const float*>(entryBuffer),
                                  reinterpret_cast<const float*>(exitBuffer), size.x, size.y, &params,
                                  output.data(), &lastStats_);
     delete[] entryBuffer;

@@ -5,6 +5,7 @@
 #include "voreen/core/properties/transfunc/transfunctypeproperty.h"
 #include "voreen/core/properties/transfunc/1d/1dkeys/transfunc1dkeysproperty.h"
 #include "voreen/core/properties/transfunc/2d/2dprimitives/transfunc2dprimitivesproperty.h"
+#include "voreen/core/datastructures/transfunc/2d/2dprimitives/transfunc2dprimitives.h"
 #include "voreen/core/properties/optionproperty.h"
 #include "voreen/core/properties/intproperty.h"
 #include "voreen/core/properties/floatproperty.h"
@@ -69,6 +70,8 @@ private:
     bool ensureContext();
     bool ensureVolume(const VolumeBase* vb, const VolumeRAM* vram);
     bool ensureTransFunc(tgt::Texture* tfTexture, const tgt::vec2& domain);
+    bool ensureGradientVolume(const VolumeBase* gvb, const VolumeRAM* gram);
+    bool ensureTransFunc2D(const TransFunc2DPrimitives* tf);
     void releaseDevice();
     void fillParams(hipdvr_params& p, const VolumeBase* vb, float step);
     void writeOutput(const std::vector<float>& rgba, int w, int h);
@@ -91,7 +94,13 @@ private:
     hipdvr_ctx* ctx_;
     hipdvr_volume* dvrVolume_;
     hipdvr_tf* dvrTF_;
+    hipdvr_tf* dvrTF2D_;
     const VolumeBase* uploadedVolume_;
+    unsigned long volumeGeneration_;      // counts volume uploads
+    const VolumeBase* uploadedGradient_;
+    unsigned long gradientGeneration_;    // volume upload the gradient volume is attached to
+    std::vector<float> uploadedTF2D_;
+    tgt::vec4 uploadedDomain2D_;
     std::vector<uint8_t> uploadedTF_;
     tgt::vec2 uploadedDomain_;
     int uploadedBrickLog2_;

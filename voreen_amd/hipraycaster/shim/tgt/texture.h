@@ -30,6 +30,7 @@ public:
     Texture(const Texture&) = delete;
     Texture& operator=(const Texture&) = delete;
 
+    GLenum getDataType() const { return dataType_; }
     int getWidth() const { return dims_.x; }
     int getHeight() const { return dims_.y; }
     ivec3 getDimensions() const { return dims_; }

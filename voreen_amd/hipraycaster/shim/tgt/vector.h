@@ -37,6 +37,8 @@ template <typename T> struct Vector4 {
     T& operator[](size_t i) { return i == 0 ? x : (i == 1 ? y : (i == 2 ? z : w)); }
     const T& operator[](size_t i) const { return i == 0 ? x : (i == 1 ? y : (i == 2 ? z : w)); }
     Vector3<T> xyz() const { return Vector3<T>(x, y, z); }
+    bool operator==(const Vector4& o) const { return x == o.x && y == o.y && z == o.z && w == o.w; }
+    bool operator!=(const Vector4& o) const { return !(*this == o); }
 };
 
 typedef Vector2<float> vec2;

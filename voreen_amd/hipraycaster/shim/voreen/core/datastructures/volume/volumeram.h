@@ -21,18 +21,22 @@ private:
 
 class VolumeRAM {
 public:
-    VolumeRAM(const void* data, const tgt::svec3& dims, const std::string& baseType)
-        : data_(data), dims_(dims), type_(baseType) {}
+    // baseType "uint8", "uint16" or "float"; channels interleaved per voxel
+    VolumeRAM(const void* data, const tgt::svec3& dims, const std::string& baseType, size_t channels = 1)
+        : data_(data), dims_(dims), type_(baseType), channels_(channels) {}
     const void* getData() const { return data_; }
     tgt::svec3 getDimensions() const { return dims_; }
     std::string getBaseType() const { return type_; }
-    size_t getNumChannels() const { return 1; }
-    size_t getBytesPerVoxel() const { return type_ == "uint8" ? 1 : 2; }
+    size_t getNumChannels() const { return channels_; }
+    size_t getBytesPerVoxel() const {
+        return (type_ == "uint8" ? 1 : (type_ == "uint16" ? 2 : 4)) * channels_;
+    }
 
 private:
     const void* data_;
     tgt::svec3 dims_;
     std::string type_;
+    size_t channels_;
 };
 
 class VolumeBase {
@@ -46,7 +50,7 @@ public:
     tgt::vec3 getSpacing() const { return spacing_; }
     tgt::vec3 getOffset() const { return offset_; }
     tgt::mat4 getWorldToPhysicalMatrix() const { return tgt::mat4(); }
-    size_t getNumChannels() const { return 1; }
+    size_t getNumChannels() const { return ram_.getNumChannels(); }
     size_t getBytesPerVoxel() const { return ram_.getBytesPerVoxel(); }
 
 private:

@@ -21,10 +21,13 @@ class TransFuncTypeProperty : public OptionProperty<TransFuncType> {
 public:
     TransFuncTypeProperty(const std::string& id, const std::string& gui, int inv = Processor::INVALID_PROGRAM)
         : OptionProperty<TransFuncType>(id, gui, inv) {}
+    // option keys of TransFuncTypeProperty::addTransFuncTypeOption
+    // (src/core/properties/transfunc/transfunctypeproperty.cpp:113-128, spelling included)
     void addTransFuncProperty(TransFuncPropertyBase* p) {
         const TransFuncType t = p->getTFType();
-        addOption(t == TFT_1DKEYS ? "1DKeys" : (t == TFT_2DPRIMITIVES ? "2DPrimitives" : "1DGaussian"),
-                  p->getGuiName(), t);
+        const char* key = t == TFT_1DKEYS ? "1D Keys Color Map"
+                        : (t == TFT_2DPRIMITIVES ? "2D Primitves Color Map" : "1D Gaussian Color Map");
+        addOption(key, key, t);
     }
 };
 

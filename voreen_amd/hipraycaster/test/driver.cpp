@@ -17,6 +17,11 @@ namespace {
 
 struct Exposed : public HIPRaycaster {
     VolumePort& vin() { return volumeInport_; }
+    VolumePort* gin() {
+        for (Port* p : getPorts())
+            if (p->getID() == "volumehandle.gradientvolumehandle") return dynamic_cast<VolumePort*>(p);
+        return 0;
+    }
     RenderPort& ein() { return entryPort_; }
     RenderPort& xin() { return exitPort_; }
     RenderPort& out() { return outport1_; }
@@ -27,6 +32,7 @@ struct Driver {
     std::unique_ptr<HIPRaycasterModule> module;
     std::unique_ptr<Exposed> proc;
     std::unique_ptr<VolumeBase> volume;
+    std::unique_ptr<VolumeBase> gradient;
     bool initialized = false;
 };
 
@@ -114,6 +120,31 @@ int hrc_set_volume(void* h, const void* data, const uint32_t dims[3], int dtype,
     return 0;
 }
 
+// Gradient volume inport (RGB/RGBA; dtype 0 uint8, 1 uint16, 2 float).
+int hrc_set_gradient_volume(void* h, const void* data, const uint32_t dims[3], int dtype, int channels) {
+    Driver* d = (Driver*)h;
+    VolumeRAM ram(data, tgt::svec3(dims[0], dims[1], dims[2]),
+                  dtype == 0 ? "uint8" : (dtype == 1 ? "uint16" : "float"), (size_t)channels);
+    d->gradient.reset(new VolumeBase(ram, RealWorldMapping(), tgt::vec3(1.f, 1.f, 1.f), tgt::vec3(0.f, 0.f, 0.f)));
+    VolumePort* g = d->proc->gin();
+    if (!g) return 1;
+    g->setData(d->gradient.get());
+    g->setConnected(true);
+    return 0;
+}
+
+// The 2D TF editor: painted texture (float RGBA w x hh) and the two domains.
+int hrc_set_tf2d(void* h, const float* rgba, int w, int hh, const float domain[4]) {
+    Driver* d = (Driver*)h;
+    TransFunc2DPrimitivesProperty* p =
+        dynamic_cast<TransFunc2DPrimitivesProperty*>(d->proc->getProperty("tFunc2DPrimitives"));
+    if (!p) return 1;
+    p->get()->setTextureData(rgba, w, hh);
+    p->get()->setDomain(tgt::vec2(domain[0], domain[1]), 0);
+    p->get()->setDomain(tgt::vec2(domain[2], domain[3]), 1);
+    return 0;
+}
+
 int hrc_set_tf(void* h, const hipdvr_tf_key* keys, int n, float dom_lo, float dom_hi) {
     Driver* d = (Driver*)h;
     TransFunc1DKeysProperty* p = dynamic_cast<TransFunc1DKeysProperty*>(d->proc->getProperty("tFunc1DKeys"));
@@ -155,6 +186,7 @@ int hrc_process(void* h, float* out, int w, int hh) {
     const size_t before = ShimLog::errors().size();
     d->proc->runProcess();
     d->proc->vin().setChanged(false);
+    if (d->proc->gin()) d->proc->gin()->setChanged(false);
     const std::vector<float>& img = tgt::Texture::lastUpload();
     const size_t n = (size_t)w * hh * 4;
     if (img.size() == n) std::memcpy(out, img.data(), n * sizeof(float));
