@@ -38,6 +38,20 @@
  *                       closed-form recurrence (delta = voxel at every sample).
  *                       A zero FHN gradient gives (0.5, 0.5, 0.5) where GLSL's
  *                       normalize is undefined: parity unpinned there.
+ *   shading models      N = L = V (and N = H) closed forms: each Phong term
+ *                       alone, the cos(theta) diffuse fall-off, toon's i/3
+ *                       quantization on the unnormalized N.L, Cook-Torrance
+ *                       with F = G = 1 and Beckmann D at NH = 1, Oren-Nayar's A
+ *                       term at alpha = beta = 0, Ward's 1/(4 pi ax ay) peak.
+ *   gradient modes      affine fields give the same down-gradient in every
+ *                       mode; x^2 (forward: -(2x+1)), x*y^2 (Sobel smooths the
+ *                       y^2 across planes: +1/2), x^3 (filtered blend 1/4 :
+ *                       3/4: -(3x^2+3.25)); spacing and rwm scaling; "none"
+ *                       = (rw-0.5)*2 and (offset-0.5)*2.
+ *   2D TF               single-sample rays: the texel at the truncated
+ *                       (intensity, gradient-magnitude) position of affine
+ *                       fields, opacity-corrected; f32/u8/u16 gradient volumes,
+ *                       3 and 4 channels, both filters.
  */
 #include "dvr_oracle.h"
 
