@@ -13,10 +13,14 @@ kernel (algorithmic bytes per launch / HIP-event launch time), and the CPU
 baseline (the FP32 CPURaycaster restatement, no shading, on the host's cores,
 on a bounded band of rows of view 0).
 
-    python bench.py [--steps K] [--warmup W] [--config c1|c0] [--kernel auto|reference|bricked|cell]
+    python bench.py [--steps K] [--warmup W] [--config c1|c0|c2|c4] [--kernel auto|reference|bricked|cell]
                     [--batch 2|4|8]
+
+`--config c2` and `--config c4` are the batched workloads (64 uneven
+volumes; the 8 GiB stress volume with 8 4K views), each printing its own line.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -35,8 +39,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"],
-                    help="c2: 64 uneven volumes at 512^2 in one batched persistent launch")
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"],
+                    help="c2: 64 uneven volumes at 512^2 in one batched persistent launch; "
+                         "c4: 8 GiB turbulence volume, 8 views of 3840x2160 in one batched launch")
     ap.add_argument("--jobs", type=int, default=64, help="c2: number of volumes")
     ap.add_argument("--no-autotune", action="store_true", help="c2: keep the default configuration")
     ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
@@ -216,6 +221,185 @@ def main_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def progress(msg):
+    print(f"[c4] {msg}", file=sys.stderr, flush=True)
+
+
+def psnr_db(a, b):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return 999.0 if mse == 0 else 10 * np.log10(1 / mse)
+
+
+def main_c4(args):
+    """Large-volume stress: 2048x2048x1024 uint16 turbulence (8 GiB, 40% of
+    the voxels empty) made on the device, 8 views of 3840x2160 with Phong.
+    One step = the 8 views in one batched persistent launch (plain path).
+    Beside it: the same views as single frames, learned stepping per view
+    (network trained on plain traces of a 512^3 volume of the same generator:
+    the trace kernel addresses the tiled copy with 32-bit offsets, below the
+    large volume's 8 GiB) with each view's PSNR against the plain one, and
+    every 32nd pixel of an unshaded view 0 against the CPU oracle."""
+    from voreen_amd import hipdvr as H, workloads as W, stepper
+    cfg = W.CONFIGS["c4"]
+    dims = cfg["dims"]
+    wimg, himg = cfg["image"]
+    nviews = args.views or cfg["views"]
+    t_setup = time.perf_counter()
+    ctx = H.Context(0)
+    buf, empty = W.make_volume_device(ctx, "c4")
+    llf, urb = W.world_box(dims)
+    sp = tuple(float((urb[i] - llf[i]) / dims[i]) for i in range(3))
+    off = tuple(float(llf[i] + 0.5 * sp[i]) for i in range(3))
+    v = ctx.volume_device(buf, dims, H.U16, spacing=sp, offset=off, brick_log2=args.brick_log2)
+    buf.free()
+    keys, tf = W.make_tf("c4")
+    t = ctx.transfunc(tf)
+    step = W.step_for(dims, cfg["rate"])
+    cams = [W.orbit_camera(i, nviews) for i in range(nviews)]
+    img_bytes = wimg * himg * 16
+    bufs = []
+    for cam in cams:
+        de, dx, do = ctx.alloc(img_bytes), ctx.alloc(img_bytes), ctx.alloc(img_bytes)
+        ctx.entry_exit_device(cam, llf, urb, wimg, himg, de, dx)
+        bufs.append((de, dx, do))
+    params = [H.default_params(step, shading=cfg["shading"], camera_pos=tuple(cam.eye)) for cam in cams]
+    progress(f"volume made on the device ({empty:.4f} empty), {nviews} views of {wimg}x{himg}: "
+             f"{time.perf_counter() - t_setup:.1f} s")
+
+    # single frames (per-view launches), counted once
+    per_view = [ctx.render_device(v, t, de, dx, wimg, himg, p, do, stats=True)
+                for (de, dx, do), p in zip(bufs, params)]
+    plain_imgs = [do.download((himg, wimg, 4), np.float32) for _, _, do in bufs]
+    samples = sum(s["samples"] for s in per_view)
+    fetched = sum(s["fetched"] for s in per_view)
+    shaded = sum(s["shaded"] for s in per_view)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for (de, dx, do), p in zip(bufs, params):
+        ctx.render_device(v, t, de, dx, wimg, himg, p, do)
+    ctx.synchronize()
+    single_s = time.perf_counter() - t0
+    progress(f"single frames: {nviews / single_s:.3f} views/s")
+
+    # the batched launch: all views, one persistent grid, captured graph
+    outs = [ctx.alloc(img_bytes) for _ in range(nviews)]
+    b = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, outs)])
+    b.set_config(H.TILE_8x8, args.brick_log2, 2)
+    st = b.launch(stats=True)
+    b.capture()
+    setup_s = time.perf_counter() - t_setup
+    for _ in range(args.warmup):
+        b.launch()
+    ctx.synchronize()
+    ctx.timing_begin(args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+        ctx.synchronize()
+        progress("batched launch done")
+    elapsed = time.perf_counter() - t0
+    launch_ms = ctx.timing_read()
+    batched_equal = all(np.array_equal(o.download((himg, wimg, 4), np.float32), ref)
+                        for o, ref in zip(outs, plain_imgs))
+    b.close()
+    bytes_step = st["fetched"] * 16 + st["shaded"] * 48 + 48 * wimg * himg * nviews
+    kernel_s = sum(launch_ms) / 1e3
+    achieved = bytes_step * args.steps / kernel_s / 1e9
+    out = {
+        "metric": "frames/s + Gsamples/s at 1024^2, 512^3 u16; learned vs plain at >=45 dB PSNR",
+        "value": round(nviews * args.steps / elapsed, 4), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"c4: {dims[0]}x{dims[1]}x{dims[2]} u16 turbulence ({empty:.3f} empty, made on "
+                                f"the device), {nviews} views of {wimg}x{himg}, phong, step 1/{int(round(1 / step))}, "
+                                "all views in one batched launch per step"),
+                   "brick": 1 << args.brick_log2, "cell_records_gib": round(dims[0] * dims[1] * dims[2] * 16 / 2**30, 1)},
+        "gsamples_per_s": round(st["samples"] * args.steps / elapsed / 1e9, 4),
+        "samples_per_step": st["samples"], "fetched_per_step": st["fetched"], "shaded_per_step": st["shaded"],
+        "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 3),
+        "batched_equals_single_frames": bool(batched_equal),
+        "single_frames_fps": round(nviews / single_s, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "setup_s": round(setup_s, 2),
+    }
+    assert st["samples"] == samples and st["fetched"] == fetched and st["shaded"] == shaded
+    for o in outs:
+        o.free()
+
+    if not args.no_learned:
+        tdims = (512, 512, 512)
+        tb, _ = W.make_volume_device(ctx, "c4", dims=tdims)
+        tl, tu = W.world_box(tdims)
+        tsp = tuple(float((tu[i] - tl[i]) / tdims[i]) for i in range(3))
+        toff = tuple(float(tl[i] + 0.5 * tsp[i]) for i in range(3))
+        tv = ctx.volume_device(tb, tdims, H.U16, spacing=tsp, offset=toff)
+        tb.free()
+        t0 = time.perf_counter()
+        mlp, info = stepper.train(ctx, tv, tdims, n_tfs=args.learned_tfs, n_views=args.learned_views)
+        train_s = time.perf_counter() - t0
+        tv.close()
+        progress(f"network trained on {tdims}: {train_s:.1f} s")
+        lr = H.Learned(ctx, v)
+        dl = [ctx.alloc(img_bytes) for _ in range(nviews)]
+        try:
+            lr.prepare(t, params[0], mlp)
+            lst = [lr.render_device(t, de, dx, wimg, himg, p, o, stats=True)
+                   for (de, dx, _), p, o in zip(bufs, params, dl)]
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                for (de, dx, _), p, o in zip(bufs, params, dl):
+                    lr.render_device(t, de, dx, wimg, himg, p, o)
+                ctx.synchronize()
+                progress("learned views done")
+            lfps = nviews * args.steps / (time.perf_counter() - t0)
+            psnrs = [psnr_db(o.download((himg, wimg, 4), np.float32), ref) for o, ref in zip(dl, plain_imgs)]
+        finally:
+            for o in dl:
+                o.free()
+            lr.close()
+            mlp.close()
+        out["learned"] = {"fps": round(lfps, 4), "speedup_vs_single_frames": round(lfps * single_s / nviews, 3),
+                          "psnr_min_db": round(min(psnrs), 2), "psnr_mean_db": round(float(np.mean(psnrs)), 2),
+                          "views_fell_back": int(sum(s["fell_back"] for s in lst)),
+                          "reconstructed_per_view": int(sum(s["samples"] for s in lst) / nviews),
+                          "train": {"volume": "512^3 turbulence, same generator", "tfs": info["tfs"],
+                                    "views": info["views"], "rows": info["rows"], "seconds": round(train_s, 1)}}
+
+    if not args.no_cpu:
+        # every 32nd pixel of view 0, unshaded, against the oracle on the downloaded volume
+        de, dx, do = bufs[0]
+        p0 = H.default_params(step, camera_pos=tuple(cams[0].eye))
+        ctx.render_device(v, t, de, dx, wimg, himg, p0, do)
+        gpu = do.download((himg, wimg, 4), np.float32)[::32, ::32]
+        en = de.download((himg, wimg, 4), np.float32)[::32, ::32]
+        ex = dx.download((himg, wimg, 4), np.float32)[::32, ::32]
+        vol = np.empty((dims[2], dims[1], dims[0]), np.uint16)
+        ctx.check(ctx.lib.hipdvr_memcpy_d2h(ctx.h, vol.ctypes.data_as(C.c_void_p), v_linear_ptr(ctx, v),
+                                            vol.nbytes), "hipdvr_memcpy_d2h")
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        cpu, n = O.render(vol, tf, (0.0, 1.0), en, ex, step, spacing=sp, offset=off)
+        out["oracle_subset"] = {"rays": int(np.sum(cpu[..., 3] > 0)), "max_abs": float(np.abs(gpu - cpu).max()),
+                                "samples": int(n), "cpu_seconds": round(time.perf_counter() - t0, 2)}
+        del vol
+    for bset in bufs:
+        for x in bset:
+            x.free()
+    v.close()
+    t.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def v_linear_ptr(ctx, v):
+    """Device pointer of the volume's linear copy (hipdvr_volume_device_ptr)."""
+    p = C.c_void_p()
+    ctx.check(ctx.lib.hipdvr_volume_device_ptr(v.h, C.byref(p)), "hipdvr_volume_device_ptr")
+    return p
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -223,6 +407,8 @@ def main():
         sys.exit(2)
     if args.config == "c2":
         return main_batch(args)
+    if args.config == "c4":
+        return main_c4(args)
     from voreen_amd import hipdvr as H, workloads as W
 
     cfg = dict(W.CONFIGS[args.config])

@@ -3,12 +3,15 @@
  *
  * These generators have no counterpart in the reference (Voreen reads real
  * volumes from disk); they produce the benchmark and test volumes
- * deterministically from a seed (SplitMix64), on the host, multi-threaded.
+ * deterministically from a seed (SplitMix64), on the host, multi-threaded,
+ * or on the device (hipdvr_synth_turbulence_device).
  */
 #ifndef HIPDVR_SYNTH_H
 #define HIPDVR_SYNTH_H
 
 #include <stdint.h>
+
+#include "hipdvr.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -27,6 +30,18 @@ int hipdvr_synth_blobs(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, int
  * value noise of amplitude 5% of the range; voxels outside the outer shell
  * (about 30% of the cube) are air, exactly 0.  uint16. */
 int hipdvr_synth_shells(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, uint16_t* out);
+
+/* Turbulence on the device, for volumes too large to make on the host: the
+ * sum over 6 octaves of 2^-o |value noise| (frequency 6*2^o lattice cells
+ * across the largest axis), normalized and quantized to uint16, then
+ * thresholded so that the fraction of voxels nearest `empty_fraction` that a
+ * boundary of a 4096-bin histogram allows is exactly 0 and the rest is
+ * rescaled linearly onto [1, 65535] (the top of the highest occupied bin
+ * maps to 65535).  d_out: device memory, x-fastest nx*ny*nz uint16.  The exact
+ * fraction of zero voxels is stored in *empty_out (may be NULL).
+ * Synchronous on the context stream. */
+int hipdvr_synth_turbulence_device(hipdvr_ctx* ctx, uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz,
+                                   float empty_fraction, uint16_t* d_out, double* empty_out);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,7 @@ __global__ void batch_reset_kernel(uint32_t* counter) {
     if (threadIdx.x == 0) counter[0] = 0;
 }
 
-template <int SHADE, bool COUNT, int K, int TILE>
+template <bool WIDE, int SHADE, bool COUNT, int K, int TILE>
 __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict__ jobs,
                                                     const uint32_t* __restrict__ tile_start, uint32_t njobs,
                                                     uint32_t total_tiles, uint32_t* __restrict__ counter,
@@ -81,12 +81,14 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
             const float4 l = a.exit[pix];
             float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
             if (!is_background(e, l)) {
+                // WIDE: 64-bit record addressing for every job (a volume's
+                // records exceed 4 GiB); else 32-bit buffer offsets
                 if (a.dtype == 0) {
-                    const CellFetch<uint8_t, false> f = cell_fetch_of<uint8_t, false>(a);
-                    res = cell_ray<uint8_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
+                    const CellFetch<uint8_t, WIDE> f = cell_fetch_of<uint8_t, WIDE>(a);
+                    res = cell_ray<uint8_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
                 } else {
-                    const CellFetch<uint16_t, false> f = cell_fetch_of<uint16_t, false>(a);
-                    res = cell_ray<uint16_t, false, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
+                    const CellFetch<uint16_t, WIDE> f = cell_fetch_of<uint16_t, WIDE>(a);
+                    res = cell_ray<uint16_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
                 }
             }
             a.out[pix] = res;
@@ -105,22 +107,27 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
 typedef void (*BatchKernel)(const MarchArgs*, const uint32_t*, uint32_t, uint32_t, uint32_t*,
                             unsigned long long*);
 
-template <int SHADE, bool COUNT>
+template <bool WIDE, int SHADE, bool COUNT>
 BatchKernel pick_kernel(int tile, int k) {
     if (k <= 2) {
-        if (tile == 0) return batch_kernel<SHADE, COUNT, 2, 0>;
-        if (tile == 1) return batch_kernel<SHADE, COUNT, 2, 1>;
-        return batch_kernel<SHADE, COUNT, 2, 2>;
+        if (tile == 0) return batch_kernel<WIDE, SHADE, COUNT, 2, 0>;
+        if (tile == 1) return batch_kernel<WIDE, SHADE, COUNT, 2, 1>;
+        return batch_kernel<WIDE, SHADE, COUNT, 2, 2>;
     }
-    if (tile == 0) return batch_kernel<SHADE, COUNT, 4, 0>;
-    if (tile == 1) return batch_kernel<SHADE, COUNT, 4, 1>;
-    return batch_kernel<SHADE, COUNT, 4, 2>;
+    if (tile == 0) return batch_kernel<WIDE, SHADE, COUNT, 4, 0>;
+    if (tile == 1) return batch_kernel<WIDE, SHADE, COUNT, 4, 1>;
+    return batch_kernel<WIDE, SHADE, COUNT, 4, 2>;
 }
 
-BatchKernel kernel_for(int shade, bool count, int tile, int k) {
-    if (shade == 2) return count ? pick_kernel<2, true>(tile, k) : pick_kernel<2, false>(tile, k);
-    if (shade == 1) return count ? pick_kernel<1, true>(tile, k) : pick_kernel<1, false>(tile, k);
-    return count ? pick_kernel<0, true>(tile, k) : pick_kernel<0, false>(tile, k);
+template <bool WIDE>
+BatchKernel kernel_for_w(int shade, bool count, int tile, int k) {
+    if (shade == 2) return count ? pick_kernel<WIDE, 2, true>(tile, k) : pick_kernel<WIDE, 2, false>(tile, k);
+    if (shade == 1) return count ? pick_kernel<WIDE, 1, true>(tile, k) : pick_kernel<WIDE, 1, false>(tile, k);
+    return count ? pick_kernel<WIDE, 0, true>(tile, k) : pick_kernel<WIDE, 0, false>(tile, k);
+}
+
+BatchKernel kernel_for(bool wide, int shade, bool count, int tile, int k) {
+    return wide ? kernel_for_w<true>(shade, count, tile, k) : kernel_for_w<false>(shade, count, tile, k);
 }
 
 }  // namespace
@@ -133,6 +140,7 @@ struct hipdvr_batch {
     hipdvr_ctx* ctx = nullptr;
     std::vector<hipdvr_batch_job> jobs;    // in launch order (largest first)
     int shade = 0;                         // 0 none, 1 Phong, 2 exact Phong
+    bool wide = false;                     // some volume's cell records exceed 4 GiB
     int sms = 0;
     // per brick size: device brick tables per job and the jobs' MarchArgs
     std::vector<void*> allocations;
@@ -178,7 +186,7 @@ void validate_config(const hipdvr_batch_config& c) {
 void launch_batch(hipdvr_batch* b, bool count) {
     hipStream_t st = ctx_stream(b->ctx);
     const int t = b->cfg.tile, bs = b->cfg.brick_log2 - 3;
-    BatchKernel k = kernel_for(b->shade, count, t, b->cfg.batch);
+    BatchKernel k = kernel_for(b->wide, b->shade, count, t, b->cfg.batch);
     int per_cu = 0;
     hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 256, 0), "occupancy query");
     if (per_cu < 1) per_cu = 1;
@@ -215,8 +223,6 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
                          : (j.params.shading == HIPDVR_SHADE_PHONG && !j.params.exact_shading) ? 1 : 2;
             if (shade >= 0 && ph != shade) fail(HIPDVR_ERR_UNSUPPORTED, "all jobs of a batch share the shading path");
             shade = ph;
-            if (cell_record_bytes(j.volume) > 0xffffff00ull)
-                fail(HIPDVR_ERR_UNSUPPORTED, "batched volumes must have cell records below 4 GiB");
         }
         ctx_set_device(ctx);
         hipStream_t st = ctx_stream(ctx);
@@ -238,7 +244,10 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
             for (uint32_t i : order) b->jobs.push_back(jobs[i]);
             // cell records, then per brick size the brick tables of every job
-            for (const hipdvr_batch_job& j : b->jobs) ensure_cells(const_cast<hipdvr_volume*>(j.volume), st);
+            for (const hipdvr_batch_job& j : b->jobs) {
+                ensure_cells(const_cast<hipdvr_volume*>(j.volume), st);
+                b->wide = b->wide || j.volume->cells_wide;
+            }
             std::vector<MarchArgs> args(n);
             for (int bs = 0; bs < kBrickSizes; ++bs) {
                 const uint32_t bl = (uint32_t)bs + 3;

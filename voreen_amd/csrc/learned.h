@@ -46,7 +46,7 @@ void launch_trace(const LearnedArgs& la, float* err, unsigned int* visits, hipSt
 void launch_targets(const float* err, const unsigned int* visits, const float* feat, uint32_t rows,
                     float tau_skip, float tau_step, float* out_x, float* out_t, unsigned int* count,
                     uint32_t capacity, hipStream_t st);
-void launch_learned(const LearnedArgs& la, bool phong, bool count, hipStream_t st);
+void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st);
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
                   int offs, float* sse, int* flag, float mse_max, hipStream_t st);
 

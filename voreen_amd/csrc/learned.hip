@@ -423,6 +423,11 @@ int hipdvr_learned_trace(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_pa
         validate_params(prm, tf);
         if (prm->filter != HIPDVR_FILTER_LINEAR || prm->classification != HIPDVR_CLASS_TF)
             fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping needs the linear filter and TF classification");
+        // the trace reads the tiled copy through 32-bit buffer offsets
+        const uint64_t vsz = l->vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
+        if (l->vol->tiled_elems * vsz > 0xffffff00ull)
+            fail(HIPDVR_ERR_UNSUPPORTED, "learned traces need a volume below 4 GiB; train on a smaller volume "
+                                         "of the same kind (the network reads brick statistics only)");
         ctx_set_device(l->ctx);
         LearnedArgs la;
         fill_learned_args(la, l, tf, prm, d_entry, d_exit, nullptr, w, h);
@@ -506,7 +511,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         LearnedArgs la;
         fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
         la.m.cells = vol->d_cells;
-        launch_learned(la, phong, count, st);
+        launch_learned(la, phong, count, vol->cells_wide, st);
         hip_check(hipStreamWaitEvent(st, l->join, 0), "hipStreamWaitEvent");
         // 3. judge: flag = PSNR below the floor
         const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);

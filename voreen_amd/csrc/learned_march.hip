@@ -408,7 +408,7 @@ void launch_targets(const float* err, const unsigned int* visits, const float* f
 // brick is cut so the next brick starts on the base grid, and each sample's
 // opacity correction is the one of the step it stands for.
 // ---------------------------------------------------------------------------
-template <typename T, bool PHONG, bool COUNT>
+template <typename T, bool WIDE, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     constexpr int K = 4;
     __shared__ uint32_t s_tf[kMaxTfWidth];
@@ -424,8 +424,8 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-    typedef typename CellFetch<T, false>::rec_t rec_t;
-    const CellFetch<T, false> f = cell_fetch_of<T, false>(a);
+    typedef typename CellFetch<T, WIDE>::rec_t rec_t;
+    const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
     unsigned long long n_samples = 0, n_shaded = 0;
     if (tile < tiles_total && px < a.w && py < a.h) {
         const size_t pix = (size_t)py * a.w + px;
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                     mk[k] = m;
                     if (valid[k]) {
                         const float sk[3] = {first[0] + tt * dir[0], first[1] + tt * dir[1], first[2] + tt * dir[2]};
-                        rk[k] = f.load(cell_of<T, false>(f, a, sk, pk[k]));
+                        rk[k] = f.load(cell_of<T, WIDE>(f, a, sk, pk[k]));
                     }
                     tt = tt + (float)m * a.step;
                 }
@@ -508,8 +508,8 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                                                  first[2] + t0 * dir[2]};
                             float gr[3];
                             if (t0 == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                            else if (!cell_gradient_interior<T, false>(f, a, ss, v, pk[k], gr))
-                                cell_gradient<T, false>(f, a, ss, gr);
+                            else if (!cell_gradient_interior<T, WIDE>(f, a, ss, v, pk[k], gr))
+                                cell_gradient<T, WIDE>(f, a, ss, gr);
                             float pos[3];
                             for (int i = 0; i < 3; ++i)
                                 pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];
@@ -539,25 +539,26 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     }
 }
 
-void launch_learned(const LearnedArgs& la, bool phong, bool count, hipStream_t st) {
+template <typename T, bool WIDE>
+static void launch_learned_t(const LearnedArgs& la, bool phong, bool count, dim3 grid, hipStream_t st) {
+    if (phong) {
+        if (count) hipLaunchKernelGGL((learned_kernel<T, WIDE, true, true>), grid, dim3(256), 0, st, la);
+        else hipLaunchKernelGGL((learned_kernel<T, WIDE, true, false>), grid, dim3(256), 0, st, la);
+    } else {
+        if (count) hipLaunchKernelGGL((learned_kernel<T, WIDE, false, true>), grid, dim3(256), 0, st, la);
+        else hipLaunchKernelGGL((learned_kernel<T, WIDE, false, false>), grid, dim3(256), 0, st, la);
+    }
+}
+
+void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st) {
     const uint32_t tiles = ((la.m.w + 15) / 16) * ((la.m.h + 15) / 16);
     const dim3 grid(((tiles + 7) / 8) * 8);
     if (la.u16) {
-        if (phong) {
-            if (count) hipLaunchKernelGGL((learned_kernel<uint16_t, true, true>), grid, dim3(256), 0, st, la);
-            else hipLaunchKernelGGL((learned_kernel<uint16_t, true, false>), grid, dim3(256), 0, st, la);
-        } else {
-            if (count) hipLaunchKernelGGL((learned_kernel<uint16_t, false, true>), grid, dim3(256), 0, st, la);
-            else hipLaunchKernelGGL((learned_kernel<uint16_t, false, false>), grid, dim3(256), 0, st, la);
-        }
+        if (wide) launch_learned_t<uint16_t, true>(la, phong, count, grid, st);
+        else launch_learned_t<uint16_t, false>(la, phong, count, grid, st);
     } else {
-        if (phong) {
-            if (count) hipLaunchKernelGGL((learned_kernel<uint8_t, true, true>), grid, dim3(256), 0, st, la);
-            else hipLaunchKernelGGL((learned_kernel<uint8_t, true, false>), grid, dim3(256), 0, st, la);
-        } else {
-            if (count) hipLaunchKernelGGL((learned_kernel<uint8_t, false, true>), grid, dim3(256), 0, st, la);
-            else hipLaunchKernelGGL((learned_kernel<uint8_t, false, false>), grid, dim3(256), 0, st, la);
-        }
+        if (wide) launch_learned_t<uint8_t, true>(la, phong, count, grid, st);
+        else launch_learned_t<uint8_t, false>(la, phong, count, grid, st);
     }
 }
 

@@ -67,7 +67,7 @@ struct CellFetch {
     typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type off_t;
     const rec_t* p;
     __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t sy, sz;   // records per group row (8*GX) and per group slab (8*GX*GY)
+    off_t sy, sz;      // records per group row (8*GX) and per group slab (8*GX*GY)
 
     __device__ __forceinline__ off_t index(const uint32_t lo[3]) const {
         return (off_t)(((lo[0] >> 1) << 3) | (lo[0] & 1)) + (off_t)(lo[1] >> 1) * sy + ((lo[1] & 1) << 1) +
@@ -388,7 +388,7 @@ __device__ __forceinline__ CellFetch<T, WIDE> cell_fetch_of(const MarchArgs& a) 
     CellFetch<T, WIDE> f;
     f.p = (const rec_t*)a.cells;
     const uint32_t GX = (a.dims[0] + 1) >> 1, GY = (a.dims[1] + 1) >> 1, GZ = (a.dims[2] + 1) >> 1;
-    f.sy = GX << 3;
+    f.sy = (typename CellFetch<T, WIDE>::off_t)GX << 3;
     f.sz = f.sy * GY;
     if (!WIDE) {
         const uint32_t bytes = f.sz * GZ * (uint32_t)sizeof(rec_t);

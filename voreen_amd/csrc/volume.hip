@@ -14,65 +14,79 @@ namespace {
 
 // One lane per tiled element: gather from the linear copy (clamped), write
 // the tiled copy contiguously.
+// (Grid-stride: the element count of an 8 GiB volume reaches 2^32, beyond
+// what one dispatch addresses.)
 template <typename T>
 __global__ __launch_bounds__(256) void tile_kernel(const T* __restrict__ src, T* __restrict__ dst,
                                                    uint32_t X, uint32_t Y, uint32_t Z, uint32_t nbx,
                                                    uint32_t nby, uint64_t total) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint32_t in = (uint32_t)(i & 63);
-    const uint64_t blk = i >> 6;
-    const uint32_t bx = (uint32_t)(blk % nbx);
-    const uint64_t r = blk / nbx;
-    const uint32_t by = (uint32_t)(r % nby);
-    const uint32_t bz = (uint32_t)(r / nby);
-    uint32_t x = bx * 4 + (in & 3), y = by * 4 + ((in >> 2) & 3), z = bz * 4 + (in >> 4);
-    x = x < X ? x : X - 1;
-    y = y < Y ? y : Y - 1;
-    z = z < Z ? z : Z - 1;
-    dst[i] = src[((uint64_t)z * Y + y) * X + x];
-}
-
-// One workgroup per brick: min/max over [b*B, b*B + B] per axis, clamped.
-template <typename T>
-__global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ src, uint2* __restrict__ mm,
-                                                     uint32_t X, uint32_t Y, uint32_t Z, uint32_t bl,
-                                                     uint32_t nbx, uint32_t nby) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const uint32_t B1 = (1u << bl) + 1;
-    const uint32_t n = B1 * B1 * B1;
-    uint32_t lo = 0xffffffffu, hi = 0;
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        uint32_t x = (bx << bl) + k % B1;
-        uint32_t y = (by << bl) + (k / B1) % B1;
-        uint32_t z = (bz << bl) + k / (B1 * B1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t in = (uint32_t)(i & 63);
+        const uint64_t blk = i >> 6;
+        const uint32_t bx = (uint32_t)(blk % nbx);
+        const uint64_t r = blk / nbx;
+        const uint32_t by = (uint32_t)(r % nby);
+        const uint32_t bz = (uint32_t)(r / nby);
+        uint32_t x = bx * 4 + (in & 3), y = by * 4 + ((in >> 2) & 3), z = bz * 4 + (in >> 4);
         x = x < X ? x : X - 1;
         y = y < Y ? y : Y - 1;
         z = z < Z ? z : Z - 1;
-        const uint32_t v = src[((uint64_t)z * Y + y) * X + x];
-        lo = v < lo ? v : lo;
-        hi = v > hi ? v : hi;
+        dst[i] = src[((uint64_t)z * Y + y) * X + x];
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
-        lo = l2 < lo ? l2 : lo;
-        hi = h2 > hi ? h2 : hi;
-    }
-    __shared__ uint32_t s_lo[4], s_hi[4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        s_lo[w] = lo;
-        s_hi[w] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
-            lo = s_lo[i] < lo ? s_lo[i] : lo;
-            hi = s_hi[i] > hi ? s_hi[i] : hi;
+}
+
+// Workgroups of a grid-stride launch: one per per_block items, at most 2^20.
+inline unsigned stride_grid(uint64_t items, uint64_t per_block = 256) {
+    const uint64_t want = (items + per_block - 1) / per_block;
+    const uint64_t cap = 1ull << 20;
+    return (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
+}
+
+// One workgroup per brick (grid-stride over the bricks): min/max over
+// [b*B, b*B + B] per axis, clamped.
+template <typename T>
+__global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ src, uint2* __restrict__ mm,
+                                                     uint32_t X, uint32_t Y, uint32_t Z, uint32_t bl,
+                                                     uint32_t nbx, uint32_t nby, uint64_t nbricks) {
+    for (uint64_t b64 = blockIdx.x; b64 < nbricks; b64 += gridDim.x) {
+        const uint32_t b = (uint32_t)b64;
+        const uint32_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+        const uint32_t B1 = (1u << bl) + 1;
+        const uint32_t n = B1 * B1 * B1;
+        uint32_t lo = 0xffffffffu, hi = 0;
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            uint32_t x = (bx << bl) + k % B1;
+            uint32_t y = (by << bl) + (k / B1) % B1;
+            uint32_t z = (bz << bl) + k / (B1 * B1);
+            x = x < X ? x : X - 1;
+            y = y < Y ? y : Y - 1;
+            z = z < Z ? z : Z - 1;
+            const uint32_t v = src[((uint64_t)z * Y + y) * X + x];
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
         }
-        mm[b] = make_uint2(lo, hi);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        }
+        __shared__ uint32_t s_lo[4], s_hi[4];
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            s_lo[w] = lo;
+            s_hi[w] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+                lo = s_lo[i] < lo ? s_lo[i] : lo;
+                hi = s_hi[i] > hi ? s_hi[i] : hi;
+            }
+            mm[b] = make_uint2(lo, hi);
+        }
+        __syncthreads();   // s_lo / s_hi are rewritten for the next brick
     }
 }
 
@@ -81,37 +95,38 @@ template <typename T>
 __global__ __launch_bounds__(256) void cells_kernel(const T* __restrict__ src, void* __restrict__ dst, uint32_t X,
                                                     uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
                                                     uint64_t total) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint32_t in = (uint32_t)(i & 7);
-    const uint64_t grp = i >> 3;
-    const uint32_t x = (uint32_t)(grp % gx) * 2 + (in & 1);
-    const uint64_t r = grp / gx;
-    const uint32_t y = (uint32_t)(r % gy) * 2 + ((in >> 1) & 1);
-    const uint32_t z = (uint32_t)(r / gy) * 2 + (in >> 2);
-    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (x < X && y < Y && z < Z) {
-        const uint32_t x1 = x + 1 < X ? x + 1 : X - 1;
-        const uint32_t y1 = y + 1 < Y ? y + 1 : Y - 1;
-        const uint32_t z1 = z + 1 < Z ? z + 1 : Z - 1;
-        const uint64_t XY = (uint64_t)X * Y;
-        const uint64_t b0 = (uint64_t)z * XY, b1 = (uint64_t)z1 * XY;
-        // trilinear order: llB lrB urB ulB llF lrF urF ulF
-        v[0] = src[b0 + (uint64_t)y * X + x];
-        v[1] = src[b0 + (uint64_t)y * X + x1];
-        v[2] = src[b0 + (uint64_t)y1 * X + x1];
-        v[3] = src[b0 + (uint64_t)y1 * X + x];
-        v[4] = src[b1 + (uint64_t)y * X + x];
-        v[5] = src[b1 + (uint64_t)y * X + x1];
-        v[6] = src[b1 + (uint64_t)y1 * X + x1];
-        v[7] = src[b1 + (uint64_t)y1 * X + x];
-    }
-    if (sizeof(T) == 2) {
-        ((uint4*)dst)[i] = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16),
-                                      v[6] | (v[7] << 16));
-    } else {
-        ((uint2*)dst)[i] = make_uint2(v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
-                                      v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t in = (uint32_t)(i & 7);
+        const uint64_t grp = i >> 3;
+        const uint32_t x = (uint32_t)(grp % gx) * 2 + (in & 1);
+        const uint64_t r = grp / gx;
+        const uint32_t y = (uint32_t)(r % gy) * 2 + ((in >> 1) & 1);
+        const uint32_t z = (uint32_t)(r / gy) * 2 + (in >> 2);
+        uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x < X && y < Y && z < Z) {
+            const uint32_t x1 = x + 1 < X ? x + 1 : X - 1;
+            const uint32_t y1 = y + 1 < Y ? y + 1 : Y - 1;
+            const uint32_t z1 = z + 1 < Z ? z + 1 : Z - 1;
+            const uint64_t XY = (uint64_t)X * Y;
+            const uint64_t b0 = (uint64_t)z * XY, b1 = (uint64_t)z1 * XY;
+            // trilinear order: llB lrB urB ulB llF lrF urF ulF
+            v[0] = src[b0 + (uint64_t)y * X + x];
+            v[1] = src[b0 + (uint64_t)y * X + x1];
+            v[2] = src[b0 + (uint64_t)y1 * X + x1];
+            v[3] = src[b0 + (uint64_t)y1 * X + x];
+            v[4] = src[b1 + (uint64_t)y * X + x];
+            v[5] = src[b1 + (uint64_t)y * X + x1];
+            v[6] = src[b1 + (uint64_t)y1 * X + x1];
+            v[7] = src[b1 + (uint64_t)y1 * X + x];
+        }
+        if (sizeof(T) == 2) {
+            ((uint4*)dst)[i] = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16),
+                                          v[6] | (v[7] << 16));
+        } else {
+            ((uint2*)dst)[i] = make_uint2(v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
+                                          v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+        }
     }
 }
 
@@ -119,13 +134,13 @@ template <typename T>
 void build_layouts(hipdvr_volume* v, hipStream_t st) {
     const uint32_t X = v->desc.dims[0], Y = v->desc.dims[1], Z = v->desc.dims[2];
     const uint64_t total = v->tiled_elems;
-    hipLaunchKernelGGL(tile_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(tile_kernel<T>, dim3(stride_grid(total)), dim3(256), 0, st,
                        (const T*)v->d_linear, (T*)v->d_tiled, X, Y, Z, v->nblocks[0], v->nblocks[1],
                        total);
     hip_check(hipGetLastError(), "tile kernel launch");
-    hipLaunchKernelGGL(minmax_kernel<T>, dim3((unsigned)v->brick_count), dim3(256), 0, st,
+    hipLaunchKernelGGL(minmax_kernel<T>, dim3(stride_grid(v->brick_count, 1)), dim3(256), 0, st,
                        (const T*)v->d_linear, v->d_minmax, X, Y, Z, v->desc.brick_log2,
-                       v->nbricks[0], v->nbricks[1]);
+                       v->nbricks[0], v->nbricks[1], v->brick_count);
     hip_check(hipGetLastError(), "min/max kernel launch");
 }
 
@@ -271,13 +286,14 @@ void volume_raw_minmax(const hipdvr_volume* v, hipStream_t st, uint32_t out[2]) 
 
 void launch_brick_minmax(const hipdvr_volume* v, uint32_t bl, const uint32_t nb[3], uint2* out, hipStream_t st) {
     const uint32_t X = v->desc.dims[0], Y = v->desc.dims[1], Z = v->desc.dims[2];
-    const dim3 grid((unsigned)((uint64_t)nb[0] * nb[1] * nb[2]));
+    const uint64_t count = (uint64_t)nb[0] * nb[1] * nb[2];
+    const dim3 grid(stride_grid(count, 1));
     if (v->desc.dtype == HIPDVR_U8)
         hipLaunchKernelGGL(minmax_kernel<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)v->d_linear, out, X, Y,
-                           Z, bl, nb[0], nb[1]);
+                           Z, bl, nb[0], nb[1], count);
     else
         hipLaunchKernelGGL(minmax_kernel<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)v->d_linear, out, X,
-                           Y, Z, bl, nb[0], nb[1]);
+                           Y, Z, bl, nb[0], nb[1], count);
 }
 
 uint64_t cell_record_bytes(const hipdvr_volume* v) {
@@ -296,7 +312,7 @@ void ensure_cells(hipdvr_volume* v, hipStream_t st) {
     const uint32_t gx = (v->desc.dims[0] + 1) / 2, gy = (v->desc.dims[1] + 1) / 2;
     void* p = nullptr;
     hip_check(hipMalloc(&p, bytes), "hipMalloc(cell records)");
-    const dim3 grid((unsigned)((total + 255) / 256));
+    const dim3 grid(stride_grid(total));
     if (v->desc.dtype == HIPDVR_U8)
         hipLaunchKernelGGL(cells_kernel<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)v->d_linear, p,
                            v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], gx, gy, total);

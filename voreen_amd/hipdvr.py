@@ -37,7 +37,7 @@ EXPORTS = (
     "hipdvr_preint_build",
     "hipdvr_render", "hipdvr_render_device", "hipdvr_entry_exit_device",
     "hipdvr_buffer_alloc", "hipdvr_buffer_free", "hipdvr_memcpy_h2d", "hipdvr_memcpy_d2h",
-    "hipdvr_synth_blobs", "hipdvr_synth_shells",
+    "hipdvr_synth_blobs", "hipdvr_synth_shells", "hipdvr_synth_turbulence_device",
     "hipdvr_timing_begin", "hipdvr_timing_read", "hipdvr_copy_device",
     "hipdvr_mlp_create", "hipdvr_mlp_destroy", "hipdvr_mlp_get_params", "hipdvr_mlp_set_params",
     "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
@@ -161,6 +161,7 @@ def load():
         "hipdvr_memcpy_d2h": ([vp, vp, vp, C.c_size_t], C.c_int),
         "hipdvr_synth_blobs": ([C.c_uint64, u32, u32, u32, i32, vp], C.c_int),
         "hipdvr_synth_shells": ([C.c_uint64, u32, u32, u32, vp], C.c_int),
+        "hipdvr_synth_turbulence_device": ([vp, C.c_uint64, u32, u32, u32, C.c_float, vp, P(C.c_double)], C.c_int),
         "hipdvr_timing_begin": ([vp, u32], C.c_int),
         "hipdvr_timing_read": ([vp, C.POINTER(C.c_float), u32, C.POINTER(u32)], C.c_int),
         "hipdvr_copy_device": ([vp, vp, vp, C.c_size_t], C.c_int),
@@ -352,6 +353,36 @@ class Context:
         self.check(self.lib.hipdvr_tf2d_create(self.h, _ptr(rgba), rgba.shape[1], rgba.shape[0], dom,
                                                C.byref(t.h)), "hipdvr_tf2d_create")
         return t
+
+    def synth_turbulence(self, seed, dims, empty_fraction=0.4):
+        """uint16 turbulence volume made on the device (hipdvr_synth_turbulence_device):
+        (device buffer, exact empty fraction); dims = (x, y, z)."""
+        nx, ny, nz = dims
+        buf = self.alloc(nx * ny * nz * 2)
+        frac = C.c_double()
+        try:
+            self.check(self.lib.hipdvr_synth_turbulence_device(self.h, seed, nx, ny, nz, empty_fraction,
+                                                               buf.ptr, C.byref(frac)),
+                       "hipdvr_synth_turbulence_device")
+        except Exception:
+            buf.free()
+            raise
+        return buf, frac.value
+
+    def volume_device(self, buf, dims, dtype=U16, rwm=(1.0, 0.0), spacing=(1.0, 1.0, 1.0),
+                      offset=(0.0, 0.0, 0.0), brick_log2=4):
+        """Volume from device-resident voxels (hipdvr_volume_create_device); dims = (x, y, z)."""
+        d = VolumeDesc()
+        d.dims[:] = list(dims)
+        d.dtype = dtype
+        d.rwm_scale, d.rwm_offset = rwm
+        d.spacing[:] = list(spacing)
+        d.offset[:] = list(offset)
+        d.brick_log2 = brick_log2
+        h = C.c_void_p()
+        self.check(self.lib.hipdvr_volume_create_device(self.h, C.byref(d), buf.ptr, C.byref(h)),
+                   "hipdvr_volume_create_device")
+        return Volume.from_handle(self, h, d)
 
     def load_vvd(self, path, brick_log2=4):
         h, d = C.c_void_p(), VolumeDesc()

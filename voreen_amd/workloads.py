@@ -57,6 +57,19 @@ def shells_keys(seed):
             (0.86, col(0)), (1.0, col(a2))]
 
 
+def turbulence_keys(seed):
+    """7 keys for the thresholded turbulence volume: the empty voxels and the
+    lowest values transparent, a faint haze over [0.06, 0.22], denser
+    structure from 0.3 up."""
+    r = SplitMix64(seed)
+
+    def col(a):
+        return (r.randint(256), r.randint(256), r.randint(256), a)
+
+    return [(0.0, col(0)), (0.04, col(0)), (0.10, col(12 + r.randint(20))), (0.22, col(4)),
+            (0.30, col(60 + r.randint(60))), (0.50, col(140 + r.randint(60))), (1.0, col(255))]
+
+
 def world_box(dims, spacing=(1.0, 1.0, 1.0)):
     """Proxy box centred at the origin, longest physical extent 1."""
     ext = np.array(dims, np.float64) * np.array(spacing, np.float64)
@@ -89,6 +102,9 @@ CONFIGS = {
                shading=H.SHADE_NONE, vol_seed=0xC0, tf_seed=0xC0F, tf_width=256),
     "c1": dict(dims=(512, 512, 512), dtype=H.U16, image=(1024, 1024), rate=2.0, views=36,
                shading=H.SHADE_PHONG, vol_seed=0xC1, tf_seed=0xC1F, tf_width=4096),
+    # large-volume stress: 8 GiB of thresholded turbulence made on the device
+    "c4": dict(dims=(2048, 2048, 1024), dtype=H.U16, image=(3840, 2160), rate=2.0, views=8,
+               shading=H.SHADE_PHONG, vol_seed=0xC4, tf_seed=0xC4F, tf_width=4096, empty=0.4),
 }
 
 
@@ -100,9 +116,17 @@ def make_volume(cfg, dims=None):
     return H.synth_shells(c["vol_seed"], dims)
 
 
+def make_volume_device(ctx, cfg, dims=None):
+    """The turbulence volume of `cfg` made on the device: (device buffer,
+    exact empty fraction)."""
+    c = CONFIGS[cfg]
+    return ctx.synth_turbulence(c["vol_seed"], dims or c["dims"], c["empty"])
+
+
 def make_tf(cfg):
     c = CONFIGS[cfg]
-    keys = random_keys(c["tf_seed"], 4) if cfg == "c0" else shells_keys(c["tf_seed"])
+    keys = (random_keys(c["tf_seed"], 4) if cfg == "c0" else
+            turbulence_keys(c["tf_seed"]) if cfg == "c4" else shells_keys(c["tf_seed"]))
     return keys, H.tf_from_keys(keys, c["tf_width"])
 
 
