@@ -94,7 +94,9 @@ int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_
 /* Training data: clear the trace accumulators, trace one view (device
  * entry/exit) into them, and append the visited rows (features of the last
  * prepare + targets) to device arrays x [capacity][16], t [capacity][3]
- * starting at *count (updated). */
+ * starting at *count (updated).  Tracing needs a volume below 4 GiB
+ * (HIPDVR_ERR_UNSUPPORTED otherwise); a network trained on a smaller volume
+ * of the same kind serves a larger one, since it reads brick statistics. */
 int hipdvr_learned_trace_reset(hipdvr_learned* l);
 int hipdvr_learned_trace(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_params* prm,
                          const float* d_entry, const float* d_exit, uint32_t w, uint32_t h);

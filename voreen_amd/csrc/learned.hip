@@ -315,9 +315,8 @@ int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_lear
     return ctx_guarded(ctx, [&] {
         if (!vol || !out) fail(HIPDVR_ERR_INVALID, "NULL argument");
         if (vol->ctx != ctx) fail(HIPDVR_ERR_INVALID, "volume from another context");
-        const uint64_t vsz = vol->desc.dtype == HIPDVR_U8 ? 1 : 2;
-        if (vol->tiled_elems * vsz > 0xffffff00ull || cell_record_bytes(vol) > 0xffffff00ull)
-            fail(HIPDVR_ERR_UNSUPPORTED, "learned stepping supports volumes whose cell records are below 4 GiB");
+        if (!cells_allowed(vol))
+            fail(HIPDVR_ERR_UNSUPPORTED, "the volume's cell records exceed the library's size limit");
         ctx_set_device(ctx);
         hipdvr_learned* l = new hipdvr_learned();
         l->ctx = ctx;
