@@ -419,8 +419,7 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     __syncthreads();
     const uint32_t tiles_x = (a.w + 15) / 16;
     const uint32_t tiles_total = tiles_x * ((a.h + 15) / 16);
-    const uint32_t per_xcd = (tiles_total + 7) / 8;
-    const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const uint32_t tile = blockIdx.x;   // row-major tiles in dispatch order (see march_cell.hip)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -552,7 +551,7 @@ static void launch_learned_t(const LearnedArgs& la, bool phong, bool count, dim3
 
 void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st) {
     const uint32_t tiles = ((la.m.w + 15) / 16) * ((la.m.h + 15) / 16);
-    const dim3 grid(((tiles + 7) / 8) * 8);
+    const dim3 grid(tiles);
     if (la.u16) {
         if (wide) launch_learned_t<uint16_t, true>(la, phong, count, grid, st);
         else launch_learned_t<uint16_t, false>(la, phong, count, grid, st);

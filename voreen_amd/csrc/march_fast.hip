@@ -12,8 +12,8 @@
 //     maps to a zero-alpha TF entry, so it is not reconstructed; inside such a
 //     brick the ray fast-forwards with t += step (the reference's float
 //     sequence) up to an analytic bound of the brick exit;
-//   * workgroups are remapped so each XCD marches a contiguous band of tiles
-//     (its L2 holds the bricks that band reads);
+//   * tiles run in dispatch order, so the rays in flight on all XCDs cover one
+//     compact image region and share the bricks in the Infinity Cache;
 //   * Phong's central differences (mod_gradients.frag:202-214: six trilinear
 //     lookups at +-1 voxel) are evaluated from per-axis plane bilinears that
 //     share the sample's weights: the same clamped stencil positions and
@@ -212,12 +212,9 @@ __device__ __forceinline__ bool gradient_interior(const TiledFetch<T, WIDE>& f, 
 
 template <typename T, bool WIDE, int FILTER, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, const uint32_t tiles_x,
-                                                         const uint32_t tiles_total,
-                                                         const uint32_t tiles_per_xcd) {
-    // XCD-aware tile order: hardware places workgroup b on XCD b % 8, so XCD k
-    // gets tiles [k*tiles_per_xcd, (k+1)*tiles_per_xcd) in row-major order.
-    const uint32_t b = blockIdx.x;
-    const uint32_t tile = (b & 7u) * tiles_per_xcd + (b >> 3);
+                                                         const uint32_t tiles_total) {
+    // row-major tiles in dispatch order (see march_cell.hip)
+    const uint32_t tile = blockIdx.x;
     if (tile >= tiles_total) return;   // whole workgroup leaves together
     if (a.cond_flag && *a.cond_flag == 0) return;
     __shared__ uint32_t s_tf[kMaxTfWidth];
@@ -406,14 +403,13 @@ template <typename T, bool WIDE, int FILTER, bool PHONG>
 static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
     const uint32_t total = tx * ty;
-    const uint32_t per_xcd = (total + 7) / 8;
-    const dim3 grid(per_xcd * 8);
+    const dim3 grid(total);
     if (count)
         hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, true>), grid, dim3(256), 0, st,
-                           a, tx, total, per_xcd);
+                           a, tx, total);
     else
         hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, false>), grid, dim3(256), 0, st,
-                           a, tx, total, per_xcd);
+                           a, tx, total);
 }
 
 template <typename T, bool WIDE>
