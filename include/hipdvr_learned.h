@@ -109,6 +109,15 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
                                  const hipdvr_learned_opts* opts, float* d_out,
                                  hipdvr_learned_stats* stats);
+/* The judge's plain re-march on its own: every opts->remarch_stride-th pixel
+ * in x and y (from stride/2), marched by the plain path with each ray cut
+ * into segments that run in parallel and are joined front to back (equal to
+ * the sequential march up to float rounding of the joins).  d_sparse
+ * receives *sw x *sh RGBA floats (row-major); synchronous. */
+int hipdvr_learned_sparse_remarch(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
+                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
+                                  const hipdvr_learned_opts* opts, float* d_sparse, uint32_t* sw,
+                                  uint32_t* sh);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,10 @@ constexpr int kDefaultBatch = 4;
 
 // Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+// The plain re-march of a sparse pixel grid (a.grid_w x a.grid_h, a.stride,
+// a.pix_offset) in ray segments (sparse.hip); seg: sparse_segment_bytes().
+size_t sparse_segment_bytes(uint32_t rays);
+void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st);
 // Streaming copy of n 16-byte vectors (bandwidth calibration).
 void launch_copy16(void* dst, const void* src, size_t n, hipStream_t st);
 void launch_entry_exit(const RayGen& g, float4* entry, float4* exit, hipStream_t st);

@@ -213,6 +213,8 @@ struct hipdvr_learned {
     float alpha_m_step = -1.f;       // step the tables were made for
     float4* d_sparse = nullptr;      // sparse plain image
     size_t sparse_cap = 0;
+    float4* d_seg = nullptr;         // its ray segments
+    size_t seg_cap = 0;
     float* d_sse = nullptr;
     int* d_flag = nullptr;
     unsigned int* d_count = nullptr;
@@ -223,6 +225,54 @@ struct hipdvr_learned {
 };
 
 namespace {
+
+// The judge's sparse pixels (gx*stride + offs, gy*stride + offs), inside the
+// image, and device buffers sized for them.
+struct SparseGrid {
+    uint32_t stride, offs;
+    int w, h;
+};
+
+SparseGrid sparse_grid(hipdvr_learned* l, uint32_t w, uint32_t h, uint32_t stride) {
+    SparseGrid g;
+    g.stride = stride;
+    g.offs = stride / 2;
+    if (g.offs > w - 1) g.offs = w - 1;
+    if (g.offs > h - 1) g.offs = h - 1;
+    g.w = (int)((w - 1 - g.offs) / stride) + 1;
+    g.h = (int)((h - 1 - g.offs) / stride) + 1;
+    const size_t need = (size_t)g.w * g.h * sizeof(float4);
+    if (need > l->sparse_cap) {
+        if (l->d_sparse) (void)hipFree(l->d_sparse);
+        l->d_sparse = nullptr;
+        l->sparse_cap = 0;
+        hip_check(hipMalloc((void**)&l->d_sparse, need), "hipMalloc(sparse)");
+        l->sparse_cap = need;
+    }
+    const size_t seg = sparse_segment_bytes((uint32_t)g.w * (uint32_t)g.h);
+    if (seg > l->seg_cap) {
+        if (l->d_seg) (void)hipFree(l->d_seg);
+        l->d_seg = nullptr;
+        l->seg_cap = 0;
+        hip_check(hipMalloc((void**)&l->d_seg, seg), "hipMalloc(sparse segments)");
+        l->seg_cap = seg;
+    }
+    return g;
+}
+
+// The plain re-march of the sparse pixels into d_out (g.w x g.h RGBA).
+void launch_sparse(hipdvr_learned* l, hipdvr_volume* vol, const hipdvr_tf* tf, const hipdvr_params* prm,
+                   const float* d_entry, const float* d_exit, uint32_t w, uint32_t h, const SparseGrid& g, int shade,
+                   float* d_out, hipStream_t st) {
+    MarchArgs sp;
+    fill_march_args(sp, vol, tf, prm, d_entry, d_exit, d_out, w, h, nullptr);
+    sp.cells = vol->d_cells;
+    sp.grid_w = g.w;
+    sp.grid_h = g.h;
+    sp.stride = (int32_t)g.stride;
+    sp.pix_offset = (int32_t)g.offs;
+    launch_sparse_remarch(sp, vol->desc.dtype, shade, vol->cells_wide, l->d_seg, st);
+}
 
 void free_learned(hipdvr_learned* l) {
     if (!l) return;
@@ -238,6 +288,7 @@ void free_learned(hipdvr_learned* l) {
     (void)hipFree(l->d_nz_prefix);
     (void)hipFree(l->d_alpha_m);
     (void)hipFree(l->d_sparse);
+    (void)hipFree(l->d_seg);
     (void)hipFree(l->d_sse);
     (void)hipFree(l->d_flag);
     (void)hipFree(l->d_count);
@@ -474,21 +525,9 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         hipStream_t st = ctx_stream(l->ctx);
         hipdvr_volume* vol = const_cast<hipdvr_volume*>(l->vol);
         ensure_occupancy(l->ctx, vol, tf);
-        // sparse judge pixels: (gx*stride + offs, gy*stride + offs), inside the image
-        const uint32_t stride = (uint32_t)opts.remarch_stride;
-        uint32_t offs = stride / 2;
-        if (offs > w - 1) offs = w - 1;
-        if (offs > h - 1) offs = h - 1;
-        const int swc = (int)((w - 1 - offs) / stride) + 1;
-        const int shc = (int)((h - 1 - offs) / stride) + 1;
-        const size_t need = (size_t)swc * shc * sizeof(float4);
-        if (need > l->sparse_cap) {
-            if (l->d_sparse) (void)hipFree(l->d_sparse);
-            l->d_sparse = nullptr;
-            l->sparse_cap = 0;
-            hip_check(hipMalloc((void**)&l->d_sparse, need), "hipMalloc(sparse)");
-            l->sparse_cap = need;
-        }
+        const SparseGrid sg = sparse_grid(l, w, h, (uint32_t)opts.remarch_stride);
+        const uint32_t stride = sg.stride, offs = sg.offs;
+        const int swc = sg.w, shc = sg.h;
         const bool phong = prm->shading == HIPDVR_SHADE_PHONG;
         const int shade = phong ? (prm->exact_shading ? 2 : 1) : 0;
         const bool count = stats != nullptr;
@@ -499,13 +538,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         ensure_cells(vol, st);
         hip_check(hipEventRecord(l->fork, st), "hipEventRecord");
         hip_check(hipStreamWaitEvent(l->side, l->fork, 0), "hipStreamWaitEvent");
-        MarchArgs sp;
-        fill_march_args(sp, vol, tf, prm, d_entry, d_exit, (float*)l->d_sparse, w, h, nullptr);
-        sp.grid_w = swc;
-        sp.grid_h = shc;
-        sp.stride = (int32_t)stride;
-        sp.pix_offset = (int32_t)offs;
-        launch_march_cell(sp, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, l->side);
+        launch_sparse(l, vol, tf, prm, d_entry, d_exit, w, h, sg, shade, (float*)l->d_sparse, l->side);
         hip_check(hipEventRecord(l->join, l->side), "hipEventRecord");
         LearnedArgs la;
         fill_learned_args(la, l, tf, prm, d_entry, d_exit, d_out, w, h);
@@ -537,6 +570,35 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
             stats->sparse_psnr = mse > 0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.f;
             stats->fell_back = flag;
         }
+    });
+}
+
+int hipdvr_learned_sparse_remarch(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
+                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
+                                  const hipdvr_learned_opts* o, float* d_sparse, uint32_t* sw, uint32_t* sh) {
+    if (!l) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(l->ctx, [&] {
+        if (!tf || !prm || !d_entry || !d_exit || !d_sparse || !sw || !sh || !w || !h)
+            fail(HIPDVR_ERR_INVALID, "bad arguments");
+        validate_params(prm, tf);
+        if (prm->filter != HIPDVR_FILTER_LINEAR || prm->classification != HIPDVR_CLASS_TF ||
+            prm->compositing != HIPDVR_COMPOSITE_DVR ||
+            (prm->shading != HIPDVR_SHADE_NONE && prm->shading != HIPDVR_SHADE_PHONG) ||
+            (prm->shading != HIPDVR_SHADE_NONE && prm->gradient != HIPDVR_GRADIENT_CENTRAL))
+            fail(HIPDVR_ERR_UNSUPPORTED, "the sparse re-march serves the learned path's options");
+        const hipdvr_learned_opts opts = default_opts(o);
+        ctx_set_device(l->ctx);
+        hipStream_t st = ctx_stream(l->ctx);
+        hipdvr_volume* vol = const_cast<hipdvr_volume*>(l->vol);
+        ensure_occupancy(l->ctx, vol, tf);
+        ensure_cells(vol, st);
+        const SparseGrid g = sparse_grid(l, w, h, (uint32_t)opts.remarch_stride);
+        const int shade = prm->shading == HIPDVR_SHADE_PHONG ? (prm->exact_shading ? 2 : 1) : 0;
+        launch_sparse(l, vol, tf, prm, d_entry, d_exit, w, h, g, shade, d_sparse, st);
+        hip_check(hipGetLastError(), "sparse re-march launch");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *sw = (uint32_t)g.w;
+        *sh = (uint32_t)g.h;
     });
 }
 

@@ -239,25 +239,22 @@ constexpr uint32_t kMaxLdsOccWords = 1024;
 // oracle's tolerance), 2 Phong bit-identical to the reference-layout kernel.  s_tf / s_alpha are the TF and
 // the opacity-correction table in LDS; a may live in kernel arguments or in
 // global memory (batched launches).
-template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
-__device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
-                                           const float* s_alpha, const uint32_t* s_occ, int px, int py, float4 e,
-                                           float4 l, RayCounters& n) {
+// The sample loop of one ray from state (t, loop, r, g, bb, A) up to sample
+// index cap (exclusive) or the ray's end; returns true when the ray is
+// finished (past tend, or terminated early when ERT).  ERT = false marches
+// through without early termination (a segment composited from A = 0).
+template <typename T, bool WIDE, int SHADE, bool COUNT, int K, bool ERT = true>
+__device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
+                                           const float* s_alpha, const uint32_t* s_occ, int px, int py,
+                                           const float first[3], const float dir[3], float tend, int cap, float& t,
+                                           int& loop, float& r, float& g, float& bb, float& A, RayCounters& n) {
     typedef typename CellFetch<T, WIDE>::rec_t rec_t;
     const uint32_t bl = a.brick_log2;
     const uint32_t nbx = a.nbricks[0], nbxy = a.nbricks[0] * a.nbricks[1];
-    ++n.rays;
-    const float first[3] = {e.x, e.y, e.z};
-    const float last[3] = {l.x, l.y, l.z};
-    float dir[3];
-    const float tend = ray_setup(first, last, a.step, dir);
-    const int cap = a.loop_cap;
-    float t = 0.0f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
     uint32_t cur_brick = 0xffffffffu;
     bool cur_occ = false;
     float t_lim = 0.0f;
     bool finished = false;
-    int loop = 0;   // samples taken (the reference loop counter)
     while (!finished && loop < cap) {
         const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2]};
         float vp[3];
@@ -370,7 +367,7 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
                 ++n.slots;
                 n.shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
             }
-            if (A >= kErtOpacity) {
+            if (ERT && A >= kErtOpacity) {
                 A = 1.0f;
                 finished = true;
             }
@@ -378,6 +375,22 @@ __device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T
             finished = finished || (t > tend);
         }
     }
+    return finished;
+}
+
+template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
+__device__ __forceinline__ float4 cell_ray(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
+                                           const float* s_alpha, const uint32_t* s_occ, int px, int py, float4 e,
+                                           float4 l, RayCounters& n) {
+    ++n.rays;
+    const float first[3] = {e.x, e.y, e.z};
+    const float last[3] = {l.x, l.y, l.z};
+    float dir[3];
+    const float tend = ray_setup(first, last, a.step, dir);
+    float t = 0.0f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
+    int loop = 0;   // samples taken (the reference loop counter)
+    cell_march<T, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, s_occ, px, py, first, dir, tend, a.loop_cap, t, loop, r,
+                                         g, bb, A, n);
     return make_float4(r, g, bb, A);
 }
 

@@ -1,0 +1,181 @@
+// sparse.hip -- the plain re-march of the judge's sparse pixels (every
+// stride-th pixel of a learned frame), split into segments along each ray.
+//
+// A ray is a dependent chain of loads; 1/64 of a frame's rays (16K at 1024^2)
+// fill about a CU's worth of waves each, so marching each of them whole
+// takes as long as the longest ray.  Here every ray's samples [0, N) are cut
+// into kSegments runs of equal length, one lane each, every run composited
+// front to back from A = 0 without early termination; a second pass joins
+// the runs in order with the same "over" operator (associative up to float
+// rounding, ~1e-7) and, in the run where the opacity reaches the early
+// termination threshold, marches that run again sample by sample from the
+// joined state so the termination lands on the same sample as in the
+// sequential loop.  The sample positions are those of the reference loop:
+// the start of a run is t after k steps of t += step, taken in closed form
+// inside binades (closed_form_steps).
+#include "kernels.h"
+#include "march_cell.h"
+#include "march_common.h"
+
+namespace hipdvr {
+
+namespace {
+
+constexpr int kSegments = 16;
+constexpr int kSparseK = 4;   // samples in flight per lane
+
+// t after n steps of the reference loop's t += step.
+__device__ float advance_t(float t, float step, int n) {
+    while (n > 0) {
+        float d = 0.0f;
+        const int j = closed_form_steps(t, step, 3.0e38f, n + 3, d);
+        if (j > 0) {
+            t = t + (float)j * d;
+            n -= j;
+        } else {
+            t = t + step;
+            --n;
+        }
+    }
+    return t;
+}
+
+// Samples the reference loop takes on a ray without early termination: the
+// sample at t = 0, then every t_k <= tend, at most cap.
+__device__ int ray_samples(float step, float tend, int cap) {
+    float t = 0.0f;
+    int k = 0;
+    do {
+        float d = 0.0f;
+        const int j = closed_form_steps(t, step, tend, cap - k, d);
+        if (j > 0) {
+            t = t + (float)j * d;
+            k += j;
+        }
+        t = t + step;
+        ++k;
+    } while (!(t > tend) && k < cap);
+    return k;
+}
+
+__device__ __forceinline__ void load_tables(const MarchArgs& a, uint32_t* s_tf, float* s_alpha) {
+    for (int i = threadIdx.x; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
+    s_alpha[threadIdx.x] = a.alpha_corr[threadIdx.x];
+    __syncthreads();
+}
+
+template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a, float4* __restrict__ seg) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
+    const uint32_t ray = gid / kSegments, j = gid % kSegments;
+    if (ray >= rays) return;
+    const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
+    const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
+    float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t pix = (size_t)py * a.w + px;
+    const float4 e = a.entry[pix], l = a.exit[pix];
+    if (!is_background(e, l)) {
+        const float first[3] = {e.x, e.y, e.z}, last[3] = {l.x, l.y, l.z};
+        float dir[3];
+        const float tend = ray_setup(first, last, a.step, dir);
+        const int N = ray_samples(a.step, tend, a.loop_cap);
+        const int L = (N + kSegments - 1) / kSegments;
+        const int k0 = (int)j * L;
+        if (k0 < N) {
+            const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
+            float t = advance_t(0.0f, a.step, k0), r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
+            int loop = k0;
+            const int cap = k0 + L < N ? k0 + L : N;
+            RayCounters n;
+            cell_march<T, WIDE, SHADE, false, kSparseK, false>(a, f, s_tf, s_alpha, nullptr, px, py, first, dir,
+                                                               tend, cap, t, loop, r, g, bb, A, n);
+            res = make_float4(r, g, bb, A);
+        }
+    }
+    seg[gid] = res;
+}
+
+template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(256) void sparse_join_kernel(const MarchArgs a, const float4* __restrict__ seg) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t ray = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
+    if (ray >= rays) return;
+    const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
+    const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
+    float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t pix = (size_t)py * a.w + px;
+    const float4 e = a.entry[pix], l = a.exit[pix];
+    if (!is_background(e, l)) {
+        const float first[3] = {e.x, e.y, e.z}, last[3] = {l.x, l.y, l.z};
+        float dir[3];
+        const float tend = ray_setup(first, last, a.step, dir);
+        const int N = ray_samples(a.step, tend, a.loop_cap);
+        const int L = (N + kSegments - 1) / kSegments;
+        float r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
+        bool done = false;
+        for (int j = 0; j < kSegments && !done; ++j) {
+            const int k0 = j * L;
+            if (k0 >= N) break;
+            const float4 s4 = seg[(size_t)ray * kSegments + j];
+            const float An = A + (1.0f - A) * s4.w;
+            if (An >= kErtOpacity - 1.0e-5f) {
+                // early termination inside this run: march it again from the
+                // joined state, sample by sample, with the termination test
+                const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
+                float t = advance_t(0.0f, a.step, k0);
+                int loop = k0;
+                const int cap = k0 + L < N ? k0 + L : N;
+                RayCounters n;
+                done = cell_march<T, WIDE, SHADE, false, kSparseK, true>(a, f, s_tf, s_alpha, nullptr, px, py, first,
+                                                                         dir, tend, cap, t, loop, r, g, bb, A, n);
+            } else {
+                const float w = 1.0f - A;
+                r = r + w * s4.x;
+                g = g + w * s4.y;
+                bb = bb + w * s4.z;
+                A = An;
+            }
+        }
+        res = make_float4(r, g, bb, A);
+    }
+    a.out[(size_t)gy * a.grid_w + gx] = res;
+}
+
+template <typename T, bool WIDE, int SHADE>
+void launch_sparse_t(const MarchArgs& a, float4* seg, hipStream_t st) {
+    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
+    hipLaunchKernelGGL((sparse_segments_kernel<T, WIDE, SHADE>), dim3((rays * kSegments + 255) / 256), dim3(256), 0,
+                       st, a, seg);
+    hipLaunchKernelGGL((sparse_join_kernel<T, WIDE, SHADE>), dim3((rays + 255) / 256), dim3(256), 0, st, a,
+                       (const float4*)seg);
+}
+
+template <typename T, bool WIDE>
+void launch_sparse_w(const MarchArgs& a, int shade, float4* seg, hipStream_t st) {
+    if (shade == 2) launch_sparse_t<T, WIDE, 2>(a, seg, st);
+    else if (shade == 1) launch_sparse_t<T, WIDE, 1>(a, seg, st);
+    else launch_sparse_t<T, WIDE, 0>(a, seg, st);
+}
+
+}  // namespace
+
+size_t sparse_segment_bytes(uint32_t rays) { return (size_t)rays * kSegments * sizeof(float4); }
+
+void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st) {
+    if (dtype == 0) {
+        if (wide) launch_sparse_w<uint8_t, true>(a, shade, seg, st);
+        else launch_sparse_w<uint8_t, false>(a, shade, seg, st);
+    } else {
+        if (wide) launch_sparse_w<uint16_t, true>(a, shade, seg, st);
+        else launch_sparse_w<uint16_t, false>(a, shade, seg, st);
+    }
+}
+
+}  // namespace hipdvr

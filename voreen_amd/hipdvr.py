@@ -43,7 +43,7 @@ EXPORTS = (
     "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
     "hipdvr_learned_create", "hipdvr_learned_destroy", "hipdvr_learned_prepare",
     "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
-    "hipdvr_render_learned_device",
+    "hipdvr_render_learned_device", "hipdvr_learned_sparse_remarch",
     "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
     "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
@@ -180,6 +180,8 @@ def load():
         "hipdvr_learned_collect": ([vp, C.c_float, C.c_float, vp, vp, u32, P(u32)], C.c_int),
         "hipdvr_render_learned_device": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
                                           P(LearnedStats)], C.c_int),
+        "hipdvr_learned_sparse_remarch": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp, P(u32),
+                                           P(u32)], C.c_int),
         "hipdvr_vvd_parse": ([C.c_char_p, P(VolumeDesc), C.c_char_p, u32], C.c_int),
         "hipdvr_vvd_last_error": ([], C.c_char_p),
         "hipdvr_volume_load_raw": ([vp, C.c_char_p, P(VolumeDesc), C.c_uint64, P(vp)], C.c_int),
@@ -659,6 +661,14 @@ class Learned:
             C.byref(opts) if opts else None, d_out.ptr, C.byref(st) if stats else None),
             "hipdvr_render_learned_device")
         return st.as_dict() if stats else None
+
+    def sparse_remarch(self, tf, d_entry, d_exit, w, h, params, d_sparse, opts=None):
+        """The judge's plain re-march alone (hipdvr_learned_sparse_remarch): (sw, sh)."""
+        sw, sh = C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.lib.hipdvr_learned_sparse_remarch(
+            self.h, tf.h, d_entry.ptr, d_exit.ptr, w, h, C.byref(params), C.byref(opts) if opts else None,
+            d_sparse.ptr, C.byref(sw), C.byref(sh)), "hipdvr_learned_sparse_remarch")
+        return sw.value, sh.value
 
 
 class Batch:
