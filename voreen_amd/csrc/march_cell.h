@@ -241,9 +241,8 @@ constexpr uint32_t kMaxLdsOccWords = 1024;
 // global memory (batched launches).
 // The sample loop of one ray from state (t, loop, r, g, bb, A) up to sample
 // index cap (exclusive) or the ray's end; returns true when the ray is
-// finished (past tend, or terminated early when ERT).  ERT = false marches
-// through without early termination (a segment composited from A = 0).
-template <typename T, bool WIDE, int SHADE, bool COUNT, int K, bool ERT = true>
+// finished (past tend, or terminated early).
+template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T, WIDE>& f, const uint32_t* s_tf,
                                            const float* s_alpha, const uint32_t* s_occ, int px, int py,
                                            const float first[3], const float dir[3], float tend, int cap, float& t,
@@ -367,7 +366,7 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
                 ++n.slots;
                 n.shade_slots += (__ballot(shade_now) != 0ull) ? 1 : 0;
             }
-            if (ERT && A >= kErtOpacity) {
+            if (A >= kErtOpacity) {
                 A = 1.0f;
                 finished = true;
             }

@@ -5,12 +5,14 @@
 // fill about a CU's worth of waves each, so marching each of them whole
 // takes as long as the longest ray.  Here every ray's samples [0, N) are cut
 // into kSegments runs of equal length, one lane each, every run composited
-// front to back from A = 0 without early termination; a second pass joins
-// the runs in order with the same "over" operator (associative up to float
-// rounding, ~1e-7) and, in the run where the opacity reaches the early
-// termination threshold, marches that run again sample by sample from the
-// joined state so the termination lands on the same sample as in the
-// sequential loop.  The sample positions are those of the reference loop:
+// front to back from A = 0; a second pass joins the runs in order with the
+// same "over" operator (associative up to float rounding, ~1e-7) and, in the
+// run where the joined opacity reaches the early-termination threshold,
+// marches that run again sample by sample from the joined state, so the
+// termination lands on the same sample as in the sequential loop.  A run may
+// stop at its own early termination: its opacity then bounds the joined one
+// from below, so the crossing lies in it or before it and its stored result
+// is never joined.  The sample positions are those of the reference loop:
 // the start of a run is t after k steps of t += step, taken in closed form
 // inside binades (closed_form_steps).
 #include "kernels.h"
@@ -91,8 +93,8 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
             int loop = k0;
             const int cap = k0 + L < N ? k0 + L : N;
             RayCounters n;
-            cell_march<T, WIDE, SHADE, false, kSparseK, false>(a, f, s_tf, s_alpha, nullptr, px, py, first, dir,
-                                                               tend, cap, t, loop, r, g, bb, A, n);
+            cell_march<T, WIDE, SHADE, false, kSparseK>(a, f, s_tf, s_alpha, nullptr, px, py, first, dir, tend, cap,
+                                                        t, loop, r, g, bb, A, n);
             res = make_float4(r, g, bb, A);
         }
     }
@@ -119,29 +121,37 @@ __global__ __launch_bounds__(256) void sparse_join_kernel(const MarchArgs a, con
         const int N = ray_samples(a.step, tend, a.loop_cap);
         const int L = (N + kSegments - 1) / kSegments;
         float r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
-        bool done = false;
-        for (int j = 0; j < kSegments && !done; ++j) {
-            const int k0 = j * L;
-            if (k0 >= N) break;
-            const float4 s4 = seg[(size_t)ray * kSegments + j];
-            const float An = A + (1.0f - A) * s4.w;
-            if (An >= kErtOpacity - 1.0e-5f) {
-                // early termination inside this run: march it again from the
-                // joined state, sample by sample, with the termination test
-                const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
-                float t = advance_t(0.0f, a.step, k0);
-                int loop = k0;
-                const int cap = k0 + L < N ? k0 + L : N;
-                RayCounters n;
-                done = cell_march<T, WIDE, SHADE, false, kSparseK, true>(a, f, s_tf, s_alpha, nullptr, px, py, first,
-                                                                         dir, tend, cap, t, loop, r, g, bb, A, n);
-            } else {
+        int j = 0;
+        while (true) {
+            // join runs up to the one in which the opacity reaches the threshold
+            int jc = -1;
+            for (; j < kSegments; ++j) {
+                if (j * L >= N) break;
+                const float4 s4 = seg[(size_t)ray * kSegments + j];
+                const float An = A + (1.0f - A) * s4.w;
+                if (An >= kErtOpacity - 1.0e-5f) {
+                    jc = j;
+                    break;
+                }
                 const float w = 1.0f - A;
                 r = r + w * s4.x;
                 g = g + w * s4.y;
                 bb = bb + w * s4.z;
                 A = An;
             }
+            if (jc < 0) break;
+            // march that run again from the joined state, sample by sample,
+            // with the termination test (the lanes of a wave do this together)
+            const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
+            const int k0 = jc * L;
+            float t = advance_t(0.0f, a.step, k0);
+            int loop = k0;
+            const int cap = k0 + L < N ? k0 + L : N;
+            RayCounters n;
+            if (cell_march<T, WIDE, SHADE, false, kSparseK>(a, f, s_tf, s_alpha, nullptr, px, py, first, dir, tend,
+                                                            cap, t, loop, r, g, bb, A, n))
+                break;
+            j = jc + 1;   // not terminated after all (rounding of the joins): go on joining
         }
         res = make_float4(r, g, bb, A);
     }
