@@ -408,14 +408,26 @@ void launch_targets(const float* err, const unsigned int* visits, const float* f
 // brick is cut so the next brick starts on the base grid, and each sample's
 // opacity correction is the one of the step it stands for.
 // ---------------------------------------------------------------------------
+// Opacity of a step m * step (m = 1 .. 8) from the corrected opacity a1 of
+// one step: 1 - (1 - a1)^m, since (1 - a)^(m*e) = ((1 - a)^e)^m.  (A table
+// per multiple would cost 7 KiB of LDS per workgroup, a quarter of the
+// kernel's occupancy.)
+__device__ __forceinline__ float alpha_for_multiple(float a1, int m) {
+    if (m == 1) return a1;
+    const float q = 1.0f - a1, q2 = q * q, q4 = q2 * q2;
+    if (m >= 8) return 1.0f - q4 * q4;
+    const float qm = ((m & 1) ? q : 1.0f) * ((m & 2) ? q2 : 1.0f) * ((m & 4) ? q4 : 1.0f);
+    return 1.0f - qm;
+}
+
 template <typename T, bool WIDE, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     constexpr int K = 4;
     __shared__ uint32_t s_tf[kMaxTfWidth];
-    __shared__ float s_alpha[kLMaxMult][256];   // multipliers 1 .. 8
+    __shared__ float s_alpha[256];   // opacity correction of the base step
     const MarchArgs& a = la.m;
     for (int i = threadIdx.x; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
-    for (int k = 0; k < kLMaxMult; ++k) s_alpha[k][threadIdx.x] = la.alpha_m[k * 256 + threadIdx.x];
+    s_alpha[threadIdx.x] = la.alpha_m[threadIdx.x];
     __syncthreads();
     const uint32_t tiles_x = (a.w + 15) / 16;
     const uint32_t tiles_total = tiles_x * ((a.h + 15) / 16);
@@ -497,7 +509,7 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                     CellRecord<T>::unpack(rk[k], v);
                     const float inten = trilerp(v, pk[k]);
                     const uint32_t c8 = s_tf[tf_index(inten, a)];
-                    const float ca = s_alpha[mk[k] - 1][c8 >> 24];
+                    const float ca = alpha_for_multiple(s_alpha[c8 >> 24], mk[k]);
                     if (ca > 0.f) {
                         float col[3] = {byte_norm(c8 & 0xffu), byte_norm((c8 >> 8) & 0xffu),
                                         byte_norm((c8 >> 16) & 0xffu)};
