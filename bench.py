@@ -76,9 +76,21 @@ def measured_traffic(config, shading):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     """Time the oracle's CPURaycaster restatement (no shading) on a band of
-    rows at the image centre; scale by the plain sample count of a frame."""
+    rows at the image centre (the whole frame when it fits the time budget):
+    one warm-up, then the median of 3..9 timed runs within about target_s;
+    frames/s = measured samples/s over the plain samples of a frame."""
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
     h = en.shape[0]
@@ -90,18 +102,27 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
         t0 = time.perf_counter()
         _, n = O.render(vol, tf, (0.0, 1.0), band_e, band_x, step, nthreads=cores)
         dt = time.perf_counter() - t0
-        if dt * 2.5 >= target_s or rows >= h:
+        if dt * 8 >= target_s or rows >= h:
             break
-        rows = min(h, max(rows * 2, int(rows * target_s / max(dt, 1e-3) * 0.8)))
+        rows = min(h, max(rows * 2, int(rows * target_s / 8 / max(dt, 1e-3))))
+    runs = max(3, min(9, int(target_s / max(dt, 1e-3))))
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, n = O.render(vol, tf, (0.0, 1.0), band_e, band_x, step, nthreads=cores)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     sps = n / dt
     return {
         "value": round(sps / samples_per_frame, 5),
         "unit": "frames/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "gsamples_per_s": round(sps / 1e9, 5),
-        "sample": (f"rows {y0}..{y0 + rows - 1} of view 0 ({rows}x{en.shape[1]} rays, {n} samples, "
-                   f"{dt:.1f} s), no shading; frames/s = samples/s / plain samples per frame"),
+        "sample": (f"rows {y0}..{y0 + rows - 1} of view 0 ({rows}x{en.shape[1]} rays, {n} samples), no "
+                   f"shading; median of {runs} runs after one warm-up ({dt:.2f} s each); frames/s = "
+                   "samples/s / plain samples per frame"),
     }
 
 
@@ -523,6 +544,8 @@ def main():
     if traffic:
         out["roofline"]["traffic_source"] = traffic["source"]
         out["roofline"]["algorithmic_bytes_per_launch"] = round(bytes_orbit / nviews)
+        # reuse: algorithmic bytes per HBM byte (records served from L2 / Infinity Cache)
+        out["roofline"]["reuse"] = round(bytes_orbit / nviews / traffic["bytes"], 3)
     if args.copy_calibration:
         nb = args.copy_calibration << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
