@@ -7,6 +7,10 @@
  * reference's evaluation order with no fused multiply-add.
  *
  * How each function is pinned (tests/test_oracle.py unless noted):
+ *   dvr_ray_setup,      bit for bit against tests/golden/tgt_ray_samples.txt:
+ *   dvr_sample_voxel    tend, direction, the t recurrence, the texture-to-voxel
+ *                       position, its clamp and rounding, as the reference's
+ *                       own tgt computes them (oracle/tgt_probe.cpp, 96 rays).
  *   dvr_sample_linear   affine fields are reproduced exactly at any position;
  *                       integer positions return the voxel; clamp at faces;
  *                       brute-force FP64 trilinear on random 2^3..5^3 volumes.
@@ -455,6 +459,40 @@ static inline void tex_to_voxel(const dvr_volume* v, const float s[3], float out
         float vi = ((float)v->dims[i] * s[i]) + (-0.5f);
         out[i] = sminf(smaxf(vi, 0.0f), (float)v->dims[i] - 1.0f);
     }
+}
+
+/* Ray parameters of CPURaycaster::directRendering, cpuraycaster.cpp:219-230:
+ * direction = last - first; tend = tgt::length(direction) = sqrt(lengthSq)
+ * with lengthSq = ((0 + x*x) + y*y) + z*z, direction *= 1/length
+ * (tgt::normalize, vector.h:802-820); a zero direction between non-zero
+ * points gives tend = step/2 (one sample). */
+float dvr_ray_setup(const float first[3], const float last[3], float step, float dir[3]) {
+    for (int i = 0; i < 3; ++i) dir[i] = last[i] - first[i];
+    int dir_zero = (dir[0] == 0.f && dir[1] == 0.f && dir[2] == 0.f);
+    int last_nz = !(last[0] == 0.f && last[1] == 0.f && last[2] == 0.f);
+    int first_nz = !(first[0] == 0.f && first[1] == 0.f && first[2] == 0.f);
+    if (dir_zero && last_nz && first_nz) return step / 2.0f;
+    float lsq = 0.0f;
+    lsq += dir[0] * dir[0];
+    lsq += dir[1] * dir[1];
+    lsq += dir[2] * dir[2];
+    float tend = sqrtf(lsq);
+    float rl = 1.0f / sqrtf(lsq);
+    dir[0] *= rl; dir[1] *= rl; dir[2] *= rl;
+    return tend;
+}
+
+/* Sample k of a ray: t after k steps of t += step (the loop of
+ * cpuraycaster.cpp:239-311), sample = first + t*dir (:240), its clamped voxel
+ * position (:243) and, for the nearest filter, tgt::iround of it (:247). */
+float dvr_sample_voxel(const dvr_volume* v, const float first[3], const float dir[3], float step, int32_t k,
+                       float vp[3], int32_t ivp[3]) {
+    float t = 0.0f;
+    for (int32_t i = 0; i < k; ++i) t += step;
+    float s[3] = { first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2] };
+    tex_to_voxel(v, s, vp);
+    for (int i = 0; i < 3; ++i) ivp[i] = iroundf_tgt(vp[i]);
+    return t;
 }
 
 static inline float reconstruct(const dvr_volume* v, int filter, const float s[3]) {
@@ -939,24 +977,8 @@ static int64_t march_f(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
                        const float first[3], const float last[3],
                        const float* entry, int w, int h, int px, int py, float res[4]) {
     const float step = P->step;
-    float tend, t = 0.0f;
-    float dir[3] = { last[0] - first[0], last[1] - first[1], last[2] - first[2] };
-    int dir_zero = (dir[0] == 0.f && dir[1] == 0.f && dir[2] == 0.f);
-    int last_nz = !(last[0] == 0.f && last[1] == 0.f && last[2] == 0.f);
-    int first_nz = !(first[0] == 0.f && first[1] == 0.f && first[2] == 0.f);
-    if (dir_zero && last_nz && first_nz) {
-        tend = step / 2.0f;
-    } else {
-        /* tgt::length = sqrt(lengthSq) with lengthSq = ((0+x*x)+y*y)+z*z;
-         * tgt::normalize multiplies by 1/length (vector.h:802-820) */
-        float lsq = 0.0f;
-        lsq += dir[0] * dir[0];
-        lsq += dir[1] * dir[1];
-        lsq += dir[2] * dir[2];
-        tend = sqrtf(lsq);
-        float rl = 1.0f / sqrtf(lsq);
-        dir[0] *= rl; dir[1] *= rl; dir[2] *= rl;
-    }
+    float t = 0.0f, dir[3];
+    const float tend = dvr_ray_setup(first, last, step, dir);
     const int W1 = tf->width - 1;
     float r = 0.f, g = 0.f, b = 0.f, A = 0.f;
     float last_int = 0.f;
@@ -1041,22 +1063,8 @@ static int64_t march_d(const dvr_volume* v, const dvr_tf* tf, const dvr_params* 
                        const float first[3], const float last[3],
                        const float* entry, int w, int h, int px, int py, double res[4]) {
     const float step = P->step;
-    float tend, t = 0.0f;
-    float dir[3] = { last[0] - first[0], last[1] - first[1], last[2] - first[2] };
-    int dir_zero = (dir[0] == 0.f && dir[1] == 0.f && dir[2] == 0.f);
-    int last_nz = !(last[0] == 0.f && last[1] == 0.f && last[2] == 0.f);
-    int first_nz = !(first[0] == 0.f && first[1] == 0.f && first[2] == 0.f);
-    if (dir_zero && last_nz && first_nz) {
-        tend = step / 2.0f;
-    } else {
-        float lsq = 0.0f;
-        lsq += dir[0] * dir[0];
-        lsq += dir[1] * dir[1];
-        lsq += dir[2] * dir[2];
-        tend = sqrtf(lsq);
-        float rl = 1.0f / sqrtf(lsq);
-        dir[0] *= rl; dir[1] *= rl; dir[2] *= rl;
-    }
+    float t = 0.0f, dir[3];
+    const float tend = dvr_ray_setup(first, last, step, dir);
     if (P->compositing != DVR_COMP_DVR) {
         /* the FP64 envelope is defined for DVR only; other modes are the FP32 result */
         float rf[4];

@@ -152,6 +152,11 @@ int64_t dvr_fp64_render(const dvr_volume* vol, const dvr_tf* tf, const dvr_param
 /* one shading evaluation (params' model, light, camera, shininess) of TF
  * colour c with gradient g at physical position pos (for unit tests) */
 void dvr_shade(const dvr_params* prm, const float g[3], const float pos[3], const float c[3], float out[3]);
+/* Ray setup of directRendering: normalized direction, returns tend. */
+float dvr_ray_setup(const float first[3], const float last[3], float step, float dir[3]);
+/* Sample k of a ray: returns t; vp = clamped voxel position, ivp = its tgt::iround. */
+float dvr_sample_voxel(const dvr_volume* vol, const float first[3], const float dir[3], float step, int32_t k,
+                       float vp[3], int32_t ivp[3]);
 /* CALC_GRADIENT of prm->gradient at texture-space position s (FP32). */
 void dvr_gradient(const dvr_volume* vol, const dvr_params* prm, const float s[3], float g[3]);
 

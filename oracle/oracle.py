@@ -94,6 +94,11 @@ def _load():
     lib.dvr_shade.restype = None
     lib.dvr_gradient.argtypes = [P(Volume), P(Params), C.c_void_p, C.c_void_p]
     lib.dvr_gradient.restype = None
+    lib.dvr_ray_setup.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    lib.dvr_ray_setup.restype = C.c_float
+    lib.dvr_sample_voxel.argtypes = [P(Volume), C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_void_p,
+                                     C.c_void_p]
+    lib.dvr_sample_voxel.restype = C.c_float
     lib.dvr_sample_linear.argtypes = [P(Volume), C.c_float, C.c_float, C.c_float]
     lib.dvr_sample_linear.restype = C.c_float
     lib.dvr_mlp_forward.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
@@ -278,6 +283,27 @@ def gradient(vol, s, mode, filter_=FILTER_LINEAR, rwm=(1.0, 0.0), spacing=(1.0, 
     lib().dvr_gradient(C.byref(v), C.byref(p), _ptr(s), _ptr(out))
     del keep
     return out
+
+
+def ray_setup(first, last, step):
+    """(tend, normalized direction) of the oracle's ray setup."""
+    f = np.ascontiguousarray(first, np.float32)
+    l = np.ascontiguousarray(last, np.float32)
+    d = np.zeros(3, np.float32)
+    tend = lib().dvr_ray_setup(_ptr(f), _ptr(l), float(step), _ptr(d))
+    return np.float32(tend), d
+
+
+def sample_voxel(dims, first, direction, step, k):
+    """(t, clamped voxel position, its rounding) of sample k (dims = x, y, z)."""
+    v = Volume()
+    v.dims[:] = list(dims)
+    f = np.ascontiguousarray(first, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    vp = np.zeros(3, np.float32)
+    iv = np.zeros(3, np.int32)
+    t = lib().dvr_sample_voxel(C.byref(v), _ptr(f), _ptr(d), float(step), int(k), _ptr(vp), _ptr(iv))
+    return np.float32(t), vp, iv
 
 
 def sample_linear(vol, pos):

@@ -651,3 +651,54 @@ def test_tf2d_single_samples(oracle, filter_, gdtype, channels):
         assert np.allclose(img[0, i], want, atol=2e-6), (i, img[0, i], want)
         checked += 1
     assert checked > 300
+
+
+# --------------------------------------------------------------------------
+# ray setup and sample positions against the reference's own tgt code
+# --------------------------------------------------------------------------
+
+GOLDEN_TGT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tgt_ray_samples.txt")
+
+
+def _parse_tgt_golden(text):
+    rays = []
+    for line in text.splitlines():
+        w = line.split()
+        if w[0] == "ray":
+            hx = lambda s: np.float32(float.fromhex(s))  # noqa: E731
+            rays.append(dict(dims=(int(w[3]), int(w[4]), int(w[5])), step=hx(w[7]),
+                             first=[hx(x) for x in w[9:12]], last=[hx(x) for x in w[13:16]], tend=hx(w[17]),
+                             dir=[hx(x) for x in w[19:22]], samples=[]))
+        elif w[0] == "k":
+            rays[-1]["samples"].append((int(w[1]), np.float32(float.fromhex(w[3])),
+                                        [np.float32(float.fromhex(x)) for x in w[5:8]],
+                                        [int(x) for x in w[9:12]]))
+    return rays
+
+
+def test_ray_setup_and_sample_positions_equal_tgt(oracle):
+    """tests/golden/tgt_ray_samples.txt comes from oracle/tgt_probe.cpp built
+    against the reference's header-only tgt: the oracle's tend, direction,
+    t recurrence, texture-to-voxel position and rounding equal it bit for bit."""
+    rays = _parse_tgt_golden(open(GOLDEN_TGT).read())
+    assert len(rays) == 96
+    checked = 0
+    for r in rays:
+        tend, d = oracle.ray_setup(r["first"], r["last"], r["step"])
+        assert tend.tobytes() == r["tend"].tobytes()
+        assert d.tobytes() == np.array(r["dir"], np.float32).tobytes()
+        for k, t, vp, iv in r["samples"]:
+            tt, ovp, oiv = oracle.sample_voxel(r["dims"], r["first"], d, r["step"], k)
+            assert tt.tobytes() == t.tobytes(), (k, tt, t)
+            assert ovp.tobytes() == np.array(vp, np.float32).tobytes(), (k, ovp, vp)
+            assert list(oiv) == iv
+            checked += 1
+    assert checked > 800
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(GOLDEN_TGT), "..", "..", "oracle", "_ref",
+                                                    "tgt_probe")), reason="oracle/_ref not built here")
+def test_tgt_fixture_matches_the_probe():
+    import subprocess
+    probe = os.path.join(os.path.dirname(GOLDEN_TGT), "..", "..", "oracle", "_ref", "tgt_probe")
+    assert subprocess.run([probe], check=True, capture_output=True, text=True).stdout == open(GOLDEN_TGT).read()
