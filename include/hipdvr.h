@@ -212,7 +212,8 @@ typedef struct hipdvr_camera {
     float eye[3];           /* world == physical space (identity transformation) */
     float focus[3];
     float up[3];
-    float fovy_deg;
+    float fovy_deg;         /* tgt::Camera (ext/tgt/camera.cpp:203-214): clamped to [6, 175], spans the
+                             * image height when w >= h, else its width (frustum ratio 1) */
     float near_dist;
 } hipdvr_camera;
 

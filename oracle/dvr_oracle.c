@@ -274,14 +274,20 @@ static void raygen_setup(const dvr_camera* cam, const float llf[3], const float 
     u[0] = r[1] * f[2] - r[2] * f[1];
     u[1] = r[2] * f[0] - r[0] * f[2];
     u[2] = r[0] * f[1] - r[1] * f[0];
-    double th = tan((double)cam->fovy_deg * 0.5 * 3.14159265358979323846 / 180.0);
+    /* tgt::Camera::getProjectionMatrix (camera.cpp:203-214), frustum ratio
+     * 1: fovy clamped to [6, 175] degrees spans the image height when the
+     * window is at least as wide as high, else its width */
+    double fovy = cam->fovy_deg < 6.0f ? 6.0 : (cam->fovy_deg > 175.0f ? 175.0 : (double)cam->fovy_deg);
+    double th = tan(fovy * 0.5 * 3.14159265358979323846 / 180.0);
     double aspect = (double)w / (double)h;
+    double th_u = aspect >= 1.0 ? th : th / aspect;   /* half height on the unit view distance */
+    double th_r = th_u * aspect;                      /* half width */
     for (int i = 0; i < 3; ++i) {
         double inv = 1.0 / ((double)urb[i] - (double)llf[i]);
         g->o[i] = (float)(((double)cam->eye[i] - (double)llf[i]) * inv);
         g->f[i] = (float)(f[i] * inv);
-        g->r[i] = (float)(r[i] * th * aspect * inv);
-        g->u[i] = (float)(u[i] * th * inv);
+        g->r[i] = (float)(r[i] * th_r * inv);
+        g->u[i] = (float)(u[i] * th_u * inv);
     }
     g->near_t = cam->near_dist;
 }

@@ -11,10 +11,14 @@
 // t += step, sample = first + t * direction, the texture-to-voxel matrix of
 // VolumeBase::getTextureToVoxelMatrix (volumebase.cpp:221-224: translation by
 // -0.5 times scale by the dimensions) applied as mat4 * vec4, tgt::clamp to
-// [0, dim - 1] and tgt::iround.  Floats are printed as C99 hex literals.
+// [0, dim - 1] and tgt::iround.  For a set of cameras and window sizes it
+// prints the projection * view matrix tgt::Camera gives MeshEntryExitPoints
+// (meshentryexitpoints.cpp:210-214: getProjectionMatrix(port size),
+// getViewMatrix()).  Floats are printed as C99 hex literals.
 #include <cstdint>
 #include <cstdio>
 
+#include "tgt/camera.h"
 #include "tgt/matrix.h"
 #include "tgt/vector.h"
 
@@ -77,6 +81,38 @@ int main() {
             t += step;
             if (t > tend) break;
         }
+    }
+
+    struct Cam {
+        tgt::vec3 eye, focus, up;
+        float fovy;
+    };
+    const Cam cams[] = {
+        // modules/base/test/render/cpuraycaster.vws:96-103 (up not orthogonal to the view)
+        {tgt::vec3(1.76227701f, 1.01016998f, 1.77568305f), tgt::vec3(-0.023203f, 0.019314f, 0.012643f),
+         tgt::vec3(-0.74486113f, 0.22378904f, 0.62857014f), 45.0f},
+        {tgt::vec3(0.9f, 1.4f, -3.1f), tgt::vec3(0.05f, -0.1f, 0.0f), tgt::vec3(0.0f, 1.0f, 0.0f), 30.0f},
+        // inside the volume: entry points on the near plane
+        {tgt::vec3(0.5f, 0.45f, 0.55f), tgt::vec3(0.5f, 0.5f, 2.0f), tgt::vec3(0.0f, 1.0f, 0.0f), 50.0f},
+        // fovy outside [6, 175] degrees: clamped by getProjectionMatrix
+        {tgt::vec3(2.0f, 0.3f, 1.5f), tgt::vec3(0.0f), tgt::vec3(0.0f, 0.0f, 1.0f), 3.0f},
+        {tgt::vec3(0.5f, 0.45f, 1.15f), tgt::vec3(0.45f, 0.5f, 0.0f), tgt::vec3(0.0f, 1.0f, 0.0f), 179.0f},
+    };
+    const int sizes[][2] = {{200, 200}, {97, 61}, {61, 97}, {320, 180}};
+    int ci = 0;
+    for (const Cam& c : cams) {
+        for (const auto& wh : sizes) {
+            const float pnear = 0.1f, pfar = 50.0f;
+            tgt::Camera cam(c.eye, c.focus, c.up, c.fovy, 1.0f, pnear, pfar);
+            const tgt::mat4 pv = cam.getProjectionMatrix(tgt::ivec2(wh[0], wh[1])) * cam.getViewMatrix();
+            std::printf("cam %d eye %a %a %a focus %a %a %a up %a %a %a fovy %a near %a w %d h %d pv", ci, c.eye.x,
+                        c.eye.y, c.eye.z, c.focus.x, c.focus.y, c.focus.z, c.up.x, c.up.y, c.up.z, c.fovy, pnear,
+                        wh[0], wh[1]);
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) std::printf(" %a", pv[i][j]);
+            std::printf("\n");
+        }
+        ++ci;
     }
     return 0;
 }

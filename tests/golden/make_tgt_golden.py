@@ -1,8 +1,9 @@
 """Writes tests/golden/tgt_ray_samples.txt: the output of oracle/_ref/tgt_probe,
 the harness that evaluates CPURaycaster's ray setup, sample recurrence,
 texture-to-voxel transform, clamp and rounding with the reference's own
-header-only tgt (built by `make -C oracle ref` where the reference tree is
-present).  Run from the repository root: python tests/golden/make_tgt_golden.py"""
+header-only tgt, and the projection * view matrices of tgt::Camera for a set
+of cameras and window sizes (built by `make -C oracle ref` from the reference's
+tgt sources where the reference tree is present).  Run from the repository root: python tests/golden/make_tgt_golden.py"""
 import os
 import subprocess
 

@@ -702,3 +702,49 @@ def test_tgt_fixture_matches_the_probe():
     import subprocess
     probe = os.path.join(os.path.dirname(GOLDEN_TGT), "..", "..", "oracle", "_ref", "tgt_probe")
     assert subprocess.run([probe], check=True, capture_output=True, text=True).stdout == open(GOLDEN_TGT).read()
+
+
+def _parse_tgt_cameras(text):
+    cams = []
+    for line in text.splitlines():
+        w = line.split()
+        if w[0] != "cam":
+            continue
+        hx = lambda s: float.fromhex(s)  # noqa: E731
+        cams.append(dict(eye=[hx(x) for x in w[3:6]], focus=[hx(x) for x in w[7:10]], up=[hx(x) for x in w[11:14]],
+                         fovy=hx(w[15]), near=hx(w[17]), w=int(w[19]), h=int(w[21]),
+                         pv=np.array([hx(x) for x in w[23:39]], np.float64).reshape(4, 4)))
+    return cams
+
+
+def test_entry_exit_points_project_to_pixel_centres_under_tgt_camera(oracle):
+    """MeshEntryExitPoints rasterizes the proxy geometry with tgt::Camera's
+    projection * view matrix (meshentryexitpoints.cpp:210-214).  The fixture
+    holds that matrix from the reference's own camera.cpp for cameras with a
+    non-orthogonal up vector, the eye inside the volume, fovy outside
+    [6, 175] degrees, and landscape, square and portrait windows: every entry
+    and exit point of the oracle projects to its pixel's centre, and an entry
+    point clipped by the near plane lies at depth -1."""
+    cams = _parse_tgt_cameras(open(GOLDEN_TGT).read())
+    assert len(cams) == 20
+    portrait = clipped = 0
+    for c in cams:
+        w, h = c["w"], c["h"]
+        en, ex = oracle.entry_exit(c["eye"], c["focus"], c["up"], c["fovy"], c["near"], w, h)
+        hit = en[..., 3] > 0
+        assert hit.sum() >= 8, c   # fovy 175: the volume covers a few pixels
+        ys, xs = np.nonzero(hit)
+        for pts in (en, ex):
+            p = np.concatenate([pts[ys, xs, :3].astype(np.float64), np.ones((len(xs), 1))], axis=1)
+            clip = p @ c["pv"].T
+            ndc = clip[:, :3] / clip[:, 3:4]
+            px = (ndc[:, 0] + 1.0) * 0.5 * w - 0.5
+            py = (ndc[:, 1] + 1.0) * 0.5 * h - 0.5
+            assert np.abs(px - xs).max() < 2e-3, (c["w"], c["h"], c["fovy"], np.abs(px - xs).max())
+            assert np.abs(py - ys).max() < 2e-3, (c["w"], c["h"], c["fovy"], np.abs(py - ys).max())
+            if pts is en:
+                on_near = np.abs(ndc[:, 2] + 1.0) < 1e-4
+                clipped += int(on_near.sum())
+        portrait += h > w
+    assert portrait == 5
+    assert clipped > 1000   # the camera inside the volume
