@@ -92,20 +92,20 @@ def _round_f32(x):
     return np.float32(best[1])
 
 
-@pytest.mark.parametrize("d", [255, 65535])
-def test_voxel_normalisation_residual_step_is_exact(d):
-    # The kernels compute v/d as q = v*r; e = fma(-q, d, v); q' = fma(e, r, q)
-    # (march_common.h VoxNorm).  Check it equals the correctly rounded v/d for
-    # every voxel value, with exact rational arithmetic for the fused steps.
+@pytest.mark.parametrize("d,r_hi,r_lo", [(255, "0x1.010102p-8", "-0x1.fdfdfep-33"),
+                                          (65535, "0x1.0001p-16", "0x1.0001p-48")])
+def test_voxel_normalisation_is_exact(d, r_hi, r_lo):
+    # The kernels compute v/d as fma(fv, r_hi, fv * r_lo) (march_common.h
+    # VoxNorm).  Check it equals the correctly rounded v/d for every stored
+    # value, with exact rational arithmetic for the fused step.
     Fr = fractions.Fraction
-    r = np.float32(1.0) / np.float32(d)
-    fr_r = Fr(float(r))
-    vals = range(0, d + 1) if d == 255 else list(range(0, 4096)) + list(range(4096, d + 1, 7)) + [d]
-    for v in vals:
-        q = np.float32(np.float32(v) * r)
-        e = _round_f32(Fr(v) - Fr(float(q)) * d)
-        q2 = _round_f32(Fr(float(e)) * fr_r + Fr(float(q)))
-        assert q2 == _round_f32(Fr(v, d)), v
+    hi, lo = np.float32(float.fromhex(r_hi)), np.float32(float.fromhex(r_lo))
+    assert abs(Fr(float(hi)) + Fr(float(lo)) - Fr(1, d)) < Fr(1, d) * Fr(1, 2 ** 46)
+    fr_hi = Fr(float(hi))
+    for v in range(0, d + 1):
+        t = np.float32(np.float32(v) * lo)            # the inner product, rounded
+        q = _round_f32(Fr(v) * fr_hi + Fr(float(t)))  # the fma: one rounding
+        assert q == _round_f32(Fr(v, d)), v
 
 
 def test_synth_is_deterministic_and_pinned():

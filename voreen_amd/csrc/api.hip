@@ -144,6 +144,10 @@ void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf
     }
     a.shininess = p->shininess;
     a.rwm_scale = v.rwm_scale;
+    {
+        const float range = v.dtype == HIPDVR_U8 ? 255.0f : (v.dtype == HIPDVR_U16 ? 65535.0f : 1.0f);
+        for (int i = 0; i < 3; ++i) a.grad_raw[i] = (a.grad_scale[i] * a.rwm_scale) / range;
+    }
     a.rwm_offset = v.rwm_offset;
     a.dom_lo = tf->domain_lo;
     a.dom_hi = tf->domain_hi;

@@ -482,6 +482,7 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                 float tk[K], pk[K][3];
                 int mk[K];
                 rec_t rk[K];
+                typename CellFetch<T, WIDE>::off_t ik[K];
                 bool valid[K];
                 float tt = t;
 #pragma unroll
@@ -496,7 +497,8 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                     mk[k] = m;
                     if (valid[k]) {
                         const float sk[3] = {first[0] + tt * dir[0], first[1] + tt * dir[1], first[2] + tt * dir[2]};
-                        rk[k] = f.load(cell_of<T, WIDE>(f, a, sk, pk[k]));
+                        ik[k] = cell_of<T, WIDE>(f, a, sk, pk[k]);
+                        rk[k] = f.load(ik[k]);
                     }
                     tt = tt + (float)m * a.step;
                 }
@@ -519,8 +521,8 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
                                                  first[2] + t0 * dir[2]};
                             float gr[3];
                             if (t0 == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                            else if (!cell_gradient_interior<T, WIDE>(f, a, ss, v, pk[k], gr))
-                                cell_gradient<T, WIDE>(f, a, ss, gr);
+                            else if (!cell_gradient_interior<T, WIDE>(f, a, ss, rk[k], ik[k], pk[k], gr))
+                                cell_gradient<T, WIDE, true>(f, a, ss, gr);
                             float pos[3];
                             for (int i = 0; i < 3; ++i)
                                 pos[i] = (((ss[i] * a.fdims[i]) - 0.5f) * a.spacing[i]) + a.offset[i];

@@ -33,6 +33,17 @@ struct CellRecord;
 template <>
 struct CellRecord<uint16_t> {
     typedef uint4 type;
+    // stored values as floats (no normalization)
+    static __device__ __forceinline__ void raw(const uint4& r, float v[8]) {
+        v[0] = (float)(r.x & 0xffffu);
+        v[1] = (float)(r.x >> 16);
+        v[2] = (float)(r.y & 0xffffu);
+        v[3] = (float)(r.y >> 16);
+        v[4] = (float)(r.z & 0xffffu);
+        v[5] = (float)(r.z >> 16);
+        v[6] = (float)(r.w & 0xffffu);
+        v[7] = (float)(r.w >> 16);
+    }
     static __device__ __forceinline__ void unpack(const uint4& r, float v[8]) {
         v[0] = VoxNorm<uint16_t>::apply(r.x & 0xffffu);
         v[1] = VoxNorm<uint16_t>::apply(r.x >> 16);
@@ -47,6 +58,17 @@ struct CellRecord<uint16_t> {
 template <>
 struct CellRecord<uint8_t> {
     typedef uint2 type;
+    // stored values as floats (no normalization)
+    static __device__ __forceinline__ void raw(const uint2& r, float v[8]) {
+        v[0] = (float)(r.x & 0xffu);
+        v[1] = (float)((r.x >> 8) & 0xffu);
+        v[2] = (float)((r.x >> 16) & 0xffu);
+        v[3] = (float)(r.x >> 24);
+        v[4] = (float)(r.y & 0xffu);
+        v[5] = (float)((r.y >> 8) & 0xffu);
+        v[6] = (float)((r.y >> 16) & 0xffu);
+        v[7] = (float)(r.y >> 24);
+    }
     static __device__ __forceinline__ void unpack(const uint2& r, float v[8]) {
         v[0] = VoxNorm<uint8_t>::apply(r.x & 0xffu);
         v[1] = VoxNorm<uint8_t>::apply((r.x >> 8) & 0xffu);
@@ -136,8 +158,11 @@ __device__ __forceinline__ int closed_form_steps(float t, float step, float boun
 }
 
 // Central differences of six trilinear lookups at s +- one voxel per axis,
-// as march_ref.hip computes them; the six loads are issued together.
-template <typename T, bool WIDE>
+// as march_ref.hip computes them; the six loads are issued together.  RAW
+// (kernels whose shading is checked by tolerance): the lookups interpolate
+// the stored values and the difference is scaled once by grad_raw, equal up
+// to float rounding.
+template <typename T, bool WIDE, bool RAW = false>
 __device__ __forceinline__ void cell_gradient(const CellFetch<T, WIDE>& f, const MarchArgs& a,
                                               const float s[3], float gr[3]) {
     typename CellFetch<T, WIDE>::rec_t rec[6];
@@ -156,70 +181,93 @@ __device__ __forceinline__ void cell_gradient(const CellFetch<T, WIDE>& f, const
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         float v[8];
-        CellRecord<T>::unpack(rec[k], v);
+        if (RAW) CellRecord<T>::raw(rec[k], v);
+        else CellRecord<T>::unpack(rec[k], v);
         vals[k] = trilerp(v, p[k]);
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+    for (int i = 0; i < 3; ++i) {
+        if (RAW) gr[i] = (vals[3 + i] - vals[i]) * a.grad_raw[i];
+        else gr[i] = ((vals[3 + i] - vals[i]) * a.grad_scale[i]) * a.rwm_scale;
+    }
 }
 
 // Interior central differences from the cell records: when every axis has
 // u_i in [1, dim_i - 3) and a proper fraction, the six lookups at +-1 voxel
 // have the sample's own weights p, and their footprints are the sample's
 // corner planes plus the outer planes lo-1 and lo+2 (records of the cells
-// lo -+ e_i).  The same arithmetic as the bricked kernel's interior path;
-// equal to the six-lookup form up to float rounding.  Returns false (and
-// leaves gr) outside the interior.
+// lo -+ e_i).  Equal to the six-lookup form up to float rounding (for the
+// kernels whose shading is checked by tolerance): the differences are taken
+// in stored units and scaled once by grad_raw = grad_scale * rwm_scale /
+// range, the four-term sums use fused multiply-adds, and the neighbour
+// records are found from the sample's own record index `own` (its low three
+// bits are the parities of lo; a step across a 2x2x2 group moves by 8, sy or
+// sz).  Returns false (and leaves gr) outside the interior.
 template <typename T, bool WIDE>
 __device__ __forceinline__ bool cell_gradient_interior(const CellFetch<T, WIDE>& f, const MarchArgs& a,
-                                                       const float s[3], const float v[8], const float p[3],
+                                                       const float s[3],
+                                                       const typename CellFetch<T, WIDE>::rec_t& rec,
+                                                       typename CellFetch<T, WIDE>::off_t own, const float p[3],
                                                        float gr[3]) {
-    uint32_t lo[3];
+    typedef typename CellFetch<T, WIDE>::off_t off_t;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float u = (a.fdims[i] * s[i]) + (-0.5f);
         if (!(u >= 1.0f) || !(u + 3.0f < a.fdims[i]) || !(p[i] > 0.0f)) return false;
-        lo[i] = (uint32_t)u;
     }
-    typename CellFetch<T, WIDE>::rec_t rec[6];
+    const uint32_t par = (uint32_t)own & 7u;
+    const off_t sy2 = f.sy - 2, sz4 = f.sz - 4;
+    const off_t nb[6] = {
+        (par & 1u) ? own - 1 : own - 7,            // x - 1
+        (par & 2u) ? own - 2 : own - sy2,          // y - 1
+        (par & 4u) ? own - 4 : own - sz4,          // z - 1
+        (par & 1u) ? own + 7 : own + 1,            // x + 1
+        (par & 2u) ? own + sy2 : own + 2,          // y + 1
+        (par & 4u) ? own + sz4 : own + 4,          // z + 1
+    };
+    typename CellFetch<T, WIDE>::rec_t nr[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        uint32_t c[3] = {lo[0], lo[1], lo[2]};
-        c[k % 3] = k < 3 ? c[k % 3] - 1 : c[k % 3] + 1;
-        rec[k] = f.load(f.index(c));
-    }
-    float m[3][8], q[3][8];
+    for (int k = 0; k < 6; ++k) nr[k] = f.load(nb[k]);
+    float v[8], m[3][8], q[3][8];
+    CellRecord<T>::raw(rec, v);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        CellRecord<T>::unpack(rec[i], m[i]);
-        CellRecord<T>::unpack(rec[3 + i], q[i]);
+        CellRecord<T>::raw(nr[i], m[i]);
+        CellRecord<T>::raw(nr[3 + i], q[i]);
     }
     const float px = p[0], py = p[1], pz = p[2];
     const float qx = 1.f - px, qy = 1.f - py, qz = 1.f - pz;
+    auto dot4 = [](float a0, float a1, float a2, float a3, const float w[4]) {
+        return __builtin_fmaf(a3, w[3], __builtin_fmaf(a2, w[2], __builtin_fmaf(a1, w[1], a0 * w[0])));
+    };
+    // (qa*bm + pa*b0) - (qa*b1 + pa*bp): the lookup at -1 voxel minus the one at +1
+    auto diff = [](float qa, float pa, float bm, float b0, float b1, float bp) {
+        return __builtin_fmaf(-pa, bp, __builtin_fmaf(-qa, b1, __builtin_fmaf(pa, b0, qa * bm)));
+    };
     // corner order: 0 x0y0z0, 1 x1y0z0, 2 x1y1z0, 3 x0y1z0, 4 x0y0z1, 5 x1y0z1, 6 x1y1z1, 7 x0y1z1
     {
         const float w[4] = {qy * qz, py * qz, qy * pz, py * pz};
-        const float bm = m[0][0] * w[0] + m[0][3] * w[1] + m[0][4] * w[2] + m[0][7] * w[3];
-        const float b0 = v[0] * w[0] + v[3] * w[1] + v[4] * w[2] + v[7] * w[3];
-        const float b1 = v[1] * w[0] + v[2] * w[1] + v[5] * w[2] + v[6] * w[3];
-        const float bp = q[0][1] * w[0] + q[0][2] * w[1] + q[0][5] * w[2] + q[0][6] * w[3];
-        gr[0] = (((qx * bm + px * b0) - (qx * b1 + px * bp)) * a.grad_scale[0]) * a.rwm_scale;
+        const float bm = dot4(m[0][0], m[0][3], m[0][4], m[0][7], w);
+        const float b0 = dot4(v[0], v[3], v[4], v[7], w);
+        const float b1 = dot4(v[1], v[2], v[5], v[6], w);
+        const float bp = dot4(q[0][1], q[0][2], q[0][5], q[0][6], w);
+        gr[0] = diff(qx, px, bm, b0, b1, bp) * a.grad_raw[0];
     }
     {
         const float w[4] = {qx * qz, px * qz, qx * pz, px * pz};
-        const float bm = m[1][0] * w[0] + m[1][1] * w[1] + m[1][4] * w[2] + m[1][5] * w[3];
-        const float b0 = v[0] * w[0] + v[1] * w[1] + v[4] * w[2] + v[5] * w[3];
-        const float b1 = v[3] * w[0] + v[2] * w[1] + v[7] * w[2] + v[6] * w[3];
-        const float bp = q[1][3] * w[0] + q[1][2] * w[1] + q[1][7] * w[2] + q[1][6] * w[3];
-        gr[1] = (((qy * bm + py * b0) - (qy * b1 + py * bp)) * a.grad_scale[1]) * a.rwm_scale;
+        const float bm = dot4(m[1][0], m[1][1], m[1][4], m[1][5], w);
+        const float b0 = dot4(v[0], v[1], v[4], v[5], w);
+        const float b1 = dot4(v[3], v[2], v[7], v[6], w);
+        const float bp = dot4(q[1][3], q[1][2], q[1][7], q[1][6], w);
+        gr[1] = diff(qy, py, bm, b0, b1, bp) * a.grad_raw[1];
     }
     {
         const float w[4] = {qx * qy, px * qy, qx * py, px * py};
-        const float bm = m[2][0] * w[0] + m[2][1] * w[1] + m[2][3] * w[2] + m[2][2] * w[3];
-        const float b0 = v[0] * w[0] + v[1] * w[1] + v[3] * w[2] + v[2] * w[3];
-        const float b1 = v[4] * w[0] + v[5] * w[1] + v[7] * w[2] + v[6] * w[3];
-        const float bp = q[2][4] * w[0] + q[2][5] * w[1] + q[2][7] * w[2] + q[2][6] * w[3];
-        gr[2] = (((qz * bm + pz * b0) - (qz * b1 + pz * bp)) * a.grad_scale[2]) * a.rwm_scale;
+        const float bm = dot4(m[2][0], m[2][1], m[2][3], m[2][2], w);
+        const float b0 = dot4(v[0], v[1], v[3], v[2], w);
+        const float b1 = dot4(v[4], v[5], v[7], v[6], w);
+        const float bp = dot4(q[2][4], q[2][5], q[2][7], q[2][6], w);
+        gr[2] = diff(qz, pz, bm, b0, b1, bp) * a.grad_raw[2];
     }
     return true;
 }
@@ -301,6 +349,7 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
         // all loads issued before the first is composited
         float tk[K], pk[K][3];
         rec_t rk[K];
+        typename CellFetch<T, WIDE>::off_t ik[K];
         bool valid[K];
         {
             float tt = t;
@@ -319,7 +368,8 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
                         sk[1] = first[1] + tt * dir[1];
                         sk[2] = first[2] + tt * dir[2];
                     }
-                    rk[k] = f.load(cell_of<T, WIDE>(f, a, sk, pk[k]));
+                    ik[k] = cell_of<T, WIDE>(f, a, sk, pk[k]);
+                    rk[k] = f.load(ik[k]);
                 }
                 tt = tt + a.step;
             }
@@ -346,8 +396,9 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
                                          first[2] + tt * dir[2]};
                     float gr[3];
                     if (tt == 0.0f) clip_border_gradient(a, px, py, ss, dir, gr);
-                    else if (SHADE == 2 || !cell_gradient_interior<T, WIDE>(f, a, ss, v, pk[k], gr))
-                        cell_gradient<T, WIDE>(f, a, ss, gr);
+                    else if (SHADE == 2) cell_gradient<T, WIDE>(f, a, ss, gr);
+                    else if (!cell_gradient_interior<T, WIDE>(f, a, ss, rk[k], ik[k], pk[k], gr))
+                        cell_gradient<T, WIDE, true>(f, a, ss, gr);
                     float pos[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i)

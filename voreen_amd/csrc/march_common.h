@@ -63,6 +63,7 @@ struct MarchArgs {
     float spacing[3], offset[3];
     float grad_off[3];          // 1/dims (datasetDimensionsRCP_)
     float grad_scale[3];        // (1/spacing) * 0.5
+    float grad_raw[3];          // grad_scale * rwm_scale / (255 or 65535): differences of stored values
     float spacing_rcp[3];       // 1/spacing (forward differences)
     int32_t gradient;           // kGrad*
     float light_pos[3], light_amb[3], light_dif[3], light_spe[3], shininess;
@@ -82,26 +83,22 @@ struct MarchArgs {
     float alpha_corr[256];      // 1 - pow(1 - a/255, step*200) per alpha byte
 };
 
-// v / 255.f and v / 65535.f, correctly rounded, as q = v*r corrected by one
-// fused residual step; exact for every 8- and 16-bit voxel value (checked
-// exhaustively on the host by the test suite).
+// value / 255.f or value / 65535.f correctly rounded (the reference's
+// getTypeAsFloat), as fma(fv, r_hi, fv * r_lo) with r_hi + r_lo = 1/range to
+// about 48 bits: the quotient of an integer by the range never lies within
+// that error of a rounding midpoint, so the single rounding of the fma gives
+// the correctly rounded quotient (checked for every stored value).
 template <typename T> struct VoxNorm;
 template <> struct VoxNorm<uint8_t> {
     static __device__ __forceinline__ float apply(uint32_t v) {
-        const float d = 255.0f, r = 1.0f / 255.0f;
-        float fv = (float)v;
-        float q = fv * r;
-        float e = __builtin_fmaf(-q, d, fv);
-        return __builtin_fmaf(e, r, q);
+        const float fv = (float)v;
+        return __builtin_fmaf(fv, 0x1.010102p-8f, fv * -0x1.fdfdfep-33f);
     }
 };
 template <> struct VoxNorm<uint16_t> {
     static __device__ __forceinline__ float apply(uint32_t v) {
-        const float d = 65535.0f, r = 1.0f / 65535.0f;
-        float fv = (float)v;
-        float q = fv * r;
-        float e = __builtin_fmaf(-q, d, fv);
-        return __builtin_fmaf(e, r, q);
+        const float fv = (float)v;
+        return __builtin_fmaf(fv, 0x1.0001p-16f, fv * 0x1.0001p-48f);
     }
 };
 
@@ -385,36 +382,31 @@ __device__ __forceinline__ void shade_model(const MarchArgs& a, const float g[3]
     c[2] = out[2];
 }
 
-// The same Phong terms with hardware reciprocal, reciprocal square root and
-// exp2/log2 (about 1 ulp each): for kernels whose shading is checked against
-// the FP64 oracle by tolerance.  A zero gradient gives N = 0; N.H = 0 gives a
+// The same Phong terms with hardware reciprocal square root and exp2/log2
+// (about 1 ulp each) and fused multiply-adds: for kernels whose shading is
+// checked against the FP64 oracle by tolerance.  A zero gradient gives N = 0; N.H = 0 gives a
 // zero specular term.
 __device__ __forceinline__ void phong_fast(const MarchArgs& a, const float g[3], const float pos[3],
                                            float c[3]) {
-    const float gl = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    const float gl = __builtin_fmaf(g[2], g[2], __builtin_fmaf(g[1], g[1], g[0] * g[0]));
     const float rn = gl > 0.f ? __builtin_amdgcn_rsqf(gl) : 0.f;
     float L[3], V[3], H[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) { L[i] = a.light_pos[i] - pos[i]; V[i] = a.cam_pos[i] - pos[i]; }
-    const float rv = __builtin_amdgcn_rsqf(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
-    const float rl = __builtin_amdgcn_rsqf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+    const float rv = __builtin_amdgcn_rsqf(__builtin_fmaf(V[2], V[2], __builtin_fmaf(V[1], V[1], V[0] * V[0])));
+    const float rl = __builtin_amdgcn_rsqf(__builtin_fmaf(L[2], L[2], __builtin_fmaf(L[1], L[1], L[0] * L[0])));
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        V[i] *= rv;
         L[i] *= rl;
-        H[i] = V[i] + L[i];
+        H[i] = __builtin_fmaf(V[i], rv, L[i]);   // normalize(V) + normalize(L)
     }
-    const float rh = __builtin_amdgcn_rsqf(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
-    const float ndl = fmaxf((g[0] * L[0] + g[1] * L[1] + g[2] * L[2]) * rn, 0.f);
-    const float ndh = fmaxf((g[0] * H[0] + g[1] * H[1] + g[2] * H[2]) * (rn * rh), 0.f);
+    const float rh = __builtin_amdgcn_rsqf(__builtin_fmaf(H[2], H[2], __builtin_fmaf(H[1], H[1], H[0] * H[0])));
+    const float ndl = fmaxf(__builtin_fmaf(g[2], L[2], __builtin_fmaf(g[1], L[1], g[0] * L[0])) * rn, 0.f);
+    const float ndh = fmaxf(__builtin_fmaf(g[2], H[2], __builtin_fmaf(g[1], H[1], g[0] * H[0])) * (rn * rh), 0.f);
     const float sp = ndh > 0.f ? __builtin_amdgcn_exp2f(a.shininess * __builtin_amdgcn_logf(ndh)) : 0.f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const float amb = c[i] * a.light_amb[i];
-        const float dif = (c[i] * a.light_dif[i]) * ndl;
-        const float spe = a.light_spe[i] * sp;
-        c[i] = (amb + dif) + spe;
-    }
+    for (int i = 0; i < 3; ++i)   // ka = kd = c: c * (amb + dif * N.L) + spe * sp
+        c[i] = __builtin_fmaf(c[i], __builtin_fmaf(a.light_dif[i], ndl, a.light_amb[i]), a.light_spe[i] * sp);
 }
 
 // fixClipBorderGradient, mod_gradients.frag:39-46 (neighbours clamp to edge).
