@@ -14,7 +14,7 @@ baseline (the FP32 CPURaycaster restatement, no shading, on the host's cores,
 on a bounded band of rows of view 0).
 
     python bench.py [--steps K] [--warmup W] [--config c1|c0|c2|c4] [--kernel auto|reference|bricked|cell]
-                    [--batch 2|4|8]
+                    [--batch 1|2|4|8]
 
 `--config c2` and `--config c4` are the batched workloads (64 uneven
 volumes; the 8 GiB stress volume with 8 4K views), each printing its own line.
@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
     ap.add_argument("--brick-log2", type=int, default=4, choices=[2, 3, 4, 5],
                     help="empty-space brick size of the volume (2^n voxels)")
-    ap.add_argument("--batch", type=int, default=0, choices=[0, 2, 4, 8],
+    ap.add_argument("--batch", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="cell kernel: samples fetched together per ray (0 = library default)")
     ap.add_argument("--views", type=int, default=0, help="override the number of orbit views")
     ap.add_argument("--shading", default="default", choices=["default", "none", "phong"],

@@ -179,8 +179,8 @@ typedef struct hipdvr_params {
     float camera_pos[3];    /* physical space */
     int32_t loop_cap;       /* 0 = 255*255 (cpuraycaster.cpp:239) */
     int32_t kernel;         /* HIPDVR_KERNEL_* */
-    int32_t batch;          /* cell kernel: samples fetched together per ray (2, 4, 8;
-                               0 = default) */
+    int32_t batch;          /* cell kernel: samples fetched together per ray (1, 2 or 4,
+                               8 runs as 4; 0 = 2 shaded, 4 unshaded) */
     float gamma;            /* MIDA gammaValue in [-1, 1] (volumeraycaster.cpp:67) */
     float iso_value;        /* ISO isoValue in [0, 1] (volumeraycaster.cpp:57) */
     int32_t exact_shading;  /* Phong in the cell kernel: 1 = bit-identical to the reference

@@ -441,7 +441,7 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             t = ctx.transfunc(tf)
             outs = {}
             runs = [("ref", H.KERNEL_REFERENCE, 0, 0), ("brick", H.KERNEL_BRICKED, 0, 0)]
-            runs += [(f"cell{b}", H.KERNEL_CELL, b, 1) for b in (2, 4, 8)]
+            runs += [(f"cell{b}", H.KERNEL_CELL, b, 1) for b in (1, 2, 4, 8)]
             runs += [("cellfast", H.KERNEL_CELL, 4, 0)]
             for name, k, batch, exact in runs:
                 for shading in (0, 1):

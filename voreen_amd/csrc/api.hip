@@ -109,8 +109,8 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
     if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
-    if (p->batch != 0 && p->batch != 2 && p->batch != 4 && p->batch != 8)
-        fail(HIPDVR_ERR_INVALID, "batch must be 0, 2, 4 or 8");
+    if (p->batch != 0 && p->batch != 1 && p->batch != 2 && p->batch != 4 && p->batch != 8)
+        fail(HIPDVR_ERR_INVALID, "batch must be 0, 1, 2, 4 or 8");
 }
 
 void fill_march_args(MarchArgs& a, const hipdvr_volume* vol, const hipdvr_tf* tf,
@@ -274,7 +274,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         // exact-shading path, whose six full lookups hold more registers
         const int shade = phong ? ((p->exact_shading || p->shading != HIPDVR_SHADE_PHONG) ? 2 : 1) : 0;
         launch_march_cell(a, vol->desc.dtype, shade, count, mvol->cells_wide,
-                          p->batch ? p->batch : (shade == 2 ? 2 : kDefaultBatch), ctx->stream);
+                          p->batch ? p->batch : default_batch(shade), ctx->stream);
 
     } else {
         // buffer descriptors address at most 2^32 - 1 bytes

@@ -30,7 +30,10 @@ void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bo
 // ray (2 or 4)
 void launch_march_cell(const MarchArgs& a, int dtype, int shade, bool count, bool wide, int batch,
                        hipStream_t st);
-constexpr int kDefaultBatch = 4;
+// Samples in flight per ray by default: four unshaded; two shaded (a shaded
+// batch holds six neighbour records per sample, and two keep the Phong
+// kernel within 128 VGPRs, four waves per SIMD).
+constexpr int default_batch(int shade) { return shade ? 2 : 4; }
 
 // Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);

@@ -553,7 +553,7 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         MarchArgs fb;
         fill_march_args(fb, vol, tf, prm, d_entry, d_exit, d_out, w, h, nullptr);
         fb.cond_flag = l->d_flag;
-        launch_march_cell(fb, vol->desc.dtype, shade, false, vol->cells_wide, kDefaultBatch, st);
+        launch_march_cell(fb, vol->desc.dtype, shade, false, vol->cells_wide, default_batch(shade), st);
         hip_check(hipGetLastError(), "learned render launch");
         if (timed) ctx_timing_stop(l->ctx);
         if (count) {
