@@ -430,15 +430,16 @@ __global__ __launch_bounds__(256) void learned_kernel(const LearnedArgs la) {
     s_alpha[threadIdx.x] = la.alpha_m[threadIdx.x];
     __syncthreads();
     const uint32_t tiles_x = (a.w + 15) / 16;
-    const uint32_t tiles_total = tiles_x * ((a.h + 15) / 16);
-    const uint32_t tile = blockIdx.x;   // row-major tiles in dispatch order (see march_cell.hip)
+    const uint32_t tiles_y = (a.h + 15) / 16;
+    uint32_t tile_x, tile_y;   // XCD-compact tile order (xcd_tile)
+    const bool has_tile = xcd_tile(blockIdx.x, tiles_x, tiles_y, tile_x, tile_y);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int px = (int)tile_x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)tile_y * 16 + (wave >> 1) * 8 + (lane >> 3);
     typedef typename CellFetch<T, WIDE>::rec_t rec_t;
     const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
     unsigned long long n_samples = 0, n_shaded = 0;
-    if (tile < tiles_total && px < a.w && py < a.h) {
+    if (has_tile && px < a.w && py < a.h) {
         const size_t pix = (size_t)py * a.w + px;
         const float4 e = a.entry[pix], l = a.exit[pix];
         float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -564,8 +565,7 @@ static void launch_learned_t(const LearnedArgs& la, bool phong, bool count, dim3
 }
 
 void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st) {
-    const uint32_t tiles = ((la.m.w + 15) / 16) * ((la.m.h + 15) / 16);
-    const dim3 grid(tiles);
+    const dim3 grid(xcd_tile_grid((la.m.w + 15) / 16, (la.m.h + 15) / 16));
     if (la.u16) {
         if (wide) launch_learned_t<uint16_t, true>(la, phong, count, grid, st);
         else launch_learned_t<uint16_t, false>(la, phong, count, grid, st);

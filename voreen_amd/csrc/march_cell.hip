@@ -1,6 +1,6 @@
 // march_cell.hip -- per-frame launches of the cell-record ray march (the
 // device code and its parity argument are in march_cell.h): one workgroup per
-// 16x16 pixel tile, tiles in dispatch order.
+// 16x16 pixel tile, tiles in XCD-compact order.
 #include "kernels.h"
 #include "march_cell.h"
 #include "march_common.h"
@@ -9,15 +9,11 @@ namespace hipdvr {
 
 template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, const uint32_t tiles_x,
-                                                         const uint32_t tiles_total) {
-    // Row-major tiles in dispatch order: consecutive workgroups (spread over
-    // the 8 XCDs by the hardware) march neighbouring tiles, so the rays in
-    // flight on the whole chip cover one compact image region and share the
-    // volume records through the chip-wide Infinity Cache.  Giving each XCD
-    // its own band of the image put 8 distant regions in flight and ran 1.44x
-    // slower on the 512^3 / 1024^2 workload (1.8x at 3840x2160).
-    const uint32_t tile = blockIdx.x;
-    if (tile >= tiles_total) return;   // whole workgroup leaves together
+                                                         const uint32_t tiles_y) {
+    // XCD-compact tile order (xcd_tile): each XCD marches 2x2 blocks of
+    // neighbouring tiles, the chip one compact image region
+    uint32_t tile_x, tile_y;
+    if (!xcd_tile(blockIdx.x, tiles_x, tiles_y, tile_x, tile_y)) return;   // whole workgroup leaves together
     if (a.cond_flag && *a.cond_flag == 0) return;
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[256];
@@ -31,8 +27,8 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
-    const int gx = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int gy = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int gx = (int)tile_x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int gy = (int)tile_y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
     const bool inside = (gx < a.grid_w) && (gy < a.grid_h) && (px < a.w) && (py < a.h);
     const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
@@ -59,8 +55,7 @@ __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, cons
 template <typename T, bool WIDE, int SHADE, int K>
 static void launch_k(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
-    const uint32_t total = tx * ty;
-    const dim3 grid(total);
+    const dim3 grid(xcd_tile_grid(tx, ty));
     size_t shm = a.occ_words <= kMaxLdsOccWords ? (size_t)a.occ_words * 4 : 0;
     if (SHADE == 0) {
         // Unshaded marches run five workgroups per CU: the dynamic LDS is
@@ -72,9 +67,9 @@ static void launch_k(const MarchArgs& a, bool count, hipStream_t st) {
         if (shm < kUnshadedLds - kStaticLds) shm = kUnshadedLds - kStaticLds;
     }
     if (count)
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), shm, st, a, tx, total);
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), shm, st, a, tx, ty);
     else
-        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, false, K>), grid, dim3(256), shm, st, a, tx, total);
+        hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, false, K>), grid, dim3(256), shm, st, a, tx, ty);
 }
 
 template <typename T, bool WIDE, int SHADE>

@@ -500,4 +500,35 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
+// Image tiles (16x16 pixels, one workgroup each) in XCD-compact order.  The
+// hardware hands consecutive workgroups to the 8 XCDs in turn; here the
+// workgroups of one XCD (b % 8) march a 2x2 block of neighbouring tiles, so
+// rays that share volume records share that XCD's L2, while the 8 XCDs walk
+// neighbouring blocks in row-major order and the rays in flight on the chip
+// cover one compact image region (which shares the Infinity Cache).  Measured
+// on 512^3 / 1024^2: plain dispatch order 1.90 ms (Phong) / 0.94 ms
+// (unshaded), 2x2 blocks 1.80 / 0.89 ms, 4x4 1.80 / 0.88 ms, 8x8 2.84 ms;
+// one contiguous image band per XCD 2.74 ms.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdBlock = 2;
+
+// Workgroups to launch for tiles_x x tiles_y tiles (blocks padded to whole
+// rounds of kXcds; the extra workgroups exit at once).
+__host__ __device__ inline uint32_t xcd_tile_grid(uint32_t tiles_x, uint32_t tiles_y) {
+    const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock, by = (tiles_y + kXcdBlock - 1) / kXcdBlock;
+    return kXcds * kXcdBlock * kXcdBlock * ((bx * by + kXcds - 1) / kXcds);
+}
+
+// Tile (tx, ty) of workgroup b; false for a workgroup with no tile.
+__device__ __forceinline__ bool xcd_tile(uint32_t b, uint32_t tiles_x, uint32_t tiles_y, uint32_t& tx,
+                                         uint32_t& ty) {
+    constexpr uint32_t S = kXcdBlock * kXcdBlock;
+    const uint32_t xcd = b % kXcds, slot = b / kXcds;
+    const uint32_t block = (slot / S) * kXcds + xcd, j = slot % S;
+    const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock;
+    tx = (block % bx) * kXcdBlock + j % kXcdBlock;
+    ty = (block / bx) * kXcdBlock + j / kXcdBlock;
+    return tx < tiles_x && ty < tiles_y;
+}
+
 }  // namespace hipdvr

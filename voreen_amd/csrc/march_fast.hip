@@ -12,8 +12,9 @@
 //     maps to a zero-alpha TF entry, so it is not reconstructed; inside such a
 //     brick the ray fast-forwards with t += step (the reference's float
 //     sequence) up to an analytic bound of the brick exit;
-//   * tiles run in dispatch order, so the rays in flight on all XCDs cover one
-//     compact image region and share the bricks in the Infinity Cache;
+//   * tiles run in XCD-compact order (xcd_tile), so the rays in flight on
+//     all XCDs cover one compact image region and share the bricks in the
+//     Infinity Cache, and each XCD's rays share its L2;
 //   * Phong's central differences (mod_gradients.frag:202-214: six trilinear
 //     lookups at +-1 voxel) are evaluated from per-axis plane bilinears that
 //     share the sample's weights: the same clamped stencil positions and
@@ -212,10 +213,10 @@ __device__ __forceinline__ bool gradient_interior(const TiledFetch<T, WIDE>& f, 
 
 template <typename T, bool WIDE, int FILTER, bool PHONG, bool COUNT>
 __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, const uint32_t tiles_x,
-                                                         const uint32_t tiles_total) {
-    // row-major tiles in dispatch order (see march_cell.hip)
-    const uint32_t tile = blockIdx.x;
-    if (tile >= tiles_total) return;   // whole workgroup leaves together
+                                                         const uint32_t tiles_y) {
+    // XCD-compact tile order (xcd_tile)
+    uint32_t tile_x, tile_y;
+    if (!xcd_tile(blockIdx.x, tiles_x, tiles_y, tile_x, tile_y)) return;   // whole workgroup leaves together
     if (a.cond_flag && *a.cond_flag == 0) return;
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[256];
@@ -225,8 +226,8 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
-    const int gx = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int gy = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int gx = (int)tile_x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int gy = (int)tile_y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
     const bool inside = (gx < a.grid_w) && (gy < a.grid_h) && (px < a.w) && (py < a.h);
 
@@ -402,14 +403,13 @@ __global__ __launch_bounds__(256) void march_fast_kernel(const MarchArgs a, cons
 template <typename T, bool WIDE, int FILTER, bool PHONG>
 static void launch_t(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
-    const uint32_t total = tx * ty;
-    const dim3 grid(total);
+    const dim3 grid(xcd_tile_grid(tx, ty));
     if (count)
         hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, true>), grid, dim3(256), 0, st,
-                           a, tx, total);
+                           a, tx, ty);
     else
         hipLaunchKernelGGL((march_fast_kernel<T, WIDE, FILTER, PHONG, false>), grid, dim3(256), 0, st,
-                           a, tx, total);
+                           a, tx, ty);
 }
 
 template <typename T, bool WIDE>
