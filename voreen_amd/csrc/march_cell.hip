@@ -10,8 +10,8 @@ namespace hipdvr {
 template <typename T, bool WIDE, int SHADE, bool COUNT, int K>
 __global__ __launch_bounds__(256) void march_cell_kernel(const MarchArgs a, const uint32_t tiles_x,
                                                          const uint32_t tiles_y) {
-    // XCD-compact tile order (xcd_tile): each XCD marches 2x2 blocks of
-    // neighbouring tiles, the chip one compact image region
+    // XCD-compact tile order (xcd_tile): each XCD marches one compact region
+    // of tiles, the chip one compact image area
     uint32_t tile_x, tile_y;
     if (!xcd_tile(blockIdx.x, tiles_x, tiles_y, tile_x, tile_y)) return;   // whole workgroup leaves together
     if (a.cond_flag && *a.cond_flag == 0) return;

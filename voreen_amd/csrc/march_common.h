@@ -501,33 +501,39 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
 }
 
 // Image tiles (16x16 pixels, one workgroup each) in XCD-compact order.  The
-// hardware hands consecutive workgroups to the 8 XCDs in turn; here the
-// workgroups of one XCD (b % 8) march a 2x2 block of neighbouring tiles, so
-// rays that share volume records share that XCD's L2, while the 8 XCDs walk
-// neighbouring blocks in row-major order and the rays in flight on the chip
-// cover one compact image region (which shares the Infinity Cache).  Measured
-// on 512^3 / 1024^2: plain dispatch order 1.90 ms (Phong) / 0.94 ms
-// (unshaded), 2x2 blocks 1.80 / 0.89 ms, 4x4 1.80 / 0.88 ms, 8x8 2.84 ms;
-// one contiguous image band per XCD 2.74 ms.
+// hardware hands consecutive workgroups to the 8 XCDs in turn (workgroup b
+// to XCD b % 8).  Here the image is cut into chip regions of 4 x 2 XCD
+// regions, each XCD region kXcdRegionW x kXcdRegionH tiles; the i-th
+// workgroup of XCD x marches tile i of XCD region x of chip region
+// i / (W*H), chip regions in row-major order.  So the rays one XCD has in
+// flight lie in one compact region and share its L2, and the 8 XCDs march
+// neighbouring regions, so the rays in flight on the chip cover one compact
+// image area and share the Infinity Cache.  Measured on 512^3 / 1024^2 with
+// Phong: plain dispatch order 1.90 ms; each XCD on 2x2-tile blocks taken in
+// turn 1.79 ms; XCD regions of 4x4 tiles 1.74 ms, 8x4 1.73 ms, 8x8 1.79
+// ms, 16x8 (chip regions as wide as the image) 2.74 ms; one contiguous
+// image band per XCD 2.74 ms.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kXcdBlock = 2;
+constexpr uint32_t kXcdRegionW = 8, kXcdRegionH = 4;
+constexpr uint32_t kChipRegionW = 4 * kXcdRegionW, kChipRegionH = 2 * kXcdRegionH;
 
-// Workgroups to launch for tiles_x x tiles_y tiles (blocks padded to whole
-// rounds of kXcds; the extra workgroups exit at once).
+// Workgroups to launch for tiles_x x tiles_y tiles (whole chip regions; the
+// workgroups of tiles outside the image exit at once).
 __host__ __device__ inline uint32_t xcd_tile_grid(uint32_t tiles_x, uint32_t tiles_y) {
-    const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock, by = (tiles_y + kXcdBlock - 1) / kXcdBlock;
-    return kXcds * kXcdBlock * kXcdBlock * ((bx * by + kXcds - 1) / kXcds);
+    const uint32_t cx = (tiles_x + kChipRegionW - 1) / kChipRegionW;
+    const uint32_t cy = (tiles_y + kChipRegionH - 1) / kChipRegionH;
+    return kXcds * kXcdRegionW * kXcdRegionH * cx * cy;
 }
 
 // Tile (tx, ty) of workgroup b; false for a workgroup with no tile.
 __device__ __forceinline__ bool xcd_tile(uint32_t b, uint32_t tiles_x, uint32_t tiles_y, uint32_t& tx,
                                          uint32_t& ty) {
-    constexpr uint32_t S = kXcdBlock * kXcdBlock;
+    constexpr uint32_t R = kXcdRegionW * kXcdRegionH;
     const uint32_t xcd = b % kXcds, slot = b / kXcds;
-    const uint32_t block = (slot / S) * kXcds + xcd, j = slot % S;
-    const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock;
-    tx = (block % bx) * kXcdBlock + j % kXcdBlock;
-    ty = (block / bx) * kXcdBlock + j / kXcdBlock;
+    const uint32_t chip = slot / R, j = slot % R;
+    const uint32_t cx_n = (tiles_x + kChipRegionW - 1) / kChipRegionW;
+    tx = (chip % cx_n) * kChipRegionW + (xcd % 4) * kXcdRegionW + j % kXcdRegionW;
+    ty = (chip / cx_n) * kChipRegionH + (xcd / 4) * kXcdRegionH + j / kXcdRegionW;
     return tx < tiles_x && ty < tiles_y;
 }
 
