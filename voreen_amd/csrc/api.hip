@@ -269,9 +269,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
     } else if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
     } else if (kernel == HIPDVR_KERNEL_CELL) {
-        // samples in flight per ray, measured on the 512^3 workload: four
-        // (450 against 440 frames/s with Phong, equal without); two for the
-        // exact-shading path, whose six full lookups hold more registers
+        // samples in flight per ray: default_batch (kernels.h)
         const int shade = phong ? ((p->exact_shading || p->shading != HIPDVR_SHADE_PHONG) ? 2 : 1) : 0;
         launch_march_cell(a, vol->desc.dtype, shade, count, mvol->cells_wide,
                           p->batch ? p->batch : default_batch(shade), ctx->stream);
