@@ -452,7 +452,7 @@ def test_bricked_equals_reference_and_skips(ctx, oracle):
             t.close()
             for shading in (0, 1):
                 a, sa = outs[("ref", shading)]
-                for name in ("cell2", "cell4", "cell8"):
+                for name in ("cell1", "cell2", "cell4", "cell8"):
                     c, sc = outs[(name, shading)]
                     assert np.array_equal(a, c), f"{name} tf{seed} shading {shading}"
                     assert sa["samples"] == sc["samples"] and sa["shaded"] == sc["shaded"]
