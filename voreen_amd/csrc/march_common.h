@@ -512,28 +512,50 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
 // Phong: plain dispatch order 1.90 ms; each XCD on 2x2-tile blocks taken in
 // turn 1.79 ms; XCD regions of 4x4 tiles 1.74 ms, 8x4 1.73 ms, 8x8 1.79
 // ms, 16x8 (chip regions as wide as the image) 2.74 ms; one contiguous
-// image band per XCD 2.74 ms.
+// image band per XCD 2.74 ms.  On 128^3 / 256^2 (16 x 16 tiles) the 2x2
+// blocks keep all XCDs busy (50,055 frames/s against 46,139 with regions).
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kXcdRegionW = 8, kXcdRegionH = 4;
 constexpr uint32_t kChipRegionW = 4 * kXcdRegionW, kChipRegionH = 2 * kXcdRegionH;
+// Images smaller than one chip region would leave XCDs without tiles; there
+// each XCD takes 2x2-tile blocks in turn instead (block b % 8 of every
+// round of 8 blocks, blocks row-major).
+constexpr uint32_t kXcdBlock = 2;
 
-// Workgroups to launch for tiles_x x tiles_y tiles (whole chip regions; the
-// workgroups of tiles outside the image exit at once).
+__host__ __device__ inline bool xcd_regions(uint32_t tiles_x, uint32_t tiles_y) {
+    return tiles_x >= kChipRegionW && tiles_y >= kChipRegionH;
+}
+
+// Workgroups to launch for tiles_x x tiles_y tiles (whole chip regions or
+// whole rounds of blocks; the workgroups of tiles outside the image exit at
+// once).
 __host__ __device__ inline uint32_t xcd_tile_grid(uint32_t tiles_x, uint32_t tiles_y) {
-    const uint32_t cx = (tiles_x + kChipRegionW - 1) / kChipRegionW;
-    const uint32_t cy = (tiles_y + kChipRegionH - 1) / kChipRegionH;
-    return kXcds * kXcdRegionW * kXcdRegionH * cx * cy;
+    if (xcd_regions(tiles_x, tiles_y)) {
+        const uint32_t cx = (tiles_x + kChipRegionW - 1) / kChipRegionW;
+        const uint32_t cy = (tiles_y + kChipRegionH - 1) / kChipRegionH;
+        return kXcds * kXcdRegionW * kXcdRegionH * cx * cy;
+    }
+    const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock, by = (tiles_y + kXcdBlock - 1) / kXcdBlock;
+    return kXcds * kXcdBlock * kXcdBlock * ((bx * by + kXcds - 1) / kXcds);
 }
 
 // Tile (tx, ty) of workgroup b; false for a workgroup with no tile.
 __device__ __forceinline__ bool xcd_tile(uint32_t b, uint32_t tiles_x, uint32_t tiles_y, uint32_t& tx,
                                          uint32_t& ty) {
-    constexpr uint32_t R = kXcdRegionW * kXcdRegionH;
     const uint32_t xcd = b % kXcds, slot = b / kXcds;
-    const uint32_t chip = slot / R, j = slot % R;
-    const uint32_t cx_n = (tiles_x + kChipRegionW - 1) / kChipRegionW;
-    tx = (chip % cx_n) * kChipRegionW + (xcd % 4) * kXcdRegionW + j % kXcdRegionW;
-    ty = (chip / cx_n) * kChipRegionH + (xcd / 4) * kXcdRegionH + j / kXcdRegionW;
+    if (xcd_regions(tiles_x, tiles_y)) {
+        constexpr uint32_t R = kXcdRegionW * kXcdRegionH;
+        const uint32_t chip = slot / R, j = slot % R;
+        const uint32_t cx_n = (tiles_x + kChipRegionW - 1) / kChipRegionW;
+        tx = (chip % cx_n) * kChipRegionW + (xcd % 4) * kXcdRegionW + j % kXcdRegionW;
+        ty = (chip / cx_n) * kChipRegionH + (xcd / 4) * kXcdRegionH + j / kXcdRegionW;
+    } else {
+        constexpr uint32_t S = kXcdBlock * kXcdBlock;
+        const uint32_t block = (slot / S) * kXcds + xcd, j = slot % S;
+        const uint32_t bx = (tiles_x + kXcdBlock - 1) / kXcdBlock;
+        tx = (block % bx) * kXcdBlock + j % kXcdBlock;
+        ty = (block / bx) * kXcdBlock + j / kXcdBlock;
+    }
     return tx < tiles_x && ty < tiles_y;
 }
 
