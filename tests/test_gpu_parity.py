@@ -121,6 +121,45 @@ def test_entry_exit_device_matches_oracle(ctx, oracle):
         assert 0.0 < np.mean(ge[..., 3] > 0) <= 1.0
 
 
+@pytest.mark.parametrize("size", [(512, 128), (560, 136), (1040, 272), (500, 500), (250, 90)])
+def test_tile_orders_write_every_pixel(ctx, oracle, size):
+    """Frames of at least one chip region (32 x 8 tiles of 16x16 pixels) are
+    marched in per-XCD tile regions, smaller ones in 2x2-tile blocks per XCD.
+    Whole, partial and ragged regions (sizes not multiples of 16): the output
+    starts as NaN and every pixel, background included, must come out equal
+    to the oracle, in the cell and bricked kernels."""
+    from voreen_amd import hipdvr as H, workloads as W
+    w, h = size
+    dims = (64, 64, 64)
+    vol = W.make_volume("c0", dims=dims)
+    _, tf = W.make_tf("c0")
+    cam = W.orbit_camera(5, 36, radius=0.9)
+    llf, urb = W.world_box(dims)
+    en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, cam.fovy_deg, cam.near_dist, w, h, llf=llf, urb=urb)
+    step = W.step_for(dims, 1.0)
+    cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+    assert np.mean(cpu[..., 3] > 0) > 0.5
+    v = ctx.volume(vol)
+    t = ctx.transfunc(tf)
+    de, dx, do = ctx.alloc(w * h * 16), ctx.alloc(w * h * 16), ctx.alloc(w * h * 16)
+    try:
+        de.upload(en)
+        dx.upload(ex)
+        for kernel in (H.KERNEL_CELL, H.KERNEL_BRICKED):
+            say(f"tile orders {w}x{h} kernel {kernel}")
+            do.upload(np.full((h, w, 4), np.nan, np.float32))
+            st = ctx.render_device(v, t, de, dx, w, h, H.default_params(step, kernel=kernel), do, stats=True)
+            gpu = do.download((h, w, 4), np.float32)
+            assert not np.isnan(gpu).any(), f"{w}x{h} kernel {kernel}: pixels not written"
+            assert np.array_equal(gpu, cpu), f"{w}x{h} kernel {kernel}"
+            assert st["samples"] == n
+    finally:
+        for b in (de, dx, do):
+            b.free()
+        v.close()
+        t.close()
+
+
 def test_c0_full_frame(ctx, oracle, kernel):
     say(f"c0 128^3 u8 256^2 step 1/256 linear kernel {kernel}")
     vol, tf, en, ex, step = c0_inputs(oracle)
