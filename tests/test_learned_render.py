@@ -62,6 +62,37 @@ def test_learned_baseline_matches_plain(ctx, scene):
     assert q > 60.0 and not ls["fell_back"]
 
 
+def test_learned_baseline_wide_frame(ctx, scene):
+    """A 560x136 frame spans more than one chip region of tiles, so the learned
+    march takes its tiles in per-XCD regions (partial ones at the edges): every
+    pixel is written and the baseline stays within 60 dB of the plain frame."""
+    from voreen_amd import hipdvr as H, workloads as W
+    print("[case] learned baseline 560x136", flush=True)
+    w, h = 560, 136
+    llf, urb = W.world_box(scene["dims"])
+    bufs = [ctx.alloc(w * h * 16) for _ in range(4)]
+    de, dx, do, dl = bufs
+    l = H.Learned(ctx, scene["v"])
+    try:
+        ctx.entry_exit_device(W.orbit_camera(8, 36, radius=1.6), llf, urb, w, h, de, dx)
+        p = H.default_params(scene["step"], camera_pos=tuple(scene["cam"].eye))
+        ctx.render_device(scene["v"], scene["t"], de, dx, w, h, p, do)
+        plain = do.download((h, w, 4), np.float32)
+        l.prepare(scene["t"], p, None)
+        dl.upload(np.full((h, w, 4), np.nan, np.float32))
+        ls = l.render_device(scene["t"], de, dx, w, h, p, dl, stats=True)
+        img = dl.download((h, w, 4), np.float32)
+    finally:
+        l.close()
+        for b in bufs:
+            b.free()
+    assert not np.isnan(img).any()
+    assert np.mean(plain[..., 3] > 0) > 0.2
+    q = psnr(img, plain)
+    print(f"wide baseline: psnr {q:.1f} dB fell_back {ls['fell_back']}", flush=True)
+    assert q > 60.0 and not ls["fell_back"]
+
+
 def test_trained_network_gains_over_baseline(ctx, scene):
     from voreen_amd import hipdvr as H, stepper
     print("[case] train stepping network (24 TFs x 8 views) and render", flush=True)
