@@ -505,7 +505,7 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
 // to XCD b % 8).  Here the image is cut into chip regions of 4 x 2 XCD
 // regions, each XCD region kXcdRegionW x kXcdRegionH tiles; the i-th
 // workgroup of XCD x marches tile i of XCD region x of chip region
-// i / (W*H), chip regions in row-major order.  So the rays one XCD has in
+// i / (W*H), chip regions from the middle of the image out (center_out).  So the rays one XCD has in
 // flight lie in one compact region and share its L2, and the 8 XCDs march
 // neighbouring regions, so the rays in flight on the chip cover one compact
 // image area and share the Infinity Cache.  Measured on 512^3 / 1024^2 with
@@ -539,6 +539,18 @@ __host__ __device__ inline uint32_t xcd_tile_grid(uint32_t tiles_x, uint32_t til
     return kXcds * kXcdBlock * kXcdBlock * ((bx * by + kXcds - 1) / kXcds);
 }
 
+// The k-th of n rows (or columns) of chip regions, from the middle out:
+// c, c + 1, c - 1, c + 2, ... (c = (n - 1) / 2), then the rest of the longer
+// side.  Long rays through the middle of the volume start first and the
+// cheap edges fill the end of the launch (1.74 -> 1.71 ms on 512^3 / 1024^2
+// with Phong), while the regions in flight stay next to each other.
+__device__ __forceinline__ uint32_t center_out(uint32_t k, uint32_t n) {
+    const uint32_t c = (n - 1) / 2;   // c rows below the middle one, n - 1 - c >= c above
+    if (k == 0) return c;
+    if (k <= 2 * c) return (k & 1) ? c + (k + 1) / 2 : c - (k + 1) / 2;
+    return c + (k - c);
+}
+
 // Tile (tx, ty) of workgroup b; false for a workgroup with no tile.
 __device__ __forceinline__ bool xcd_tile(uint32_t b, uint32_t tiles_x, uint32_t tiles_y, uint32_t& tx,
                                          uint32_t& ty) {
@@ -547,8 +559,10 @@ __device__ __forceinline__ bool xcd_tile(uint32_t b, uint32_t tiles_x, uint32_t 
         constexpr uint32_t R = kXcdRegionW * kXcdRegionH;
         const uint32_t chip = slot / R, j = slot % R;
         const uint32_t cx_n = (tiles_x + kChipRegionW - 1) / kChipRegionW;
-        tx = (chip % cx_n) * kChipRegionW + (xcd % 4) * kXcdRegionW + j % kXcdRegionW;
-        ty = (chip / cx_n) * kChipRegionH + (xcd / 4) * kXcdRegionH + j / kXcdRegionW;
+        const uint32_t cy_n = (tiles_y + kChipRegionH - 1) / kChipRegionH;
+        const uint32_t cx = center_out(chip % cx_n, cx_n), cy = center_out(chip / cx_n, cy_n);
+        tx = cx * kChipRegionW + (xcd % 4) * kXcdRegionW + j % kXcdRegionW;
+        ty = cy * kChipRegionH + (xcd / 4) * kXcdRegionH + j / kXcdRegionW;
     } else {
         constexpr uint32_t S = kXcdBlock * kXcdBlock;
         const uint32_t block = (slot / S) * kXcds + xcd, j = slot % S;
