@@ -37,6 +37,18 @@ void launch_march_cell(const MarchArgs& a, int dtype, int shade, bool count, boo
 // too).
 constexpr int default_batch(int shade) { return shade ? 2 : 4; }
 
+// Unshaded marches run five workgroups per CU: their dynamic LDS is padded so
+// that a workgroup holds 32 KiB of the CU's 160 KiB.  With the seven their
+// VGPRs allow, the rays in flight evict each other's records from the caches
+// (512^3 / 1024^2: plain 1.07 against 1.00 ms; learned 1974 against 2053
+// frames/s).  Returns the dynamic LDS bytes for a kernel whose static LDS is
+// the TF table and the opacity table and that needs `dyn` bytes itself.
+constexpr size_t kUnshadedLdsBytes = 32768;
+constexpr size_t kMarchStaticLds = sizeof(uint32_t) * kMaxTfWidth + sizeof(float) * 256;
+constexpr size_t unshaded_dyn_lds(size_t dyn) {
+    return dyn < kUnshadedLdsBytes - kMarchStaticLds ? kUnshadedLdsBytes - kMarchStaticLds : dyn;
+}
+
 // Zero n 64-bit counters with a kernel (graph-capture safe, unlike a memset node).
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // The plain re-march of a sparse pixel grid (a.grid_w x a.grid_h, a.stride,

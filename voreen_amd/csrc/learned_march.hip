@@ -561,8 +561,9 @@ static void launch_learned_t(const LearnedArgs& la, bool phong, bool count, dim3
         if (count) hipLaunchKernelGGL((learned_kernel<T, WIDE, true, true>), grid, dim3(256), 0, st, la);
         else hipLaunchKernelGGL((learned_kernel<T, WIDE, true, false>), grid, dim3(256), 0, st, la);
     } else {
-        if (count) hipLaunchKernelGGL((learned_kernel<T, WIDE, false, true>), grid, dim3(256), 0, st, la);
-        else hipLaunchKernelGGL((learned_kernel<T, WIDE, false, false>), grid, dim3(256), 0, st, la);
+        const size_t shm = unshaded_dyn_lds(0);   // five workgroups per CU
+        if (count) hipLaunchKernelGGL((learned_kernel<T, WIDE, false, true>), grid, dim3(256), shm, st, la);
+        else hipLaunchKernelGGL((learned_kernel<T, WIDE, false, false>), grid, dim3(256), shm, st, la);
     }
 }
 

@@ -57,15 +57,7 @@ static void launch_k(const MarchArgs& a, bool count, hipStream_t st) {
     const uint32_t tx = (a.grid_w + 15) / 16, ty = (a.grid_h + 15) / 16;
     const dim3 grid(xcd_tile_grid(tx, ty));
     size_t shm = a.occ_words <= kMaxLdsOccWords ? (size_t)a.occ_words * 4 : 0;
-    if (SHADE == 0) {
-        // Unshaded marches run five workgroups per CU: the dynamic LDS is
-        // sized so that a workgroup holds 32 KiB of the CU's 160 KiB.  With
-        // seven in flight (their VGPR limit) the rays evict each other's
-        // records from the caches: 1.07 against 1.00 ms on 512^3 / 1024^2.
-        constexpr size_t kStaticLds = sizeof(uint32_t) * kMaxTfWidth + sizeof(float) * 256;
-        constexpr size_t kUnshadedLds = 32768;
-        if (shm < kUnshadedLds - kStaticLds) shm = kUnshadedLds - kStaticLds;
-    }
+    if (SHADE == 0) shm = unshaded_dyn_lds(shm);   // five workgroups per CU
     if (count)
         hipLaunchKernelGGL((march_cell_kernel<T, WIDE, SHADE, true, K>), grid, dim3(256), shm, st, a, tx, ty);
     else
