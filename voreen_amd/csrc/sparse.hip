@@ -64,10 +64,14 @@ __device__ int ray_samples(float step, float tend, int cap) {
 }
 
 __device__ __forceinline__ void load_tables(const MarchArgs& a, uint32_t* s_tf, float* s_alpha) {
-    for (int i = threadIdx.x; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
-    s_alpha[threadIdx.x] = a.alpha_corr[threadIdx.x];
+    for (int i = threadIdx.x; i < a.tf_width; i += blockDim.x) s_tf[i] = a.tf[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_alpha[i] = a.alpha_corr[i];
     __syncthreads();
 }
+
+// The join runs one lane per sparse ray (16K at 1024^2): one-wave workgroups
+// spread them over all CUs instead of a quarter of them.
+constexpr uint32_t kJoinBlock = 64;
 
 template <typename T, bool WIDE, int SHADE>
 __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a, float4* __restrict__ seg) {
@@ -106,11 +110,11 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
 }
 
 template <typename T, bool WIDE, int SHADE>
-__global__ __launch_bounds__(256) void sparse_join_kernel(const MarchArgs a, const float4* __restrict__ seg) {
+__global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs a, const float4* __restrict__ seg) {
     __shared__ uint32_t s_tf[kMaxTfWidth];
     __shared__ float s_alpha[256];
     load_tables(a, s_tf, s_alpha);
-    const uint32_t ray = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t ray = blockIdx.x * kJoinBlock + threadIdx.x;
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
     if (ray >= rays) return;
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
@@ -168,7 +172,8 @@ void launch_sparse_t(const MarchArgs& a, float4* seg, hipStream_t st) {
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
     hipLaunchKernelGGL((sparse_segments_kernel<T, WIDE, SHADE>), dim3((rays * kSegments + 255) / 256), dim3(256), 0,
                        st, a, seg);
-    hipLaunchKernelGGL((sparse_join_kernel<T, WIDE, SHADE>), dim3((rays + 255) / 256), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((sparse_join_kernel<T, WIDE, SHADE>), dim3((rays + kJoinBlock - 1) / kJoinBlock),
+                       dim3(kJoinBlock), 0, st, a,
                        (const float4*)seg);
 }
 
