@@ -70,7 +70,7 @@ typedef struct hipdvr_learned hipdvr_learned;
 
 typedef struct hipdvr_learned_opts {
     float psnr_floor;        /* dB; below it the frame is rendered plainly (default 45) */
-    int32_t remarch_stride;  /* sparse judge: every n-th pixel in x and y (default 8 = 1/64) */
+    int32_t remarch_stride;  /* sparse judge: 1/n of the pixels along x and y, in blocks of 8 (default 8 = 1/64) */
     float skip_logit;        /* skip a brick when the skip logit exceeds this (default 0) */
     int32_t max_log2_step;   /* largest step multiplier, as log2 (0..3, default 3) */
 } hipdvr_learned_opts;
@@ -109,8 +109,11 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
                                  const hipdvr_learned_opts* opts, float* d_out,
                                  hipdvr_learned_stats* stats);
-/* The judge's plain re-march on its own: every opts->remarch_stride-th pixel
- * in x and y (from stride/2), marched by the plain path with each ray cut
+/* The judge's plain re-march on its own: 1/remarch_stride^2 of the pixels,
+ * along each axis blocks of 8 consecutive pixels, one block in every
+ * 8 * remarch_stride pixels, starting at (remarch_stride - 1) * 4 (grid
+ * column g -> pixel (g / 8) * 8 * stride + offset + g % 8, the offset
+ * clamped to the image), marched by the plain path with each ray cut
  * into segments that run in parallel and are joined front to back (equal to
  * the sequential march up to float rounding of the joins).  d_sparse
  * receives *sw x *sh RGBA floats (row-major); synchronous. */

@@ -48,6 +48,6 @@ void launch_targets(const float* err, const unsigned int* visits, const float* f
                     uint32_t capacity, hipStream_t st);
 void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st);
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
-                  int offs, float* sse, int* flag, float mse_max, hipStream_t st);
+                  int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st);
 
 }  // namespace hipdvr

@@ -226,21 +226,35 @@ struct hipdvr_learned {
 
 namespace {
 
-// The judge's sparse pixels (gx*stride + offs, gy*stride + offs), inside the
-// image, and device buffers sized for them.
+// The judge's sparse pixels, inside the image, and device buffers sized for
+// them: along each axis blocks of kJudgeBlock consecutive pixels, one block
+// in every stride * kJudgeBlock pixels, centred in it (judge_pixel).  The
+// same 1/stride^2 of the pixels as a uniform grid, but neighbouring judge
+// rays share their cell records in the caches: on 512^3 / 1024^2 with Phong
+// the re-march takes 0.70 ms (segments 0.43, join 0.28) against 0.78 ms for
+// every stride-th pixel, and the learned frame runs 1161 against 1090
+// frames/s.
+constexpr uint32_t kJudgeBlock = 8;
+
 struct SparseGrid {
-    uint32_t stride, offs;
+    uint32_t stride, block, offs;
     int w, h;
 };
 
 SparseGrid sparse_grid(hipdvr_learned* l, uint32_t w, uint32_t h, uint32_t stride) {
     SparseGrid g;
     g.stride = stride;
-    g.offs = stride / 2;
+    g.block = kJudgeBlock;
+    g.offs = (stride - 1) * kJudgeBlock / 2;
     if (g.offs > w - 1) g.offs = w - 1;
     if (g.offs > h - 1) g.offs = h - 1;
-    g.w = (int)((w - 1 - g.offs) / stride) + 1;
-    g.h = (int)((h - 1 - g.offs) / stride) + 1;
+    // grid columns g with judge_pixel(g) <= w - 1
+    auto count = [&](uint32_t n) {
+        const uint32_t q = n - 1 - g.offs, span = stride * kJudgeBlock, r = q % span + 1;
+        return (int)((q / span) * kJudgeBlock + (r < kJudgeBlock ? r : kJudgeBlock));
+    };
+    g.w = count(w);
+    g.h = count(h);
     const size_t need = (size_t)g.w * g.h * sizeof(float4);
     if (need > l->sparse_cap) {
         if (l->d_sparse) (void)hipFree(l->d_sparse);
@@ -270,6 +284,7 @@ void launch_sparse(hipdvr_learned* l, hipdvr_volume* vol, const hipdvr_tf* tf, c
     sp.grid_w = g.w;
     sp.grid_h = g.h;
     sp.stride = (int32_t)g.stride;
+    sp.pix_block = (int32_t)g.block;
     sp.pix_offset = (int32_t)g.offs;
     launch_sparse_remarch(sp, vol->desc.dtype, shade, vol->cells_wide, l->d_seg, st);
 }
@@ -533,8 +548,8 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(l->ctx);
-        // 1. sparse plain re-march (the plain cell kernel on every stride-th
-        //    pixel) on the side stream, beside 2. the learned frame
+        // 1. sparse plain re-march (the plain march of the judge's pixels,
+        //    sparse_grid) on the side stream, beside 2. the learned frame
         ensure_cells(vol, st);
         hip_check(hipEventRecord(l->fork, st), "hipEventRecord");
         hip_check(hipStreamWaitEvent(l->side, l->fork, 0), "hipStreamWaitEvent");
@@ -547,7 +562,8 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         hip_check(hipStreamWaitEvent(st, l->join, 0), "hipStreamWaitEvent");
         // 3. judge: flag = PSNR below the floor
         const float mse_max = powf(10.f, -opts.psnr_floor / 10.f);
-        launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)offs,
+        launch_judge((const float4*)d_out, (int)w, (int)h, l->d_sparse, swc, shc, (int)stride, (int)sg.block,
+                     (int)offs,
                      l->d_sse, l->d_flag, mse_max, st);
         // 4. fallback: the plain frame, launched always, runs only if flagged
         MarchArgs fb;

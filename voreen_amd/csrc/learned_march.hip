@@ -585,11 +585,11 @@ void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hi
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void judge_kernel(const float4* __restrict__ learned, int w, int h,
                                                     const float4* __restrict__ sparse, int sw, int sh,
-                                                    int stride, int offs, float* __restrict__ sse) {
+                                                    int stride, int block, int offs, float* __restrict__ sse) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float e = 0.f;
     if (i < sw * sh) {
-        const int x = (i % sw) * stride + offs, y = (i / sw) * stride + offs;
+        const int x = judge_pixel(i % sw, stride, block, offs), y = judge_pixel(i / sw, stride, block, offs);
         const float4 p = learned[(size_t)y * w + x], q = sparse[i];
         e = (p.x - q.x) * (p.x - q.x) + (p.y - q.y) * (p.y - q.y) + (p.z - q.z) * (p.z - q.z) +
             (p.w - q.w) * (p.w - q.w);
@@ -609,10 +609,10 @@ __global__ void zero_f32_kernel_l(float* p, int n) {
 }
 
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
-                  int offs, float* sse, int* flag, float mse_max, hipStream_t st) {
+                  int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st) {
     hipLaunchKernelGGL(zero_f32_kernel_l, dim3(1), dim3(64), 0, st, sse, 1);
     hipLaunchKernelGGL(judge_kernel, dim3((sw * sh + 255) / 256), dim3(256), 0, st, learned, w, h, sparse, sw,
-                       sh, stride, offs, sse);
+                       sh, stride, block, offs, sse);
     hipLaunchKernelGGL(judge_flag_kernel, dim3(1), dim3(64), 0, st, sse, (float)(sw * sh * 4), mse_max, flag);
 }
 

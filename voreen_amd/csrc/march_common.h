@@ -75,7 +75,8 @@ struct MarchArgs {
     int32_t w, h;               // image size (entry/exit indexing)
     // launch grid in pixels: grid pixel (gx, gy) marches image pixel
     // (gx*stride + pix_offset, gy*stride + pix_offset), writes out[gy*grid_w + gx]
-    int32_t grid_w, grid_h, stride, pix_offset;
+    // (the judge's re-march places grid pixels with judge_pixel and pix_block)
+    int32_t grid_w, grid_h, stride, pix_offset, pix_block;
     const int* cond_flag;       // if set, the launch does nothing unless *cond_flag != 0
     int32_t compositing;        // kComp*
     float gamma, iso_value;     // MIDA gamma, ISO value
@@ -514,6 +515,13 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
 // ms, 16x8 (chip regions as wide as the image) 2.74 ms; one contiguous
 // image band per XCD 2.74 ms.  On 128^3 / 256^2 (16 x 16 tiles) the 2x2
 // blocks keep all XCDs busy (50,055 frames/s against 46,139 with regions).
+// Image column (or row) of the judge's grid column g: blocks of `block`
+// consecutive pixels, one block in every stride * block pixels, from offs
+// (block 1: every stride-th pixel).
+__host__ __device__ inline int judge_pixel(int g, int stride, int block, int offs) {
+    return (g / block) * (stride * block) + offs + g % block;
+}
+
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kXcdRegionW = 8, kXcdRegionH = 4;
 constexpr uint32_t kChipRegionW = 4 * kXcdRegionW, kChipRegionH = 2 * kXcdRegionH;

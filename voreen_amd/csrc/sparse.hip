@@ -1,5 +1,6 @@
-// sparse.hip -- the plain re-march of the judge's sparse pixels (every
-// stride-th pixel of a learned frame), split into segments along each ray.
+// sparse.hip -- the plain re-march of the judge's sparse pixels (1/stride^2
+// of a learned frame's pixels, in blocks: judge_pixel), split into segments
+// along each ray.
 //
 // A ray is a dependent chain of loads; 1/64 of a frame's rays (16K at 1024^2)
 // fill about a CU's worth of waves each, so marching each of them whole
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
     const uint32_t ray = gid / kSegments, j = gid % kSegments;
     if (ray >= rays) return;
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
-    const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
+    const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
+    const int py = judge_pixel(gy, a.stride, a.pix_block, a.pix_offset);
     float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
     const size_t pix = (size_t)py * a.w + px;
     const float4 e = a.entry[pix], l = a.exit[pix];
@@ -118,7 +120,8 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
     if (ray >= rays) return;
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
-    const int px = gx * a.stride + a.pix_offset, py = gy * a.stride + a.pix_offset;
+    const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
+    const int py = judge_pixel(gy, a.stride, a.pix_block, a.pix_offset);
     float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
     const size_t pix = (size_t)py * a.w + px;
     const float4 e = a.entry[pix], l = a.exit[pix];
