@@ -5,8 +5,8 @@
 // A ray is a dependent chain of loads; 1/64 of a frame's rays (16K at 1024^2)
 // fill about a CU's worth of waves each, so marching each of them whole
 // takes as long as the longest ray.  Here every ray's samples [0, N) are cut
-// into kSegments runs of equal length, one lane each, every run composited
-// front to back from A = 0; a second pass joins the runs in order with the
+// into segments<SHADE>() runs of equal length, one lane each, every run
+// composited front to back from A = 0; a second pass joins the runs in order with the
 // same "over" operator (associative up to float rounding, ~1e-7) and, in the
 // run where the joined opacity reaches the early-termination threshold,
 // marches that run again sample by sample from the joined state, so the
@@ -24,9 +24,12 @@ namespace hipdvr {
 
 namespace {
 
-// Segments per ray (learned C1 frame with the judge's pixels in blocks: 16
-// segments 1164, 32 1216, 64 1203 frames/s).
-constexpr int kSegments = 32;
+// Segments per ray, measured on the learned 512^3 / 1024^2 frame with the
+// judge's pixels in blocks: shaded 16 -> 1164, 32 -> 1216, 64 -> 1203
+// frames/s; unshaded 16 -> 2068, 32 -> 1904.
+template <int SHADE>
+constexpr int segments() { return SHADE ? 32 : 16; }
+constexpr int kMaxSegments = 32;   // the segment buffer holds this many per ray
 // samples in flight per lane: as the plain march (default_batch); two when
 // shaded lift the learned C1 frame from 1090 to 1129 frames/s
 template <int SHADE>
@@ -83,7 +86,8 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
     load_tables(a, s_tf, s_alpha);
     const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
-    const uint32_t ray = gid / kSegments, j = gid % kSegments;
+    constexpr uint32_t kSeg = segments<SHADE>();
+    const uint32_t ray = gid / kSeg, j = gid % kSeg;
     if (ray >= rays) return;
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
     const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
         float dir[3];
         const float tend = ray_setup(first, last, a.step, dir);
         const int N = ray_samples(a.step, tend, a.loop_cap);
-        const int L = (N + kSegments - 1) / kSegments;
+        const int L = (N + segments<SHADE>() - 1) / segments<SHADE>();
         const int k0 = (int)j * L;
         if (k0 < N) {
             const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
@@ -132,15 +136,15 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
         float dir[3];
         const float tend = ray_setup(first, last, a.step, dir);
         const int N = ray_samples(a.step, tend, a.loop_cap);
-        const int L = (N + kSegments - 1) / kSegments;
+        const int L = (N + segments<SHADE>() - 1) / segments<SHADE>();
         float r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
         int j = 0;
         while (true) {
             // join runs up to the one in which the opacity reaches the threshold
             int jc = -1;
-            for (; j < kSegments; ++j) {
+            for (; j < segments<SHADE>(); ++j) {
                 if (j * L >= N) break;
-                const float4 s4 = seg[(size_t)ray * kSegments + j];
+                const float4 s4 = seg[(size_t)ray * segments<SHADE>() + j];
                 const float An = A + (1.0f - A) * s4.w;
                 if (An >= kErtOpacity - 1.0e-5f) {
                     jc = j;
@@ -175,7 +179,8 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
 template <typename T, bool WIDE, int SHADE>
 void launch_sparse_t(const MarchArgs& a, float4* seg, hipStream_t st) {
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
-    hipLaunchKernelGGL((sparse_segments_kernel<T, WIDE, SHADE>), dim3((rays * kSegments + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL((sparse_segments_kernel<T, WIDE, SHADE>), dim3((rays * segments<SHADE>() + 255) / 256),
+                       dim3(256), 0,
                        st, a, seg);
     hipLaunchKernelGGL((sparse_join_kernel<T, WIDE, SHADE>), dim3((rays + kJoinBlock - 1) / kJoinBlock),
                        dim3(kJoinBlock), 0, st, a,
@@ -191,7 +196,7 @@ void launch_sparse_w(const MarchArgs& a, int shade, float4* seg, hipStream_t st)
 
 }  // namespace
 
-size_t sparse_segment_bytes(uint32_t rays) { return (size_t)rays * kSegments * sizeof(float4); }
+size_t sparse_segment_bytes(uint32_t rays) { return (size_t)rays * kMaxSegments * sizeof(float4); }
 
 void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st) {
     if (dtype == 0) {
