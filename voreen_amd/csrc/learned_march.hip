@@ -583,37 +583,39 @@ void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hi
 // plainly re-marched sparse image; sets flag[0] = 1 when the PSNR is below
 // the floor (mse > mse_max)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void judge_kernel(const float4* __restrict__ learned, int w, int h,
-                                                    const float4* __restrict__ sparse, int sw, int sh,
-                                                    int stride, int block, int offs, float* __restrict__ sse) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup sums every sparse pixel's error (16K pixels at 1024^2, a few
+// microseconds) and writes the sum and the flag: one launch, no atomics, the
+// same sum on every run.
+constexpr int kJudgeThreads = 1024;
+
+__global__ __launch_bounds__(kJudgeThreads) void judge_kernel(const float4* __restrict__ learned, int w, int h,
+                                                              const float4* __restrict__ sparse, int sw, int sh,
+                                                              int stride, int block, int offs, float denom,
+                                                              float mse_max, float* __restrict__ sse,
+                                                              int* __restrict__ flag) {
+    __shared__ float part[kJudgeThreads / 64];
     float e = 0.f;
-    if (i < sw * sh) {
+    for (int i = threadIdx.x; i < sw * sh; i += kJudgeThreads) {
         const int x = judge_pixel(i % sw, stride, block, offs), y = judge_pixel(i / sw, stride, block, offs);
         const float4 p = learned[(size_t)y * w + x], q = sparse[i];
-        e = (p.x - q.x) * (p.x - q.x) + (p.y - q.y) * (p.y - q.y) + (p.z - q.z) * (p.z - q.z) +
-            (p.w - q.w) * (p.w - q.w);
+        e += (p.x - q.x) * (p.x - q.x) + (p.y - q.y) * (p.y - q.y) + (p.z - q.z) * (p.z - q.z) +
+             (p.w - q.w) * (p.w - q.w);
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(sse, e);
-}
-
-__global__ void judge_flag_kernel(const float* __restrict__ sse, float denom, float mse_max,
-                                  int* __restrict__ flag) {
-    if (threadIdx.x == 0) flag[0] = (sse[0] / denom > mse_max) ? 1 : 0;
-}
-
-__global__ void zero_f32_kernel_l(float* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.f;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int k = 0; k < kJudgeThreads / 64; ++k) t += part[k];
+        sse[0] = t;
+        flag[0] = (t / denom > mse_max) ? 1 : 0;
+    }
 }
 
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
                   int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st) {
-    hipLaunchKernelGGL(zero_f32_kernel_l, dim3(1), dim3(64), 0, st, sse, 1);
-    hipLaunchKernelGGL(judge_kernel, dim3((sw * sh + 255) / 256), dim3(256), 0, st, learned, w, h, sparse, sw,
-                       sh, stride, block, offs, sse);
-    hipLaunchKernelGGL(judge_flag_kernel, dim3(1), dim3(64), 0, st, sse, (float)(sw * sh * 4), mse_max, flag);
+    hipLaunchKernelGGL(judge_kernel, dim3(1), dim3(kJudgeThreads), 0, st, learned, w, h, sparse, sw, sh, stride,
+                       block, offs, (float)(sw * sh * 4), mse_max, sse, flag);
 }
 
 }  // namespace hipdvr
