@@ -501,20 +501,6 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long 
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
-// Image tiles (16x16 pixels, one workgroup each) in XCD-compact order.  The
-// hardware hands consecutive workgroups to the 8 XCDs in turn (workgroup b
-// to XCD b % 8).  Here the image is cut into chip regions of 4 x 2 XCD
-// regions, each XCD region kXcdRegionW x kXcdRegionH tiles; the i-th
-// workgroup of XCD x marches tile i of XCD region x of chip region
-// i / (W*H), chip regions from the middle of the image out (center_out).  So the rays one XCD has in
-// flight lie in one compact region and share its L2, and the 8 XCDs march
-// neighbouring regions, so the rays in flight on the chip cover one compact
-// image area and share the Infinity Cache.  Measured on 512^3 / 1024^2 with
-// Phong: plain dispatch order 1.90 ms; each XCD on 2x2-tile blocks taken in
-// turn 1.79 ms; XCD regions of 4x4 tiles 1.74 ms, 8x4 1.73 ms, 8x8 1.79
-// ms, 16x8 (chip regions as wide as the image) 2.74 ms; one contiguous
-// image band per XCD 2.74 ms.  On 128^3 / 256^2 (16 x 16 tiles) the 2x2
-// blocks keep all XCDs busy (50,055 frames/s against 46,139 with regions).
 // Image column (or row) of the judge's grid column g: blocks of `block`
 // consecutive pixels, one block in every stride * block pixels, from offs
 // (block 1: every stride-th pixel).
@@ -522,6 +508,20 @@ __host__ __device__ inline int judge_pixel(int g, int stride, int block, int off
     return (g / block) * (stride * block) + offs + g % block;
 }
 
+// Image tiles (16x16 pixels, one workgroup each) in XCD-compact order.  The
+// hardware hands consecutive workgroups to the 8 XCDs in turn (workgroup b
+// to XCD b % 8).  Here the image is cut into chip regions of 4 x 2 XCD
+// regions, each XCD region kXcdRegionW x kXcdRegionH tiles; the i-th
+// workgroup of XCD x marches tile i of XCD region x of chip region
+// i / (W*H), chip regions from the middle of the image out (center_out).
+// So the rays one XCD has in flight lie in one compact region and share its
+// L2, and the 8 XCDs march neighbouring regions, so the rays in flight on
+// the chip cover one compact image area and share the Infinity Cache.  Measured on 512^3 / 1024^2 with
+// Phong: plain dispatch order 1.90 ms; each XCD on 2x2-tile blocks taken in
+// turn 1.79 ms; XCD regions of 4x4 tiles 1.74 ms, 8x4 1.73 ms, 8x8 1.79
+// ms, 16x8 (chip regions as wide as the image) 2.74 ms; one contiguous
+// image band per XCD 2.74 ms.  On 128^3 / 256^2 (16 x 16 tiles) the 2x2
+// blocks keep all XCDs busy (51,220 frames/s against 46,139 with regions).
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kXcdRegionW = 8, kXcdRegionH = 4;
 constexpr uint32_t kChipRegionW = 4 * kXcdRegionW, kChipRegionH = 2 * kXcdRegionH;
