@@ -77,13 +77,37 @@ def measured_traffic(config, shading):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it: the model-name string of the
+    first processor, with its family/model/stepping numbers."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" not in line:
+                if info:
+                    break
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    return f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, " \
+           f"stepping {info.get('stepping', '?')})"
+
+
+def host_cores():
+    """(threads for the baseline, logical CPUs this process may run on, CPUs
+    online).  The baseline uses every CPU the process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (a shared GPU box sets it to
+    the CPU share of one GPU)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    online = os.cpu_count() or affinity
+    omp = int(os.environ.get("OMP_NUM_THREADS") or 0)
+    use = min(affinity, omp) if omp > 0 else affinity
+    return use, affinity, online
 
 
 def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
@@ -92,7 +116,7 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     one warm-up, then the median of 3..9 timed runs within about target_s;
     frames/s = measured samples/s over the plain samples of a frame."""
     from oracle import oracle as O
-    cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
+    cores, affinity, online = host_cores()
     h = en.shape[0]
     rows = 8
     while True:
@@ -117,6 +141,9 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
         "value": round(sps / samples_per_frame, 5),
         "unit": "frames/s",
         "cores": cores,
+        "cpus_affinity": affinity,
+        "cpus_online": online,
+        "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
         "cpu_model": cpu_model(),
         "kind": "port",
         "gsamples_per_s": round(sps / 1e9, 5),

@@ -15,9 +15,10 @@
  * from plain-marcher traces.
  *
  * The plain marcher stays the judge. For each learned frame, 1/64 of the rays
- * (every 8th pixel in x and y) are re-marched plainly and compared. If the
- * PSNR of that sparse set falls below the floor (45 dB by default), the frame
- * is rendered plainly instead.
+ * are re-marched plainly and compared: 8x8-pixel blocks, one block in every
+ * 64x64 pixels (hipdvr_learned_sparse_dims gives the grid).  If the PSNR of
+ * that sparse set falls below the floor (45 dB by default), the frame is
+ * rendered plainly instead.
  *
  * Features of one (brick, octant) row, normalized intensities in [0, 1]:
  *   0 min  1 max  2 mean  3 std of the brick's voxels (16^3 + 1-voxel apron)
@@ -109,18 +110,25 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
                                  const hipdvr_learned_opts* opts, float* d_out,
                                  hipdvr_learned_stats* stats);
-/* The judge's plain re-march on its own: 1/remarch_stride^2 of the pixels,
- * along each axis blocks of 8 consecutive pixels, one block in every
- * 8 * remarch_stride pixels, starting at (remarch_stride - 1) * 4 (grid
- * column g -> pixel (g / 8) * 8 * stride + offset + g % 8, the offset
- * clamped to the image), marched by the plain path with each ray cut
+/* The judge's sparse grid for a w x h image: along each axis blocks of 8
+ * consecutive pixels, one block in every 8 * remarch_stride pixels, starting
+ * at (remarch_stride - 1) * 4 clamped to the image (grid column g -> pixel
+ * (g / 8) * 8 * stride + offset + g % 8).  *sw x *sh is the number of grid
+ * pixels; a ragged image edge can add up to one partial block per axis, so
+ * it may exceed ceil(w / stride) x ceil(h / stride).  Host arithmetic only
+ * (opts may be NULL: stride 8). */
+int hipdvr_learned_sparse_dims(uint32_t w, uint32_t h, const hipdvr_learned_opts* opts, uint32_t* sw,
+                               uint32_t* sh);
+/* The judge's plain re-march on its own, on the grid of
+ * hipdvr_learned_sparse_dims, marched by the plain path with each ray cut
  * into segments that run in parallel and are joined front to back (equal to
- * the sequential march up to float rounding of the joins).  d_sparse
- * receives *sw x *sh RGBA floats (row-major); synchronous. */
+ * the sequential march up to float rounding of the joins).  d_sparse holds
+ * `capacity` RGBA float pixels and receives *sw x *sh of them (row-major);
+ * HIPDVR_ERR_INVALID when capacity is below that.  Synchronous. */
 int hipdvr_learned_sparse_remarch(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
                                   const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
-                                  const hipdvr_learned_opts* opts, float* d_sparse, uint32_t* sw,
-                                  uint32_t* sh);
+                                  const hipdvr_learned_opts* opts, float* d_sparse, uint64_t capacity,
+                                  uint32_t* sw, uint32_t* sh);
 
 #ifdef __cplusplus
 }

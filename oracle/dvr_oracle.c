@@ -184,7 +184,9 @@ void dvr_preint_build(const dvr_tf_key* keys, int32_t nkeys, float gamma, float 
                     float scale = 1.0f / (fabsf((float)(sb - sf)) + 1);
                     int incr = (sb < sf) ? -1 : 1;
                     float r = 0.f, g = 0.f, b = 0.f, a = 0.f;
-                    for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && (a < 0.95f); s += incr) {
+                    /* preintegrationtable.cpp:115 compares the float alpha with the
+                     * double 0.95 (a composite alpha of exactly 0.95f continues) */
+                    for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && ((double)a < 0.95); s += incr) {
                         const float* cc = tfb + 4 * s;
                         float ca = cc[3];
                         if (ca > 0.0f) {

@@ -9,6 +9,7 @@ The oracle restates Voreen's CPURaycaster
 transfer-function, pre-integration and volume-access code it calls; see the
 header of dvr_oracle.c for the per-function reference lines and pinning.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -66,10 +67,12 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def _load():
-    if not os.path.exists(_LIB_PATH):
-        build()
-    lib = C.CDLL(_LIB_PATH)
+def _load(path=None):
+    if path is None:
+        path = _LIB_PATH
+        if not os.path.exists(path):
+            build()
+    lib = C.CDLL(path)
     P = C.POINTER
     lib.dvr_splitmix64.argtypes = [P(C.c_uint64)]
     lib.dvr_splitmix64.restype = C.c_uint64
@@ -337,3 +340,17 @@ def mlp_loss_grad(params, x, t, quantize=True):
     g = np.zeros(MLP_PARAMS, np.float64)
     loss = lib().dvr_mlp_loss_grad(_ptr(params), int(quantize), _ptr(x), _ptr(t), x.shape[0], _ptr(g))
     return loss, g
+
+
+@contextlib.contextmanager
+def variant(path):
+    """Route every wrapper of this module through another build of the oracle
+    (a copy with a planted mistake, built by the mutation tests) for the
+    duration of the block."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved

@@ -33,6 +33,9 @@ def lib():
         l.hrc_is_ready.argtypes = [vp]
         l.hrc_step.argtypes = [vp]
         l.hrc_step.restype = C.c_float
+        l.hrc_render_step.argtypes = [vp]
+        l.hrc_render_step.restype = C.c_float
+        l.hrc_set_interaction.argtypes = [C.c_int]
         l.hrc_process.argtypes = [vp, vp, C.c_int, C.c_int]
         l.hrc_last_error.argtypes = [vp, C.c_char_p, C.c_int]
         l.hrc_stats.argtypes = [vp, vp]
@@ -108,6 +111,14 @@ class Raycaster:
 
     def step(self):
         return lib().hrc_step(self.h)
+
+    def render_step(self):
+        """The step HIPRaycaster marches with."""
+        return lib().hrc_render_step(self.h)
+
+    @staticmethod
+    def set_interaction(on):
+        lib().hrc_set_interaction(int(on))
 
     def process(self):
         w, h = self.size

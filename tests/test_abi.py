@@ -80,6 +80,45 @@ def test_preint_build_matches_oracle(oracle, integral):
             assert np.array_equal(a, b)
 
 
+def test_preint_build_pins(oracle):
+    """The library's host table passes the oracle's closed-form pins: the
+    orientation of a two-colour step TF and the double 0.95 tie."""
+    import test_oracle as T
+    from voreen_amd import hipdvr as H
+    a = H.preint_build(T._STEP_KEYS, 32, 1 / 64)
+    assert np.array_equal(a, oracle.preint_build(T._STEP_KEYS, 32, 1 / 64))
+    for lo, hi in ((5, 25), (15, 16)):
+        e = (1 / 64) * 200 / (hi - lo + 1)
+        assert np.allclose(a[hi, lo], T._step_entry(lo, hi, e, True), atol=2e-5)
+    keys, step, _ = T.preint_tie_case()
+    tie = H.preint_build(keys, 2, step)
+    assert np.array_equal(tie, oracle.preint_build(keys, 2, step))
+    assert tie[1, 0][3] > np.float32(0.95) + 1e-3
+
+
+def test_learned_sparse_dims_match_the_judge_pixels():
+    """hipdvr_learned_sparse_dims counts the grid columns whose judge pixel
+    (blocks of 8, one in every 8*stride pixels, from (stride-1)*4 clamped to
+    the image) lies inside the image, for ragged sizes and several strides."""
+    from voreen_amd import hipdvr as H
+
+    def brute(n, stride, offs):
+        g = 0
+        while (g // 8) * 8 * stride + offs + g % 8 <= n - 1:
+            g += 1
+        return g
+
+    for w, h in ((100, 37), (1, 1), (8, 9), (1024, 1024), (3840, 2160), (513, 257), (29, 300)):
+        for stride in (1, 2, 3, 8, 16):
+            offs = min((stride - 1) * 4, w - 1, h - 1)
+            sw, sh = H.learned_sparse_dims(w, h, stride)
+            assert (sw, sh) == (brute(w, stride, offs), brute(h, stride, offs)), (w, h, stride)
+    # the ragged edge can exceed ceil(w / stride): 100 wide at stride 8 is 16 columns
+    assert H.learned_sparse_dims(100, 100, 8) == (16, 16)
+    with pytest.raises(H.HipDvrError):
+        H.learned_sparse_dims(0, 10)
+
+
 def _round_f32(x):
     """Correctly rounded (nearest-even) float32 of an exact Fraction."""
     c = np.float32(float(x))

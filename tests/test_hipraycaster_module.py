@@ -74,8 +74,27 @@ def test_is_ready_needs_volume_and_both_inports(rc):
 def test_sampling_step_is_the_reference_rule(rc):
     rc.set_volume(np.zeros((96, 114, 128), np.uint16))   # max dim 128
     assert rc.step() == np.float32(1.0) / (np.float32(128) * np.float32(2.0))
+    assert rc.render_step() == rc.step()
     assert rc.set("samplingRate", 4.0) == 0
     assert rc.step() == np.float32(1.0) / (np.float32(128) * np.float32(4.0))
+    assert rc.render_step() == rc.step()
+
+
+def test_interaction_mode_keeps_cpuraycasters_step(rc):
+    """In interaction mode VolumeRaycaster::getSamplingStepSize divides by
+    interactionQuality (volumeraycaster.cpp:312-322); CPURaycaster marches
+    1/(max(dim)*rate) regardless (cpuraycaster.cpp:104), and so does
+    HIPRaycaster."""
+    rc.set_volume(np.zeros((96, 114, 128), np.uint16))
+    assert rc.set("interactionQuality", 0.5) == 0
+    base = np.float32(1.0) / (np.float32(128) * np.float32(2.0))
+    try:
+        rc.set_interaction(True)
+        assert rc.step() == np.float32(base / np.float32(0.5))
+        assert rc.render_step() == base
+    finally:
+        rc.set_interaction(False)
+    assert rc.step() == base and rc.render_step() == base
 
 
 def test_unsupported_options_log_and_clear(rc):
@@ -151,6 +170,31 @@ def test_process_matches_oracle(rc, oracle, kernel):
     # second frame reuses the uploads (no re-upload needed) and is identical
     out2, nerr = rc.process()
     assert nerr == 0 and np.array_equal(out, out2)
+
+
+@pytest.mark.gpu
+def test_process_in_interaction_mode_marches_cpuraycasters_step(rc, oracle):
+    """With interaction mode on and interactionQuality 0.5 the frame is still
+    CPURaycaster's: the oracle image at 1/(max(dim)*rate) and its sample count."""
+    vol, keys, en, ex, cam = _c0_case(oracle)
+    print("[case] HIPRaycaster.process in interaction mode", flush=True)
+    rc.set_volume(vol)
+    rc.set_tf(keys)
+    rc.set_images(en, ex)
+    assert rc.set("interactionQuality", 0.5) == 0
+    try:
+        rc.set_interaction(True)
+        out, nerr = rc.process()
+        vr_step = rc.step()
+    finally:
+        rc.set_interaction(False)
+    assert nerr == 0, rc.last_error()
+    step = rc.render_step()
+    assert vr_step == np.float32(step / np.float32(0.5))
+    tf = oracle.tf_build(keys, 256)
+    ref, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, step)
+    assert np.array_equal(out, ref)
+    assert rc.stats()["samples"] == n
 
 
 @pytest.mark.gpu

@@ -160,11 +160,18 @@ def test_sparse_remarch_equals_plain(ctx, scene, shading):
     print(f"[case] sparse re-march shading {shading}", flush=True)
     plain, st, p = _plain(ctx, scene, shading)
     l = H.Learned(ctx, scene["v"])
-    ds = ctx.alloc(scene["w"] * scene["h"] * 16)
+    gw, gh = H.learned_sparse_dims(scene["w"], scene["h"])
+    ds = ctx.alloc(gw * gh * 16)
+    small = ctx.alloc(gw * gh * 16 - 16)
     try:
+        # one pixel short of the grid: refused before any launch
+        with pytest.raises(H.HipDvrError):
+            l.sparse_remarch(scene["t"], scene["de"], scene["dx"], scene["w"], scene["h"], p, small)
         sw, sh = l.sparse_remarch(scene["t"], scene["de"], scene["dx"], scene["w"], scene["h"], p, ds)
+        assert (sw, sh) == (gw, gh)
         sparse = ds.download((sh, sw, 4), np.float32)
     finally:
+        small.free()
         ds.free()
         l.close()
     # the judge's pixels: blocks of 8 in every 64 along each axis, from 28

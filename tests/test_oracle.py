@@ -306,6 +306,166 @@ def test_preint_constant_tf_closed_forms(oracle):
     assert np.allclose(np.diagonal(integ[..., 3]), ac, atol=2e-6)
 
 
+# Two-colour step TF for the orientation pins: opaque-ish red below 0.5,
+# blue above (split key), n = 32 table entries (i/31: i <= 15 red, i >= 16 blue).
+_RED = (255, 0, 0, 40)
+_BLUE = (0, 0, 255, 90)
+_STEP_KEYS = [(0.0, _RED), (0.5, _RED, _BLUE), (1.0, _BLUE)]
+
+
+def _runs(a, m, e):
+    """Opacity of m samples of TF alpha a, each corrected with exponent e:
+    1 - ((1 - a)^e)^m (a run of equal samples composites to this)."""
+    return 1.0 - (1.0 - a) ** (e * m)
+
+
+def _over(front, back):
+    """Front-to-back 'over' of two un-premultiplied (rgb, A) layers."""
+    (c1, a1), (c2, a2) = front, back
+    a = a1 + (1 - a1) * a2
+    return (np.asarray(c1) * a1 + (1 - a1) * a2 * np.asarray(c2)) / a, a
+
+
+def _step_entry(lo, hi, e, front_red):
+    """Table entry compositing indices lo..hi of the step TF (lo red, hi blue)
+    with the front at lo (front_red) or at hi, in closed form."""
+    m_red, m_blue = 16 - lo, hi - 15
+    lr = (np.array([1.0, 0, 0]), _runs(_RED[3] / 255, m_red, e))
+    lb = (np.array([0, 0, 1.0]), _runs(_BLUE[3] / 255, m_blue, e))
+    rgb, a = _over(lr, lb) if front_red else _over(lb, lr)
+    return np.array([*rgb, a])
+
+
+def pin_preint_orientation(oracle):
+    """PreIntegrationTable::computeTable (preintegrationtable.cpp:98-140):
+    element [sb*N + sf] composites the TF from sf (front) to sb (back), each
+    sample corrected with exponent step*200/(|sb-sf|+1).  With red in front
+    the entry is red-dominated; the transposed table would be blue."""
+    n, step = 32, 1 / 64
+    tfb = oracle.tf_build(_STEP_KEYS, n)
+    assert np.all(tfb[:16] == _RED) and np.all(tfb[16:] == _BLUE)
+    table = oracle.preint_build(_STEP_KEYS, n, step)
+    for lo, hi in ((5, 25), (0, 31), (15, 16), (10, 20)):
+        e = step * 200 / (hi - lo + 1)
+        fwd = _step_entry(lo, hi, e, front_red=True)
+        bwd = _step_entry(lo, hi, e, front_red=False)
+        assert np.abs(fwd - bwd).max() > 0.05          # the check can tell them apart
+        assert np.allclose(table[hi, lo], fwd, atol=2e-5), (lo, hi, table[hi, lo], fwd)
+        assert np.allclose(table[lo, hi], bwd, atol=2e-5), (lo, hi, table[lo, hi], bwd)
+    # a homogeneous run of d+1 samples at exponent e/(d+1) is the full-step
+    # opacity correction of its alpha
+    for lo, hi in ((2, 9), (17, 30)):
+        c = _RED if hi < 16 else _BLUE
+        want = 1 - (1 - c[3] / 255) ** (step * 200)
+        assert abs(table[hi, lo][3] - want) < 2e-6 and abs(table[lo, hi][3] - want) < 2e-6
+
+
+def preint_tie_case():
+    """Two TF entries (red alpha 234, blue alpha 128) and a step for which the
+    red sample alone composites to exactly 0.95f: the exponent is
+    step*200/(1+1) and 1 - powf(1 - 234/255, exponent) == 0.95f (searched
+    here with libm's powf).  Returns (keys, step, exponent)."""
+    f = np.float32
+    a0 = 234
+    x = f(1) - f(f(a0) / f(255))
+    e0 = f(math.log(0.05) / math.log(float(x)))
+    for k in range(-400, 401):
+        ek = f(e0 + f(k) * np.spacing(e0))
+        if f(f(1) - np.power(x, ek)) != f(0.95):
+            continue
+        step = f(f(2) * ek) / f(200)
+        for j in range(-3, 4):
+            s2 = f(step + f(j) * np.spacing(step))
+            if f(f(s2 * f(200)) * f(0.5)) == ek:
+                return [(0.0, (255, 0, 0, a0)), (1.0, (0, 0, 255, 128))], float(s2), float(ek)
+    raise AssertionError("no tie step found")
+
+
+def pin_preint_early_termination(oracle):
+    """The exact table's loop runs while the float alpha is below the double
+    0.95 (preintegrationtable.cpp:115): a composite alpha of exactly 0.95f
+    composites the next sample."""
+    keys, step, ek = preint_tie_case()
+    table = oracle.preint_build(keys, 2, step)
+    ca1 = 1.0 - (1.0 - 128 / 255.0) ** ek
+    a95 = float(np.float32(0.95))
+    want_a = a95 + (1 - a95) * ca1
+    want = np.array([a95 / want_a, 0.0, (1 - a95) * ca1 / want_a, want_a])
+    assert np.allclose(table[1, 0], want, atol=2e-6), (table[1, 0], want)
+    assert table[1, 0][3] > a95 + 1e-3 and table[1, 0][2] > 1e-3
+
+
+def pin_preint_classify_order(oracle):
+    """CPURaycaster classifies each sample by table[i2*N + i1] with i1 the
+    previous intensity and i2 the current one (cpuraycaster.cpp:258-263,
+    preintegrationtable.cpp:256-279; the first sample's previous intensity
+    is 0).  A two-sample ray from voxel 0 (red intensity) to voxel 1 (blue):
+    sample 1 is entry [i0, 0], sample 2 entry [i1, i0] (red in front)."""
+    n = 32
+    step = 1 / 64
+    vol = np.zeros((1, 1, 64), np.uint8)
+    vol[0, 0, 0], vol[0, 0, 1] = 40, 200
+    i0, i1 = round(40 / 255 * 31), round(200 / 255 * 31)
+    assert (i0, i1) == (5, 24)
+    en = np.array([[[0.5 / 64, 0.5, 0.5, 1.0]]], np.float32)
+    ex = np.array([[[1.5 / 64, 0.5, 0.5, 1.0]]], np.float32)
+    table = oracle.preint_build(_STEP_KEYS, n, step)
+    img, samples = oracle.render(vol, None, (0.0, 1.0), en, ex, step, classification=oracle.CLASS_PREINT,
+                                 preint=table)
+    assert samples == 2
+    # entry [i0, 0]: indices 0..i0, all red: red at the full-step opacity
+    first = (np.array([1.0, 0, 0]), 1 - (1 - _RED[3] / 255) ** (step * 200))
+    sec = _step_entry(i0, i1, step * 200 / (i1 - i0 + 1), front_red=True)
+    rgb, a = _over(first, (sec[:3], sec[3]))
+    want = np.array([*(rgb * a), a])          # the pixel is premultiplied
+    swapped = _step_entry(i0, i1, step * 200 / (i1 - i0 + 1), front_red=False)
+    rgb_s, a_s = _over(first, (swapped[:3], swapped[3]))
+    assert np.abs(np.array([*(rgb_s * a_s), a_s]) - want).max() > 0.02
+    assert np.allclose(img[0, 0], want, atol=2e-5), (img[0, 0], want)
+
+
+def pin_tf_gamma_ramp(oracle):
+    """TransFunc1D::applyGammaToIntensity (transfunc1d.cpp:144-149) raises the
+    entry's intensity to gamma before the key lookup: the standard ramp
+    (transfunc1dkeys.cpp:413-421) becomes trunc(255 * (i/(W-1))^gamma)."""
+    keys = [(0.0, (0, 0, 0, 0)), (1.0, (255, 255, 255, 255))]
+    for width in (256, 4096):
+        x = np.arange(width) / (width - 1)
+        for gamma in (0.45, 2.2, 3.0):
+            tf = oracle.tf_build(keys, width, gamma=gamma)
+            exact = 255.0 * x ** gamma
+            fl = np.floor(exact)
+            got = tf[:, 0].astype(float)
+            near = np.abs(exact - np.round(exact)) < 1e-4   # float rounding may land either side
+            assert np.all((got == fl) | (near & (np.abs(got - exact) < 1.0 + 1e-9))), (width, gamma)
+            assert np.array_equal(tf[:, 0], tf[:, 3])
+
+
+def test_preint_table_orientation_closed_form(oracle):
+    pin_preint_orientation(oracle)
+
+
+def test_preint_early_termination_compares_with_double(oracle):
+    pin_preint_early_termination(oracle)
+
+
+def test_preint_classify_uses_previous_then_current(oracle):
+    pin_preint_classify_order(oracle)
+
+
+def test_tf_gamma_ramp_closed_form(oracle):
+    pin_tf_gamma_ramp(oracle)
+
+
+def test_double_095_is_float_le_095f():
+    # the device kernel tests a <= 0.95f for the reference's (double)a < 0.95
+    f = np.float32
+    c = f(0.95)
+    around = [np.nextafter(c, f(0)), c, np.nextafter(c, f(1))]
+    for a in around + [f(0.5), f(0.9499), f(0.96), f(1.0)]:
+        assert (float(a) < 0.95) == bool(a <= c)
+
+
 # --------------------------------------------------------------------------
 # entry/exit points of the unit cube
 # --------------------------------------------------------------------------

@@ -72,11 +72,31 @@ def build_module(verbose=True):
     return MODULE_OUT
 
 
-def build(verbose=True):
+def _stale(srcs):
+    """Sources whose object is missing or older than the source or than any
+    header (headers are shared, so a changed header rebuilds everything)."""
+    hdr_dirs = (CSRC, os.path.join(HERE, "..", "include"))
+    newest_hdr = max(os.path.getmtime(os.path.join(d, f)) for d in hdr_dirs for f in os.listdir(d)
+                     if f.endswith(".h"))
+    out = []
+    for s in srcs:
+        obj = os.path.join(BUILD_DIR, s + ".o")
+        if (not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(os.path.join(CSRC, s))
+                or os.path.getmtime(obj) < newest_hdr):
+            out.append(s)
+    return out
+
+
+def build(verbose=True, incremental=False):
+    """Compile every source (incremental=True: only those whose object is
+    stale) and link libhipdvr.so and the HIPRaycaster module."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = sources()
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    todo = _stale(srcs) if incremental else srcs
+    if todo:
+        with cf.ThreadPoolExecutor(max_workers=min(8, len(todo))) as ex:
+            list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD_DIR, s + ".o") for s in srcs]
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -91,7 +111,7 @@ def build(verbose=True):
 
 if __name__ == "__main__":
     try:
-        build()
+        build(incremental="--incremental" in sys.argv)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -241,20 +241,26 @@ struct SparseGrid {
     int w, h;
 };
 
-SparseGrid sparse_grid(hipdvr_learned* l, uint32_t w, uint32_t h, uint32_t stride) {
+// Grid size and placement only (host arithmetic, no allocation).
+SparseGrid sparse_dims(uint32_t w, uint32_t h, uint32_t stride) {
     SparseGrid g;
     g.stride = stride;
     g.block = kJudgeBlock;
     g.offs = (stride - 1) * kJudgeBlock / 2;
     if (g.offs > w - 1) g.offs = w - 1;
     if (g.offs > h - 1) g.offs = h - 1;
-    // grid columns g with judge_pixel(g) <= w - 1
+    // grid columns g with judge_pixel(g) <= n - 1
     auto count = [&](uint32_t n) {
         const uint32_t q = n - 1 - g.offs, span = stride * kJudgeBlock, r = q % span + 1;
         return (int)((q / span) * kJudgeBlock + (r < kJudgeBlock ? r : kJudgeBlock));
     };
     g.w = count(w);
     g.h = count(h);
+    return g;
+}
+
+SparseGrid sparse_grid(hipdvr_learned* l, uint32_t w, uint32_t h, uint32_t stride) {
+    const SparseGrid g = sparse_dims(w, h, stride);
     const size_t need = (size_t)g.w * g.h * sizeof(float4);
     if (need > l->sparse_cap) {
         if (l->d_sparse) (void)hipFree(l->d_sparse);
@@ -321,7 +327,7 @@ hipdvr_learned_opts default_opts(const hipdvr_learned_opts* o) {
     d.max_log2_step = 3;
     if (o) {
         if (o->psnr_floor > 0.f) d.psnr_floor = o->psnr_floor;
-        if (o->remarch_stride > 0) d.remarch_stride = o->remarch_stride;
+        if (o->remarch_stride > 0) d.remarch_stride = o->remarch_stride > 1024 ? 1024 : o->remarch_stride;
         d.skip_logit = o->skip_logit;
         d.max_log2_step = o->max_log2_step < 0 ? 0 : (o->max_log2_step > 3 ? 3 : o->max_log2_step);
     }
@@ -589,9 +595,19 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
     });
 }
 
+int hipdvr_learned_sparse_dims(uint32_t w, uint32_t h, const hipdvr_learned_opts* o, uint32_t* sw, uint32_t* sh) {
+    if (!w || !h || !sw || !sh) return HIPDVR_ERR_INVALID;
+    const hipdvr_learned_opts opts = default_opts(o);
+    const SparseGrid g = sparse_dims(w, h, (uint32_t)opts.remarch_stride);
+    *sw = (uint32_t)g.w;
+    *sh = (uint32_t)g.h;
+    return HIPDVR_OK;
+}
+
 int hipdvr_learned_sparse_remarch(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
                                   const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
-                                  const hipdvr_learned_opts* o, float* d_sparse, uint32_t* sw, uint32_t* sh) {
+                                  const hipdvr_learned_opts* o, float* d_sparse, uint64_t capacity, uint32_t* sw,
+                                  uint32_t* sh) {
     if (!l) return HIPDVR_ERR_INVALID;
     return ctx_guarded(l->ctx, [&] {
         if (!tf || !prm || !d_entry || !d_exit || !d_sparse || !sw || !sh || !w || !h)
@@ -603,6 +619,10 @@ int hipdvr_learned_sparse_remarch(hipdvr_learned* l, const hipdvr_tf* tf, const 
             (prm->shading != HIPDVR_SHADE_NONE && prm->gradient != HIPDVR_GRADIENT_CENTRAL))
             fail(HIPDVR_ERR_UNSUPPORTED, "the sparse re-march serves the learned path's options");
         const hipdvr_learned_opts opts = default_opts(o);
+        const SparseGrid need = sparse_dims(w, h, (uint32_t)opts.remarch_stride);
+        if ((uint64_t)need.w * (uint64_t)need.h > capacity)
+            fail(HIPDVR_ERR_INVALID, "d_sparse holds " + std::to_string(capacity) + " pixels, the grid needs " +
+                                         std::to_string(need.w) + " x " + std::to_string(need.h));
         ctx_set_device(l->ctx);
         hipStream_t st = ctx_stream(l->ctx);
         hipdvr_volume* vol = const_cast<hipdvr_volume*>(l->vol);

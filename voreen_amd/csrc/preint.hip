@@ -37,7 +37,10 @@ __global__ __launch_bounds__(256) void preint_exact_kernel(const float4* __restr
         const float scale = 1.0f / (fabsf((float)(sb - sf)) + 1);
         const int incr = (sb < sf) ? -1 : 1;
         float r = 0.f, g = 0.f, b = 0.f, a = 0.f;
-        for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && (a < 0.95f); s += incr) {
+        // the reference compares the float alpha with the double 0.95
+        // (preintegrationtable.cpp:115); 0.95f < 0.95 < the next float up, so
+        // that test is a <= 0.95f, without double arithmetic
+        for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && (a <= 0.95f); s += incr) {
             const float4 cc = tfb[s];
             float ca = cc.w;
             if (ca > 0.0f) {

@@ -138,7 +138,9 @@ extern "C" int hipdvr_preint_build(const hipdvr_tf_key* keys, int32_t nkeys, flo
                     const float scale = 1.0f / (std::fabs((float)(sb - sf)) + 1);
                     const int incr = (sb < sf) ? -1 : 1;
                     float r = 0.f, g = 0.f, b = 0.f, a = 0.f;
-                    for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && (a < 0.95f); s += incr) {
+                    // the float alpha against the double 0.95 (preintegrationtable.cpp:115):
+                    // a composite alpha of exactly 0.95f composites one more sample
+                    for (int s = sf; (incr == 1 ? s <= sb : s >= sb) && ((double)a < 0.95); s += incr) {
                         const float* cc = &tfb[4 * (size_t)s];
                         float ca = cc[3];
                         if (ca > 0.0f) {

@@ -43,7 +43,7 @@ EXPORTS = (
     "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
     "hipdvr_learned_create", "hipdvr_learned_destroy", "hipdvr_learned_prepare",
     "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
-    "hipdvr_render_learned_device", "hipdvr_learned_sparse_remarch",
+    "hipdvr_render_learned_device", "hipdvr_learned_sparse_remarch", "hipdvr_learned_sparse_dims",
     "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
     "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
@@ -180,8 +180,9 @@ def load():
         "hipdvr_learned_collect": ([vp, C.c_float, C.c_float, vp, vp, u32, P(u32)], C.c_int),
         "hipdvr_render_learned_device": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
                                           P(LearnedStats)], C.c_int),
-        "hipdvr_learned_sparse_remarch": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp, P(u32),
-                                           P(u32)], C.c_int),
+        "hipdvr_learned_sparse_remarch": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
+                                           C.c_uint64, P(u32), P(u32)], C.c_int),
+        "hipdvr_learned_sparse_dims": ([u32, u32, P(LearnedOpts), P(u32), P(u32)], C.c_int),
         "hipdvr_vvd_parse": ([C.c_char_p, P(VolumeDesc), C.c_char_p, u32], C.c_int),
         "hipdvr_vvd_last_error": ([], C.c_char_p),
         "hipdvr_volume_load_raw": ([vp, C.c_char_p, P(VolumeDesc), C.c_uint64, P(vp)], C.c_int),
@@ -663,12 +664,24 @@ class Learned:
         return st.as_dict() if stats else None
 
     def sparse_remarch(self, tf, d_entry, d_exit, w, h, params, d_sparse, opts=None):
-        """The judge's plain re-march alone (hipdvr_learned_sparse_remarch): (sw, sh)."""
+        """The judge's plain re-march alone (hipdvr_learned_sparse_remarch) into
+        the device buffer d_sparse (its size bounds the grid): (sw, sh)."""
         sw, sh = C.c_uint32(), C.c_uint32()
         self.ctx.check(self.ctx.lib.hipdvr_learned_sparse_remarch(
             self.h, tf.h, d_entry.ptr, d_exit.ptr, w, h, C.byref(params), C.byref(opts) if opts else None,
-            d_sparse.ptr, C.byref(sw), C.byref(sh)), "hipdvr_learned_sparse_remarch")
+            d_sparse.ptr, d_sparse.nbytes // 16, C.byref(sw), C.byref(sh)), "hipdvr_learned_sparse_remarch")
         return sw.value, sh.value
+
+
+def learned_sparse_dims(w, h, remarch_stride=0):
+    """The judge's sparse grid size for a w x h image (host arithmetic)."""
+    o = LearnedOpts()
+    o.remarch_stride = remarch_stride
+    sw, sh = C.c_uint32(), C.c_uint32()
+    rc = load().hipdvr_learned_sparse_dims(w, h, C.byref(o), C.byref(sw), C.byref(sh))
+    if rc != OK:
+        raise HipDvrError(rc, "hipdvr_learned_sparse_dims")
+    return sw.value, sh.value
 
 
 class Batch:

@@ -335,6 +335,15 @@ void HIPRaycaster::writeOutput(const std::vector<float>& rgba, int w, int h) {
     outport1_.deactivateTarget();
 }
 
+float HIPRaycaster::renderStepSize(const VolumeRAM* vram) const {
+    // the sampling rate over the largest dimension, as CPURaycaster computes
+    // it (cpuraycaster.cpp:104), in interaction mode too: it does not call
+    // VolumeRaycaster::getSamplingStepSize, whose interaction mode divides by
+    // interactionQuality (volumeraycaster.cpp:312-322)
+    const tgt::svec3 volDim = vram->getDimensions();
+    return 1.f / (tgt::max(volDim) * samplingRate_.get());
+}
+
 void HIPRaycaster::process() {
     const VolumeBase* vb = volumeInport_.getData();
     const VolumeRAM* vram = vb ? vb->getRepresentation<VolumeRAM>() : 0;
@@ -386,8 +395,7 @@ void HIPRaycaster::process() {
         return;
     }
 
-    // sampling step, volumeraycaster.cpp:312-322 (CPURaycaster: cpuraycaster.cpp:104)
-    const float step = getSamplingStepSize(vb);
+    const float step = renderStepSize(vram);
 
     hipdvr_params params;
     fillParams(params, vb, step);

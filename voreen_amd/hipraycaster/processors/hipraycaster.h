@@ -66,6 +66,10 @@ protected:
     virtual void initialize();
     virtual void deinitialize();
 
+    /// The march step: 1/(max(dim)*samplingRate), CPURaycaster's rule
+    /// (cpuraycaster.cpp:104) in every quality mode.
+    float renderStepSize(const VolumeRAM* vram) const;
+
 private:
     bool ensureContext();
     bool ensureVolume(const VolumeBase* vb, const VolumeRAM* vram);

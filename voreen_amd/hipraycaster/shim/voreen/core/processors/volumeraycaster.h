@@ -5,6 +5,7 @@
 // volumerenderer.cpp:41-54, src/core/utils/glsl.cpp:397-409,
 // classificationmodes.cpp:84-90).
 #pragma once
+#include "voreen/core/utils/voreenqualitymode.h"
 #include "voreen/core/ports/volumeport.h"
 #include "voreen/core/properties/optionproperty.h"
 
@@ -79,10 +80,13 @@ public:
         addProperty(interactionQuality_);
     }
 
-    // volumeraycaster.cpp:312-322 (outside interaction mode)
+    // volumeraycaster.cpp:312-322
     float getSamplingStepSize(const VolumeBase* vol) {
         const tgt::svec3 dim = vol->getDimensions();
-        return 1.f / ((float)tgt::max(dim) * samplingRate_.get());
+        float samplingStepSize = 1.f / ((float)tgt::max(dim) * samplingRate_.get());
+        if (QualityMode.isInteractionMode())
+            samplingStepSize /= interactionQuality_.get();
+        return samplingStepSize;
     }
 
 protected:

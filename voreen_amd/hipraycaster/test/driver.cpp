@@ -26,6 +26,7 @@ struct Exposed : public HIPRaycaster {
     RenderPort& xin() { return exitPort_; }
     RenderPort& out() { return outport1_; }
     float step(const VolumeBase* v) { return getSamplingStepSize(v); }
+    float renderStep(const VolumeBase* v) { return renderStepSize(v->getRepresentation<VolumeRAM>()); }
 };
 
 struct Driver {
@@ -169,6 +170,15 @@ int hrc_connect(void* h, int entry, int exit_) {
 }
 
 int hrc_is_ready(void* h) { return ((Driver*)h)->proc->isReady() ? 1 : 0; }
+
+// QualityMode.setInteractionMode (the network evaluator's interaction flag)
+void hrc_set_interaction(int on) { QualityMode.setInteractionMode(on != 0); }
+
+// the step HIPRaycaster marches with
+float hrc_render_step(void* h) {
+    Driver* d = (Driver*)h;
+    return d->volume ? d->proc->renderStep(d->volume.get()) : -1.f;
+}
 
 float hrc_step(void* h) {
     Driver* d = (Driver*)h;
