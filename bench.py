@@ -205,6 +205,27 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
     }
 
 
+def host_path_leg(ctx, v, t, bufs, params, wimg, himg, kernel_ms, reps=20):
+    """What HIPRaycaster::process pays per frame: hipdvr_render with the
+    entry/exit images and the output in pageable host memory (numpy arrays),
+    view 0, the staging buffers kept by the context after the first frame."""
+    de, dx, _ = bufs[0]
+    en = de.download((himg, wimg, 4), np.float32)
+    ex = dx.download((himg, wimg, 4), np.float32)
+    out = np.empty((himg, wimg, 4), np.float32)
+    ctx.render(v, t, en, ex, params[0], stats=False, out=out)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.render(v, t, en, ex, params[0], stats=False, out=out)
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
+    return {"fps": round(1e3 / ms, 3), "ms_per_frame": round(ms, 3), "frames": reps,
+            "march_kernel_ms": round(kernel_ms, 3), "transfer_and_host_ms": round(ms - kernel_ms, 3),
+            "sample": f"view 0, median of {reps} hipdvr_render calls: {2 * wimg * himg * 16 >> 20} MiB up, "
+                      f"{wimg * himg * 16 >> 20} MiB down per frame"}
+
+
 def main_batch(args):
     """64 uneven volumes (log-uniform 64..384 per axis, u8/u16, blob volumes,
     own TF and orbit camera), 512^2 each, one persistent launch replayed from
@@ -581,6 +602,7 @@ def main():
         a.free()
         b.free()
         out["copy_calibration_bytes"] = nb
+    out["host_path"] = host_path_leg(ctx, v, t, bufs, params, wimg, himg, launch_ms[0])
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, fps, samples / nviews,
                                      args)

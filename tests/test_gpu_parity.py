@@ -160,6 +160,40 @@ def test_tile_orders_write_every_pixel(ctx, oracle, size):
         t.close()
 
 
+def test_host_buffer_render_keeps_staging_across_sizes(ctx, oracle):
+    """hipdvr_render (host buffers, the processor's path) keeps its device and
+    pinned staging between frames and grows it for a larger frame: frames of
+    256^2, 64x40, 300x280 and 256^2 again equal the device-buffer render."""
+    from voreen_amd import hipdvr as H, workloads as W
+    vol = W.make_volume("c0")
+    _, tf = W.make_tf("c0")
+    llf, urb = W.world_box((128, 128, 128))
+    v = ctx.volume(vol)
+    t = ctx.transfunc(tf)
+    try:
+        for k, (w, h) in enumerate(((256, 256), (64, 40), (300, 280), (256, 256))):
+            say(f"host-buffer render {w}x{h}")
+            cam = W.orbit_camera(3 + k, 36)
+            en, ex = oracle.entry_exit(cam.eye, cam.focus, cam.up, 30.0, 0.1, w, h, llf=llf, urb=urb)
+            p = H.default_params(W.step_for((128, 128, 128), 2.0))
+            img, st = ctx.render(v, t, en, ex, p)
+            de, dx, do = (ctx.alloc(w * h * 16) for _ in range(3))
+            try:
+                de.upload(en)
+                dx.upload(ex)
+                st2 = ctx.render_device(v, t, de, dx, w, h, p, do, stats=True)
+                dev = do.download((h, w, 4), np.float32)
+            finally:
+                for b in (de, dx, do):
+                    b.free()
+            assert np.array_equal(img, dev) and st["samples"] == st2["samples"]
+            cpu, n = oracle.render(vol, tf, (0.0, 1.0), en, ex, p.step)
+            assert np.array_equal(img, cpu) and st["samples"] == n
+    finally:
+        v.close()
+        t.close()
+
+
 def test_c0_full_frame(ctx, oracle, kernel):
     say(f"c0 128^3 u8 256^2 step 1/256 linear kernel {kernel}")
     vol, tf, en, ex, step = c0_inputs(oracle)

@@ -15,6 +15,7 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hipdvr.h"
@@ -44,6 +45,43 @@ uint64_t fbits(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
     return u;
+}
+
+// Grow the context's staging of hipdvr_render to `bytes` device and pinned
+// host bytes (kept between frames; only a larger frame reallocates).
+void ensure_stage(hipdvr_ctx* ctx, size_t bytes) {
+    if (ctx->stage_bytes >= bytes) return;
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    ctx->d_stage = nullptr;
+    ctx->h_stage = nullptr;
+    ctx->stage_bytes = 0;
+    hip_check(hipMalloc((void**)&ctx->d_stage, bytes), "hipMalloc(render staging)");
+    hip_check(hipHostMalloc((void**)&ctx->h_stage, bytes, hipHostMallocDefault), "hipHostMalloc(render staging)");
+    ctx->stage_bytes = bytes;
+}
+
+// memcpy of a frame between pageable and pinned host memory, split over up
+// to 8 host threads in 64-byte aligned pieces (one thread below 1 MiB).
+void host_copy(void* dst, const void* src, size_t bytes) {
+    const size_t kMin = 1u << 20;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 8 ? 8 : nt);
+    if (bytes < kMin || nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t piece = ((bytes + nt - 1) / nt + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) {
+        const size_t b = piece * t;
+        if (b >= bytes) break;
+        const size_t n = bytes - b < piece ? bytes - b : piece;
+        th.emplace_back([=] { std::memcpy((char*)dst + b, (const char*)src + b, n); });
+    }
+    std::memcpy(dst, src, piece < bytes ? piece : bytes);
+    for (auto& x : th) x.join();
 }
 
 }  // namespace
@@ -382,7 +420,10 @@ int hipdvr_destroy(hipdvr_ctx* ctx) {
     if (!ctx) return HIPDVR_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
@@ -561,24 +602,21 @@ int hipdvr_render(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_tf* tf
         if (!entry || !exit_ || !out) fail(HIPDVR_ERR_INVALID, "NULL argument");
         if (w == 0 || h == 0) fail(HIPDVR_ERR_INVALID, "empty image");
         set_device(ctx);
-        const size_t bytes = (size_t)w * h * 4 * sizeof(float);
-        float *de = nullptr, *dx = nullptr, *dout = nullptr;
-        struct Bufs {
-            float **a, **b, **c;
-            ~Bufs() {
-                if (*a) (void)hipFree(*a);
-                if (*b) (void)hipFree(*b);
-                if (*c) (void)hipFree(*c);
-            }
-        } guard{&de, &dx, &dout};
-        hip_check(hipMalloc((void**)&de, bytes), "hipMalloc(entry)");
-        hip_check(hipMalloc((void**)&dx, bytes), "hipMalloc(exit)");
-        hip_check(hipMalloc((void**)&dout, bytes), "hipMalloc(out)");
-        hip_check(hipMemcpyAsync(de, entry, bytes, hipMemcpyHostToDevice, ctx->stream), "upload entry");
-        hip_check(hipMemcpyAsync(dx, exit_, bytes, hipMemcpyHostToDevice, ctx->stream), "upload exit");
+        const size_t px = (size_t)w * h * 4, bytes = px * sizeof(float);
+        ensure_stage(ctx, 3 * bytes);
+        float *de = ctx->d_stage, *dx = de + px, *dout = dx + px;
+        float *pe = ctx->h_stage, *pxb = pe + px, *po = pxb + px;
+        // the caller's pageable buffers through the pinned ones: each copy
+        // into pinned memory is split over host threads, and the entry
+        // upload runs while the exit points are copied
+        host_copy(pe, entry, bytes);
+        hip_check(hipMemcpyAsync(de, pe, bytes, hipMemcpyHostToDevice, ctx->stream), "upload entry");
+        host_copy(pxb, exit_, bytes);
+        hip_check(hipMemcpyAsync(dx, pxb, bytes, hipMemcpyHostToDevice, ctx->stream), "upload exit");
         render_device_impl(ctx, vol, tf, de, dx, w, h, prm, dout, stats);
-        hip_check(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream), "download out");
+        hip_check(hipMemcpyAsync(po, dout, bytes, hipMemcpyDeviceToHost, ctx->stream), "download out");
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        host_copy(out, po, bytes);
     });
 }
 

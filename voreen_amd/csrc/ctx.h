@@ -22,6 +22,12 @@ struct hipdvr_ctx {
     // per-launch timing (hipdvr_timing_begin/read)
     std::vector<hipEvent_t> tev;                 // 2 per slot
     uint32_t tcap = 0, tcount = 0;
+    // staging of hipdvr_render (host entry/exit in, host image out): the
+    // device copies of entry, exit and output, and pinned host buffers of
+    // the same size; grown when a frame is larger, freed with the context
+    float* d_stage = nullptr;
+    float* h_stage = nullptr;
+    size_t stage_bytes = 0;                      // each of the two, 3 images
 };
 
 namespace hipdvr {

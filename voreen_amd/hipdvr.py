@@ -357,18 +357,21 @@ class Context:
                                                C.byref(t.h)), "hipdvr_tf2d_create")
         return t
 
-    def synth_turbulence(self, seed, dims, empty_fraction=0.4):
+    def synth_turbulence(self, seed, dims, empty_fraction=0.4, out=None):
         """uint16 turbulence volume made on the device (hipdvr_synth_turbulence_device):
-        (device buffer, exact empty fraction); dims = (x, y, z)."""
+        (device buffer, exact empty fraction); dims = (x, y, z); into `out`
+        when given, else a new buffer."""
         nx, ny, nz = dims
-        buf = self.alloc(nx * ny * nz * 2)
+        buf = out if out is not None else self.alloc(nx * ny * nz * 2)
+        if buf.nbytes < nx * ny * nz * 2:
+            raise ValueError("output buffer too small")
         frac = C.c_double()
+        fn = "hipdvr_synth_turbulence_device"
         try:
-            self.check(self.lib.hipdvr_synth_turbulence_device(self.h, seed, nx, ny, nz, empty_fraction,
-                                                               buf.ptr, C.byref(frac)),
-                       "hipdvr_synth_turbulence_device")
+            self.check(getattr(self.lib, fn)(self.h, seed, nx, ny, nz, empty_fraction, buf.ptr, C.byref(frac)), fn)
         except Exception:
-            buf.free()
+            if out is None:
+                buf.free()
             raise
         return buf, frac.value
 
@@ -427,13 +430,18 @@ class Context:
         self.check(self.lib.hipdvr_copy_device(self.h, dst.ptr, src.ptr, nbytes), "hipdvr_copy_device")
 
     # ---- rendering ----
-    def render(self, vol, tf, entry, exit_, params, stats=True):
+    def render(self, vol, tf, entry, exit_, params, stats=True, out=None):
+        """hipdvr_render: host entry/exit images in, host image out (into
+        `out` when given, a C-contiguous (H, W, 4) float32 array)."""
         entry = np.ascontiguousarray(entry, np.float32)
         exit_ = np.ascontiguousarray(exit_, np.float32)
         if entry.shape != exit_.shape or entry.ndim != 3 or entry.shape[2] != 4:
             raise ValueError("entry/exit must be (H, W, 4) float32 arrays of equal shape")
         h, w = entry.shape[:2]
-        out = np.zeros((h, w, 4), np.float32)
+        if out is None:
+            out = np.zeros((h, w, 4), np.float32)
+        elif out.shape != entry.shape or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of the entry image's shape")
         st = Stats()
         rc = self.lib.hipdvr_render(self.h, vol.h, tf.h, _ptr(entry), _ptr(exit_), w, h,
                                     C.byref(params), _ptr(out), C.byref(st) if stats else None)
