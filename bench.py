@@ -44,7 +44,7 @@ def parse():
                          "c4: 8 GiB turbulence volume, 8 views of 3840x2160 in one batched launch")
     ap.add_argument("--jobs", type=int, default=64, help="c2: number of volumes")
     ap.add_argument("--no-autotune", action="store_true", help="c2: keep the default configuration")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "bricked", "cell", "slab"])
     ap.add_argument("--brick-log2", type=int, default=4, choices=[2, 3, 4, 5],
                     help="empty-space brick size of the volume (2^n voxels)")
     ap.add_argument("--batch", type=int, default=0, choices=[0, 1, 2, 4, 8],
@@ -487,7 +487,7 @@ def main():
     wimg, himg = cfg["image"]
     nviews = args.views or cfg["views"]
     kernel = {"auto": H.KERNEL_AUTO, "reference": H.KERNEL_REFERENCE,
-              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL}[args.kernel]
+              "bricked": H.KERNEL_BRICKED, "cell": H.KERNEL_CELL, "slab": H.KERNEL_SLAB}[args.kernel]
     t_setup = time.perf_counter()
     vol = W.make_volume(args.config)
     keys, tf = W.make_tf(args.config)

@@ -145,8 +145,11 @@ enum {
     HIPDVR_KERNEL_AUTO = 0,      /* fastest exact kernel for the options */
     HIPDVR_KERNEL_REFERENCE = 1, /* linear layout, one lane per ray, no skipping */
     HIPDVR_KERNEL_BRICKED = 2,   /* bricked layout + exact empty-space skipping */
-    HIPDVR_KERNEL_CELL = 3       /* corner-replicated cell records + exact empty-space
+    HIPDVR_KERNEL_CELL = 3,      /* corner-replicated cell records + exact empty-space
                                     skipping; linear filter, TF classification */
+    HIPDVR_KERNEL_SLAB = 4       /* the linear volume staged into LDS per image tile, one
+                                    slab of planes at a time + exact empty-space skipping;
+                                    linear filter, TF classification */
 };
 
 typedef struct hipdvr_ctx hipdvr_ctx;

@@ -103,6 +103,48 @@ int hipdvr_learned_trace(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_pa
                          const float* d_entry, const float* d_exit, uint32_t w, uint32_t h);
 int hipdvr_learned_collect(hipdvr_learned* l, float tau_skip, float tau_step, float* d_x, float* d_t,
                            uint32_t capacity, uint32_t* count);
+/* Training of the network for one volume, the whole loop on the device:
+ * for each of n_tfs seeded random transfer functions (4-8 keys, intensities
+ * U[0,1], RGBA8 U{0..255}) and each of the caller's extra tables, trace the
+ * volume plainly from n_views orbit cameras (radius 3.5 around the proxy box,
+ * fovy 30, elevations cycling 10..60 degrees) at trace_size^2, collect the
+ * visited rows with targets at tau_skip / tau_step, then take
+ * max(min_steps, epochs * rows / batch) Adam steps at rate lr on random
+ * contiguous minibatches.  Zero fields take the defaults in brackets.
+ * Replaces the learned state's last prepare (call hipdvr_learned_prepare
+ * again before rendering).  Synchronous. */
+typedef struct hipdvr_learned_train_opts {
+    uint32_t n_tfs;          /* random TFs traced [200] */
+    uint32_t n_views;        /* orbit views per TF [16] */
+    uint32_t trace_size;     /* trace image side in pixels [192] */
+    float sampling_rate;     /* step 1/(max(dim)*rate) [2] */
+    float tau_skip;          /* skip-label error threshold [3e-3] */
+    float tau_step;          /* step-label error threshold [8e-3] */
+    uint32_t min_steps;      /* Adam steps at least [2000] */
+    uint32_t epochs;         /* passes over the rows at most [4] */
+    uint32_t batch;          /* minibatch rows [8192] */
+    float lr;                /* Adam learning rate [2e-3] */
+    uint64_t seed;           /* TF and minibatch seed [0x5EED] */
+    const uint8_t* extra_tfs; /* n_extra RGBA8 tables of extra_width entries, traced after the random ones */
+    uint32_t n_extra;
+    uint32_t extra_width;    /* [256 for uint8 volumes, 4096 for uint16] */
+    uint64_t max_rows;       /* training rows at most [24,000,000] */
+} hipdvr_learned_train_opts;
+
+typedef struct hipdvr_learned_train_info {
+    uint64_t rows;           /* training rows collected */
+    uint32_t tfs;            /* tables traced */
+    uint32_t views;
+    uint32_t trace_size;
+    uint32_t steps;          /* Adam steps taken */
+    float loss_first, loss_last;
+    float trace_seconds;     /* tracing and collecting */
+    float seconds;           /* the whole call */
+} hipdvr_learned_train_info;
+
+int hipdvr_learned_train(hipdvr_learned* l, hipdvr_mlp* mlp, const hipdvr_learned_train_opts* opts,
+                         hipdvr_learned_train_info* info);
+
 /* Render with learned stepping; the sparse plain re-march judges it and the
  * frame falls back to the plain marcher below the PSNR floor.  Asynchronous
  * unless stats != NULL. */

@@ -39,6 +39,7 @@ def lib():
         l.hrc_process.argtypes = [vp, vp, C.c_int, C.c_int]
         l.hrc_last_error.argtypes = [vp, C.c_char_p, C.c_int]
         l.hrc_stats.argtypes = [vp, vp]
+        l.hrc_learned_stats.argtypes = [vp, vp]
         l.hrc_invalidation.argtypes = [vp]
         _lib = l
     return _lib
@@ -136,6 +137,13 @@ class Raycaster:
         st = H.Stats()
         lib().hrc_stats(self.h, C.byref(st))
         return st.as_dict()
+
+    def learned_stats(self):
+        """(learned, stats dict) of the last frame."""
+        from voreen_amd import hipdvr as H
+        st = H.LearnedStats()
+        learned = lib().hrc_learned_stats(self.h, C.byref(st))
+        return bool(learned), st.as_dict()
 
     def invalidation(self):
         return lib().hrc_invalidation(self.h)

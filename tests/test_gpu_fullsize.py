@@ -78,12 +78,13 @@ def c1_view(ctx, c, view):
     return cam, en, ex
 
 
-@pytest.mark.parametrize("view", [0, 18])
-def test_c1_unshaded_full_frame_equals_oracle(ctx, oracle, c1, view):
+@pytest.mark.parametrize("view,kernel", [(0, "auto"), (18, "auto"), (0, "slab"), (18, "cell")])
+def test_c1_unshaded_full_frame_equals_oracle(ctx, oracle, c1, view, kernel):
     from voreen_amd import hipdvr as H
-    say(f"c1 512^3 u16 1024^2 view {view} unshaded, every pixel")
+    say(f"c1 512^3 u16 1024^2 view {view} unshaded, every pixel, kernel {kernel}")
     cam, en, ex = c1_view(ctx, c1, view)
-    p = H.default_params(c1["step"], camera_pos=tuple(cam.eye))
+    k = {"auto": H.KERNEL_AUTO, "slab": H.KERNEL_SLAB, "cell": H.KERNEL_CELL}[kernel]
+    p = H.default_params(c1["step"], camera_pos=tuple(cam.eye), kernel=k)
     st = ctx.render_device(c1["v"], c1["t"], c1["de"], c1["dx"], c1["w"], c1["h"], p, c1["do"], stats=True)
     gpu = c1["do"].download((c1["h"], c1["w"], 4), np.float32)
     t0 = time.perf_counter()
@@ -95,15 +96,17 @@ def test_c1_unshaded_full_frame_equals_oracle(ctx, oracle, c1, view):
     assert d.max() <= TOL
     assert np.array_equal(gpu, cpu)
     assert st["samples"] == n
-    assert np.mean(cpu[..., 3] > 0) > 0.3
+    assert np.mean(cpu[..., 3] > 0) > 0.15
 
 
-@pytest.mark.parametrize("exact", [0, 1])
-def test_c1_phong_every_16th_pixel(ctx, oracle, c1, exact):
+@pytest.mark.parametrize("exact,kernel", [(0, "auto"), (1, "auto"), (0, "slab"), (1, "slab")])
+def test_c1_phong_every_16th_pixel(ctx, oracle, c1, exact, kernel):
     from voreen_amd import hipdvr as H
-    say(f"c1 512^3 u16 1024^2 view 4 phong exact_shading={exact}, every 16th pixel")
+    say(f"c1 512^3 u16 1024^2 view 4 phong exact_shading={exact}, every 16th pixel, kernel {kernel}")
     cam, en, ex = c1_view(ctx, c1, 4)
-    p = H.default_params(c1["step"], shading=H.SHADE_PHONG, exact_shading=exact, camera_pos=tuple(cam.eye))
+    k = {"auto": H.KERNEL_AUTO, "slab": H.KERNEL_SLAB}[kernel]
+    p = H.default_params(c1["step"], shading=H.SHADE_PHONG, exact_shading=exact, camera_pos=tuple(cam.eye),
+                         kernel=k)
     st = ctx.render_device(c1["v"], c1["t"], c1["de"], c1["dx"], c1["w"], c1["h"], p, c1["do"], stats=True)
     gpu = c1["do"].download((c1["h"], c1["w"], 4), np.float32)[::16, ::16]
     sub_e = np.ascontiguousarray(en[::16, ::16])
@@ -116,7 +119,7 @@ def test_c1_phong_every_16th_pixel(ctx, oracle, c1, exact):
           f"psnr_fp32={psnr(gpu, cpu):.1f} psnr_fp64={psnr(gpu, f64):.1f} shaded={st['shaded']}", flush=True)
     assert np.mean(d > TOL) < 0.002
     assert psnr(gpu, f64) > 45.0
-    assert st["shaded"] > 0 and np.mean(cpu[..., 3] > 0) > 0.3
+    assert st["shaded"] > 0 and np.mean(cpu[..., 3] > 0) > 0.15
 
 
 # ---------------------------------------------------------------------------

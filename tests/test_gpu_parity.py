@@ -22,17 +22,18 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-3
 
 
-@pytest.fixture(params=["reference", "bricked", "cell"])
+@pytest.fixture(params=["reference", "bricked", "cell", "slab"])
 def kernel(request):
     from voreen_amd import hipdvr as H
     return {"reference": H.KERNEL_REFERENCE, "bricked": H.KERNEL_BRICKED,
-            "cell": H.KERNEL_CELL}[request.param]
+            "cell": H.KERNEL_CELL, "slab": H.KERNEL_SLAB}[request.param]
 
 
 def filters_for(kernel):
-    """Filter modes a kernel serves (the cell records hold trilinear footprints)."""
+    """Filter modes a kernel serves (the cell records hold trilinear
+    footprints; the slab kernel reconstructs trilinear samples only)."""
     from voreen_amd import hipdvr as H
-    return (1,) if kernel == H.KERNEL_CELL else (0, 1)
+    return (1,) if kernel in (H.KERNEL_CELL, H.KERNEL_SLAB) else (0, 1)
 
 
 def say(msg):
@@ -145,7 +146,7 @@ def test_tile_orders_write_every_pixel(ctx, oracle, size):
     try:
         de.upload(en)
         dx.upload(ex)
-        for kernel in (H.KERNEL_CELL, H.KERNEL_BRICKED):
+        for kernel in (H.KERNEL_CELL, H.KERNEL_BRICKED, H.KERNEL_SLAB):
             say(f"tile orders {w}x{h} kernel {kernel}")
             do.upload(np.full((h, w, 4), np.nan, np.float32))
             st = ctx.render_device(v, t, de, dx, w, h, H.default_params(step, kernel=kernel), do, stats=True)
@@ -298,7 +299,7 @@ def test_shading_models(ctx, oracle, model):
     off = tuple(float(llf[i] + 0.5 / 64) for i in range(3))
     name = H.SHADING_KEYS[model]
     for kernel, filt in ((H.KERNEL_REFERENCE, 0), (H.KERNEL_REFERENCE, 1), (H.KERNEL_CELL, 1),
-                         (H.KERNEL_AUTO, 1), (H.KERNEL_AUTO, 0)):
+                         (H.KERNEL_SLAB, 1), (H.KERNEL_AUTO, 1), (H.KERNEL_AUTO, 0)):
         say(f"shading {name} kernel {kernel} filter {filt} 64^3 u16 96^2")
         r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, W.step_for(dims, 2.0), shading=model,
                         filter_=filt, spacing=sp, offset=off, camera_pos=tuple(cam.eye), kernel=kernel)
@@ -453,7 +454,7 @@ def test_compositing_modes_reject_skipping_kernels(ctx):
     en = np.zeros((2, 2, 4), np.float32)
     say("compositing modes: invalid combinations")
     try:
-        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL):
+        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL, H.KERNEL_SLAB):
             with pytest.raises(H.HipDvrError) as e:
                 ctx.render(v, t, en, en, H.default_params(0.1, kernel=k, compositing=H.COMPOSITE_MIP))
             assert e.value.code == H.ERR_UNSUPPORTED
@@ -665,7 +666,7 @@ def test_gradient_modes(ctx, oracle, mode, name):
     v = ctx.volume(vol)
     t = ctx.transfunc(tf)
     try:
-        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL):
+        for k in (H.KERNEL_BRICKED, H.KERNEL_CELL, H.KERNEL_SLAB):
             with pytest.raises(H.HipDvrError) as e:
                 ctx.render(v, t, en, ex, H.default_params(step, kernel=k, shading=1, gradient=mode))
             assert e.value.code == H.ERR_UNSUPPORTED
@@ -764,7 +765,7 @@ def test_tf2d_rejections(ctx, oracle):
                 ctx.render(v, t, en, ex, H.default_params(step, **kw))
             assert e.value.code == code
         v.set_gradient(grad)
-        for kw in (dict(kernel=H.KERNEL_CELL), dict(kernel=H.KERNEL_BRICKED),
+        for kw in (dict(kernel=H.KERNEL_CELL), dict(kernel=H.KERNEL_BRICKED), dict(kernel=H.KERNEL_SLAB),
                    dict(compositing=H.COMPOSITE_MIP)):
             with pytest.raises(H.HipDvrError) as e:
                 ctx.render(v, t2, en, ex, H.default_params(step, classification=H.CLASS_TF2D, **kw))

@@ -145,7 +145,7 @@ void validate_params(const hipdvr_params* p, const hipdvr_tf* tf) {
         fail(HIPDVR_ERR_INVALID, "gamma and iso_value must be finite");
     if (p->shading < HIPDVR_SHADE_NONE || p->shading > HIPDVR_SHADE_WARD)
         fail(HIPDVR_ERR_UNSUPPORTED, "unsupported shading mode");
-    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_CELL)
+    if (p->kernel < HIPDVR_KERNEL_AUTO || p->kernel > HIPDVR_KERNEL_SLAB)
         fail(HIPDVR_ERR_INVALID, "unknown kernel selector");
     if (p->batch != 0 && p->batch != 1 && p->batch != 2 && p->batch != 4 && p->batch != 8)
         fail(HIPDVR_ERR_INVALID, "batch must be 0, 1, 2, 4 or 8");
@@ -283,10 +283,11 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         else kernel = HIPDVR_KERNEL_REFERENCE;
     }
     const bool cell = kernel == HIPDVR_KERNEL_CELL;
-    if ((kernel == HIPDVR_KERNEL_BRICKED || cell) && p->classification != HIPDVR_CLASS_TF)
+    const bool slab = kernel == HIPDVR_KERNEL_SLAB;
+    if ((kernel == HIPDVR_KERNEL_BRICKED || cell || slab) && p->classification != HIPDVR_CLASS_TF)
         fail(HIPDVR_ERR_UNSUPPORTED, "the skipping kernels support transfer-function classification only");
-    if (cell && p->filter != HIPDVR_FILTER_LINEAR)
-        fail(HIPDVR_ERR_UNSUPPORTED, "the cell kernel supports the linear filter only");
+    if ((cell || slab) && p->filter != HIPDVR_FILTER_LINEAR)
+        fail(HIPDVR_ERR_UNSUPPORTED, "the cell and slab kernels support the linear filter only");
     if (kernel == HIPDVR_KERNEL_BRICKED && p->shading != HIPDVR_SHADE_NONE && p->shading != HIPDVR_SHADE_PHONG)
         fail(HIPDVR_ERR_UNSUPPORTED, "the bricked kernel shades with Phong only");
     hipdvr_volume* mvol = const_cast<hipdvr_volume*>(vol);
@@ -295,7 +296,7 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         ensure_tf2d(mtf, p->step, ctx->stream);
         a.tf2d = mtf->d_tf2d;
     }
-    if (kernel == HIPDVR_KERNEL_BRICKED || cell) ensure_occupancy(ctx, mvol, tf);
+    if (kernel == HIPDVR_KERNEL_BRICKED || cell || slab) ensure_occupancy(ctx, mvol, tf);
     if (cell) {
         ensure_cells(mvol, ctx->stream);
         a.cells = mvol->d_cells;
@@ -306,6 +307,9 @@ void render_device_impl(hipdvr_ctx* ctx, const hipdvr_volume* vol, const hipdvr_
         launch_march_modes(a, vol->desc.dtype, p->filter, phong, ctx->stream);
     } else if (kernel == HIPDVR_KERNEL_REFERENCE) {
         launch_march_ref(a, vol->desc.dtype, p->filter, p->classification, phong, count, ctx->stream);
+    } else if (slab) {
+        const int shade = phong ? ((p->exact_shading || p->shading != HIPDVR_SHADE_PHONG) ? 2 : 1) : 0;
+        launch_march_slab(a, vol->desc.dtype, shade, count, ctx->stream);
     } else if (kernel == HIPDVR_KERNEL_CELL) {
         // samples in flight per ray: default_batch (kernels.h)
         const int shade = phong ? ((p->exact_shading || p->shading != HIPDVR_SHADE_PHONG) ? 2 : 1) : 0;

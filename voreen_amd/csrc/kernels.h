@@ -30,6 +30,9 @@ void launch_march_fast(const MarchArgs& a, int dtype, int filter, bool phong, bo
 // ray (2 or 4)
 void launch_march_cell(const MarchArgs& a, int dtype, int shade, bool count, bool wide, int batch,
                        hipStream_t st);
+// The slab-staged march (march_slab.h): linear filter, TF classification,
+// DVR; shade as for the cell kernel.  Reads the linear volume.
+void launch_march_slab(const MarchArgs& a, int dtype, int shade, bool count, hipStream_t st);
 // Samples in flight per ray by default, measured on 512^3 / 1024^2: four
 // unshaded (1.00 ms against 1.04 with two and 1.28 with one); two shaded
 // (1.88 ms against 2.01 with four and 2.18 with one, Phong within

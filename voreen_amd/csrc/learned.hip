@@ -224,6 +224,11 @@ struct hipdvr_learned {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
+namespace hipdvr {
+hipdvr_ctx* learned_ctx(const hipdvr_learned* l) { return l->ctx; }
+const hipdvr_volume* learned_volume(const hipdvr_learned* l) { return l->vol; }
+}  // namespace hipdvr
+
 namespace {
 
 // The judge's sparse pixels, inside the image, and device buffers sized for

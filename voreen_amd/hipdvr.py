@@ -24,7 +24,7 @@ COMPOSITE_DVR, COMPOSITE_MIP, COMPOSITE_MOP, COMPOSITE_MIDA, COMPOSITE_ISO, COMP
 SHADING_KEYS = ("none", "phong", "phong-diffuse", "phong-specular", "phong-diffuse-ambient",
                 "phong-diffuse-specular", "toon", "cook-torrance", "oren-nayar", "ward")
 ALPHA_ZERO, ALPHA_USE, ALPHA_ONE = 0, 1, 2
-KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_BRICKED, KERNEL_CELL, KERNEL_SLAB = 0, 1, 2, 3, 4
 
 # every symbol include/hipdvr.h and include/hipdvr_synth.h declare
 EXPORTS = (
@@ -44,6 +44,7 @@ EXPORTS = (
     "hipdvr_learned_create", "hipdvr_learned_destroy", "hipdvr_learned_prepare",
     "hipdvr_learned_trace_reset", "hipdvr_learned_trace", "hipdvr_learned_collect",
     "hipdvr_render_learned_device", "hipdvr_learned_sparse_remarch", "hipdvr_learned_sparse_dims",
+    "hipdvr_learned_train",
     "hipdvr_vvd_parse", "hipdvr_vvd_last_error", "hipdvr_volume_load_raw", "hipdvr_volume_load_vvd",
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
     "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
@@ -89,6 +90,20 @@ class Stats(C.Structure):
 class LearnedOpts(C.Structure):
     _fields_ = [("psnr_floor", C.c_float), ("remarch_stride", C.c_int32), ("skip_logit", C.c_float),
                 ("max_log2_step", C.c_int32)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("n_tfs", C.c_uint32), ("n_views", C.c_uint32), ("trace_size", C.c_uint32),
+                ("sampling_rate", C.c_float), ("tau_skip", C.c_float), ("tau_step", C.c_float),
+                ("min_steps", C.c_uint32), ("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("seed", C.c_uint64), ("extra_tfs", C.c_void_p), ("n_extra", C.c_uint32),
+                ("extra_width", C.c_uint32), ("max_rows", C.c_uint64)]
+
+
+class TrainInfo(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("tfs", C.c_uint32), ("views", C.c_uint32), ("trace_size", C.c_uint32),
+                ("steps", C.c_uint32), ("loss_first", C.c_float), ("loss_last", C.c_float),
+                ("trace_seconds", C.c_float), ("seconds", C.c_float)]
 
 
 class LearnedStats(C.Structure):
@@ -183,6 +198,7 @@ def load():
         "hipdvr_learned_sparse_remarch": ([vp, vp, vp, vp, u32, u32, P(Params), P(LearnedOpts), vp,
                                            C.c_uint64, P(u32), P(u32)], C.c_int),
         "hipdvr_learned_sparse_dims": ([u32, u32, P(LearnedOpts), P(u32), P(u32)], C.c_int),
+        "hipdvr_learned_train": ([vp, vp, P(TrainOpts), P(TrainInfo)], C.c_int),
         "hipdvr_vvd_parse": ([C.c_char_p, P(VolumeDesc), C.c_char_p, u32], C.c_int),
         "hipdvr_vvd_last_error": ([], C.c_char_p),
         "hipdvr_volume_load_raw": ([vp, C.c_char_p, P(VolumeDesc), C.c_uint64, P(vp)], C.c_int),
@@ -662,6 +678,30 @@ class Learned:
         self.ctx.check(self.ctx.lib.hipdvr_learned_collect(self.h, tau_skip, tau_step, dx.ptr, dt.ptr,
                                                            capacity, C.byref(n)), "hipdvr_learned_collect")
         return n.value
+
+    def train(self, mlp, extra_tfs=(), **kw):
+        """hipdvr_learned_train: trace, collect and Adam steps for this
+        volume; keyword arguments are the fields of hipdvr_learned_train_opts
+        (n_tfs, n_views, trace_size, sampling_rate, tau_skip, tau_step,
+        min_steps, epochs, batch, lr, seed, max_rows).  Returns the info dict."""
+        o = TrainOpts()
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown training option {k}")
+            setattr(o, k, v)
+        keep = None
+        if len(extra_tfs):
+            keep = np.ascontiguousarray(np.stack([np.asarray(t, np.uint8) for t in extra_tfs]), np.uint8)
+            o.extra_tfs = keep.ctypes.data
+            o.n_extra = keep.shape[0]
+            o.extra_width = keep.shape[1]
+        info = TrainInfo()
+        self.ctx.check(self.ctx.lib.hipdvr_learned_train(self.h, mlp.h, C.byref(o), C.byref(info)),
+                       "hipdvr_learned_train")
+        del keep
+        return dict(rows=int(info.rows), tfs=int(info.tfs), views=int(info.views), trace_size=int(info.trace_size),
+                    steps=int(info.steps), loss_first=float(info.loss_first), loss_last=float(info.loss_last),
+                    data_s=round(float(info.trace_seconds), 2), total_s=round(float(info.seconds), 2))
 
     def render_device(self, tf, d_entry, d_exit, w, h, params, d_out, opts=None, stats=False):
         st = LearnedStats()

@@ -24,6 +24,11 @@ HIPRaycaster::HIPRaycaster()
     , hipDevice_("hipDevice", "HIP device", 0, 0, 63, Processor::INVALID_PROGRAM)
     , hipBrickSize_("hipBrickSize", "Skipping brick size", Processor::INVALID_PROGRAM)
     , hipKernel_("hipKernel", "March kernel")
+    , hipLearned_("hipLearnedStepping", "Learned stepping", false)
+    , hipPsnrFloor_("hipPsnrFloor", "Learned stepping: PSNR floor (dB)", 45.f, 20.f, 99.f)
+    , hipRemarchStride_("hipRemarchStride", "Learned stepping: judge re-marches 1/n^2 of the pixels", 8, 1, 64)
+    , hipLearnedTFs_("hipLearnedTrainingTFs", "Learned stepping: training TFs", 200, 1, 2000,
+                     Processor::INVALID_PROGRAM)
     , ctx_(0)
     , dvrVolume_(0)
     , dvrTF_(0)
@@ -35,6 +40,16 @@ HIPRaycaster::HIPRaycaster()
     , uploadedDomain_(0.f, 0.f)
     , uploadedBrickLog2_(0)
     , uploadedPreintStep_(-1.f)
+    , tfGeneration_(0)
+    , learned_(0)
+    , mlp_(0)
+    , learnedGeneration_(0)
+    , learnedTFs_(0)
+    , dEntry_(0)
+    , dExit_(0)
+    , dOut_(0)
+    , dBytes_(0)
+    , lastFrameLearned_(false)
 {
     // ports and properties of CPURaycaster (cpuraycaster.cpp:44-76)
     addPort(gradientVolumePort_);
@@ -86,10 +101,20 @@ HIPRaycaster::HIPRaycaster()
     hipKernel_.addOption("reference", "Reference layout, no skipping");
     hipKernel_.addOption("bricked", "Bricked, empty-space skipping");
     hipKernel_.addOption("cell", "Cell records, empty-space skipping");
+    hipKernel_.addOption("slab", "Slabs staged in LDS, empty-space skipping");
     hipKernel_.selectByKey("auto");
     addProperty(hipKernel_);
+    // learned stepping (include/hipdvr_learned.h): the network proposes skip
+    // distances and step multiples per brick; the plain march re-marches
+    // 1/hipRemarchStride^2 of the pixels every frame and the frame falls back
+    // to the plain march below hipPsnrFloor
+    addProperty(hipLearned_);
+    addProperty(hipPsnrFloor_);
+    addProperty(hipRemarchStride_);
+    addProperty(hipLearnedTFs_);
 
     std::memset(&lastStats_, 0, sizeof(lastStats_));
+    std::memset(&lastLearned_, 0, sizeof(lastLearned_));
 }
 
 HIPRaycaster::~HIPRaycaster() {
@@ -113,7 +138,25 @@ void HIPRaycaster::deinitialize() {
     VolumeRaycaster::deinitialize();
 }
 
+void HIPRaycaster::releaseLearned() {
+    if (learned_) hipdvr_learned_destroy(learned_);
+    if (mlp_) hipdvr_mlp_destroy(mlp_);
+    if (ctx_) {
+        if (dEntry_) hipdvr_buffer_free(ctx_, dEntry_);
+        if (dExit_) hipdvr_buffer_free(ctx_, dExit_);
+        if (dOut_) hipdvr_buffer_free(ctx_, dOut_);
+    }
+    learned_ = 0;
+    mlp_ = 0;
+    learnedGeneration_ = 0;
+    learnedTFs_ = 0;
+    preparedKey_.clear();
+    dEntry_ = dExit_ = dOut_ = 0;
+    dBytes_ = 0;
+}
+
 void HIPRaycaster::releaseDevice() {
+    releaseLearned();
     if (dvrTF_) hipdvr_tf_destroy(dvrTF_);
     if (dvrTF2D_) hipdvr_tf_destroy(dvrTF2D_);
     if (dvrVolume_) hipdvr_volume_destroy(dvrVolume_);
@@ -261,6 +304,92 @@ bool HIPRaycaster::ensureTransFunc(tgt::Texture* tfTexture, const tgt::vec2& dom
     }
     uploadedTF_.assign(data, data + bytes);
     uploadedDomain_ = domain;
+    ++tfGeneration_;
+    return true;
+}
+
+// The learned path serves the plain DVR march: 1D TF classification, linear
+// filter, DVR compositing, no shading or Phong with central differences.
+bool HIPRaycaster::learnedEligible(const hipdvr_params& p, bool tf2d, const std::string& cls) const {
+    return !tf2d && cls == "transfer-function" && p.filter == HIPDVR_FILTER_LINEAR &&
+           p.compositing == HIPDVR_COMPOSITE_DVR &&
+           (p.shading == HIPDVR_SHADE_NONE ||
+            (p.shading == HIPDVR_SHADE_PHONG && p.gradient == HIPDVR_GRADIENT_CENTRAL));
+}
+
+// One learned frame: (re)train the network when the volume or the number of
+// training TFs changed (on the device, hipdvr_learned_train, with the current
+// TF among the tables traced), prepare the stepping table when the TF, step or
+// options changed, then render through device buffers kept between frames.
+bool HIPRaycaster::renderLearned(const hipdvr_params& params, const float* entry, const float* exit, int w, int h,
+                                 float* output) {
+    if (!learned_ || learnedGeneration_ != volumeGeneration_) {
+        releaseLearned();
+        if (hipdvr_learned_create(ctx_, dvrVolume_, &learned_) != HIPDVR_OK) {
+            learned_ = 0;
+            return fail(std::string("learned state: ") + hipdvr_last_error(ctx_));
+        }
+        learnedGeneration_ = volumeGeneration_;
+    }
+    if (!mlp_ || learnedTFs_ != hipLearnedTFs_.get()) {
+        if (mlp_) hipdvr_mlp_destroy(mlp_);
+        mlp_ = 0;
+        preparedKey_.clear();
+        if (hipdvr_mlp_create(ctx_, 0x5EED, &mlp_) != HIPDVR_OK) {
+            mlp_ = 0;
+            return fail(std::string("stepping network: ") + hipdvr_last_error(ctx_));
+        }
+        hipdvr_learned_train_opts topts;
+        std::memset(&topts, 0, sizeof(topts));
+        topts.n_tfs = static_cast<uint32_t>(hipLearnedTFs_.get());
+        topts.sampling_rate = samplingRate_.get();
+        topts.extra_tfs = uploadedTF_.data();
+        topts.n_extra = 1;
+        topts.extra_width = static_cast<uint32_t>(uploadedTF_.size() / 4);
+        hipdvr_learned_train_info info;
+        if (hipdvr_learned_train(learned_, mlp_, &topts, &info) != HIPDVR_OK) {
+            hipdvr_mlp_destroy(mlp_);
+            mlp_ = 0;
+            return fail(std::string("training the stepping network failed: ") + hipdvr_last_error(ctx_));
+        }
+        learnedTFs_ = hipLearnedTFs_.get();
+    }
+    hipdvr_learned_opts lopts;
+    std::memset(&lopts, 0, sizeof(lopts));
+    lopts.psnr_floor = hipPsnrFloor_.get();
+    lopts.remarch_stride = hipRemarchStride_.get();
+    lopts.skip_logit = 0.f;
+    lopts.max_log2_step = 3;
+    const std::string key = std::to_string(tfGeneration_) + ":" + std::to_string(params.step) + ":" +
+                            std::to_string(lopts.psnr_floor) + ":" + std::to_string(lopts.remarch_stride);
+    if (key != preparedKey_) {
+        if (hipdvr_learned_prepare(learned_, dvrTF_, &params, mlp_, &lopts) != HIPDVR_OK) {
+            preparedKey_.clear();
+            return fail(std::string("learned stepping table: ") + hipdvr_last_error(ctx_));
+        }
+        preparedKey_ = key;
+    }
+    const size_t bytes = 4 * sizeof(float) * static_cast<size_t>(w) * h;
+    if (bytes > dBytes_) {
+        if (dEntry_) hipdvr_buffer_free(ctx_, dEntry_);
+        if (dExit_) hipdvr_buffer_free(ctx_, dExit_);
+        if (dOut_) hipdvr_buffer_free(ctx_, dOut_);
+        dEntry_ = dExit_ = dOut_ = 0;
+        dBytes_ = 0;
+        if (hipdvr_buffer_alloc(ctx_, bytes, reinterpret_cast<void**>(&dEntry_)) != HIPDVR_OK ||
+            hipdvr_buffer_alloc(ctx_, bytes, reinterpret_cast<void**>(&dExit_)) != HIPDVR_OK ||
+            hipdvr_buffer_alloc(ctx_, bytes, reinterpret_cast<void**>(&dOut_)) != HIPDVR_OK)
+            return fail(std::string("device buffers: ") + hipdvr_last_error(ctx_));
+        dBytes_ = bytes;
+    }
+    if (hipdvr_memcpy_h2d(ctx_, dEntry_, entry, bytes) != HIPDVR_OK ||
+        hipdvr_memcpy_h2d(ctx_, dExit_, exit, bytes) != HIPDVR_OK)
+        return fail(std::string("entry/exit upload: ") + hipdvr_last_error(ctx_));
+    if (hipdvr_render_learned_device(learned_, dvrTF_, dEntry_, dExit_, static_cast<uint32_t>(w),
+                                     static_cast<uint32_t>(h), &params, &lopts, dOut_, &lastLearned_) != HIPDVR_OK)
+        return fail(std::string("learned render: ") + hipdvr_last_error(ctx_));
+    if (hipdvr_memcpy_d2h(ctx_, output, dOut_, bytes) != HIPDVR_OK)
+        return fail(std::string("image download: ") + hipdvr_last_error(ctx_));
     return true;
 }
 
@@ -309,6 +438,7 @@ void HIPRaycaster::fillParams(hipdvr_params& p, const VolumeBase* vb, float step
     p.kernel = k == "reference" ? HIPDVR_KERNEL_REFERENCE
              : k == "bricked"   ? HIPDVR_KERNEL_BRICKED
              : k == "cell"      ? HIPDVR_KERNEL_CELL
+             : k == "slab"      ? HIPDVR_KERNEL_SLAB
                                 : HIPDVR_KERNEL_AUTO;
     p.batch = 0;
     p.exact_shading = 0;
@@ -484,15 +614,29 @@ void HIPRaycaster::process() {
         exitPort_.getColorTexture()->downloadTextureToBuffer(GL_RGBA, GL_FLOAT));
 
     std::vector<float> output(4 * static_cast<size_t>(size.x) * size.y, 0.f);
-    const int rc = hipdvr_render(ctx_, dvrVolume_, renderTF, reinterpret_cast<const float*>(entryBuffer),
-                                 reinterpret_cast<const float*>(exitBuffer), size.x, size.y, &params,
-                                 output.data(), &lastStats_);
+    lastFrameLearned_ = false;
+    if (hipLearned_.get()) {
+        if (learnedEligible(params, tf2d, cls)) {
+            lastFrameLearned_ = renderLearned(params, reinterpret_cast<const float*>(entryBuffer),
+                                              reinterpret_cast<const float*>(exitBuffer), size.x, size.y,
+                                              output.data());
+            if (!lastFrameLearned_) LWARNING("learned stepping failed; rendering plainly");
+        } else {
+            LWARNING("learned stepping serves DVR with a 1D TF, the linear filter and no shading or Phong "
+                     "(central differences); rendering plainly");
+        }
+    }
+    if (!lastFrameLearned_) {
+        const int rc = hipdvr_render(ctx_, dvrVolume_, renderTF, reinterpret_cast<const float*>(entryBuffer),
+                                     reinterpret_cast<const float*>(exitBuffer), size.x, size.y, &params,
+                                     output.data(), &lastStats_);
+        if (rc != HIPDVR_OK) {
+            fail(std::string("hipdvr_render failed: ") + hipdvr_last_error(ctx_));
+            output.clear();
+        }
+    }
     delete[] entryBuffer;
     delete[] exitBuffer;
-    if (rc != HIPDVR_OK) {
-        fail(std::string("hipdvr_render failed: ") + hipdvr_last_error(ctx_));
-        output.clear();
-    }
     writeOutput(output, size.x, size.y);
 }
 

@@ -14,6 +14,7 @@
 #include "voreen/core/ports/volumeport.h"
 
 #include "hipdvr.h"
+#include "hipdvr_learned.h"
 
 #include <string>
 #include <vector>
@@ -32,7 +33,11 @@ namespace voreen {
  * tFunc2DPrimitives, textureFilterMode_, preIntegrationTableSize from
  * CPURaycaster; camera as in SingleVolumeRaycaster).  The march runs in the
  * HIP kernels of libhipdvr.so through its C-ABI (hipdvr.h).  Properties added
- * here have ids starting with "hip".
+ * here have ids starting with "hip", among them the learned stepping:
+ * hipLearnedStepping (on/off), hipPsnrFloor (dB below which a frame is
+ * rendered plainly), hipRemarchStride (the judge re-marches 1/stride^2 of
+ * the pixels plainly every frame) and hipLearnedTrainingTFs (random TFs the
+ * network is trained on, on the device, once per volume).
  *
  * Invalidation: a new volume (INVALID_PROGRAM on the volume inport,
  * src/core/processors/volumeraycaster.cpp:44) re-uploads the volume; a new TF
@@ -55,6 +60,11 @@ public:
     /// Statistics of the last frame (rays, samples, reconstructed samples,
     /// shaded samples, kernel time).
     const hipdvr_stats& getLastStats() const { return lastStats_; }
+
+    /// Whether the last frame went through the learned path, and its
+    /// statistics (reconstructed samples, the judge's PSNR, fallback).
+    bool lastFrameLearned() const { return lastFrameLearned_; }
+    const hipdvr_learned_stats& getLastLearnedStats() const { return lastLearned_; }
 
 protected:
     virtual void setDescriptions() {
@@ -80,6 +90,11 @@ private:
     void fillParams(hipdvr_params& p, const VolumeBase* vb, float step);
     void writeOutput(const std::vector<float>& rgba, int w, int h);
     bool fail(const std::string& msg);
+    // learned stepping: the options it serves, and the frame through it
+    bool learnedEligible(const hipdvr_params& p, bool tf2d, const std::string& cls) const;
+    bool renderLearned(const hipdvr_params& params, const float* entry, const float* exit, int w, int h,
+                       float* output);
+    void releaseLearned();
 
     VolumePort gradientVolumePort_;
 
@@ -94,6 +109,10 @@ private:
     IntProperty hipDevice_;
     IntOptionProperty hipBrickSize_;
     StringOptionProperty hipKernel_;
+    BoolProperty hipLearned_;
+    FloatProperty hipPsnrFloor_;
+    IntProperty hipRemarchStride_;
+    IntProperty hipLearnedTFs_;
 
     hipdvr_ctx* ctx_;
     hipdvr_volume* dvrVolume_;
@@ -110,7 +129,21 @@ private:
     int uploadedBrickLog2_;
     float uploadedPreintStep_;
     std::string uploadedPreintKey_;
+    unsigned long tfGeneration_;          // counts 1D TF uploads
     hipdvr_stats lastStats_;
+
+    // learned stepping state: per volume upload, the network trained for it
+    hipdvr_learned* learned_;
+    hipdvr_mlp* mlp_;
+    unsigned long learnedGeneration_;     // volume upload learned_ and mlp_ belong to
+    int learnedTFs_;                      // training TFs of mlp_
+    std::string preparedKey_;             // TF upload, step and options of the last prepare
+    float* dEntry_;
+    float* dExit_;
+    float* dOut_;
+    size_t dBytes_;
+    bool lastFrameLearned_;
+    hipdvr_learned_stats lastLearned_;
 
     static const std::string loggerCat_;
 };

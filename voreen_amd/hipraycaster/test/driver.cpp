@@ -214,6 +214,12 @@ int hrc_stats(void* h, hipdvr_stats* st) {
     return 0;
 }
 
+// 1 if the last frame went through the learned path, with its statistics
+int hrc_learned_stats(void* h, hipdvr_learned_stats* st) {
+    *st = ((Driver*)h)->proc->getLastLearnedStats();
+    return ((Driver*)h)->proc->lastFrameLearned() ? 1 : 0;
+}
+
 int hrc_invalidation(void* h) { return ((Driver*)h)->proc->getInvalidationLevel(); }
 
 }  // extern "C"
