@@ -43,6 +43,20 @@ int hipdvr_synth_shells(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, ui
 int hipdvr_synth_turbulence_device(hipdvr_ctx* ctx, uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz,
                                    float empty_fraction, uint16_t* d_out, double* empty_out);
 
+/* The same volume as hipdvr_synth_turbulence_device, bit for bit, without
+ * allocating: no hipMalloc, hipFree or other allocating call.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under stream capture, serving loops, harnesses that forbid it).  The
+ * generator's scratch -- the 4096-bin histogram and the zero counter, 32,776
+ * bytes whatever the volume size -- is a file-scope device array of the
+ * library, one copy per GPU, held until the process ends; its address on a
+ * device is looked up once by hipdvr_create on that device.  Every call
+ * clears it first (with a fill kernel).  Calls on one GPU share it: calls
+ * that overlap on one GPU (several contexts or host threads) race and must
+ * be serialized by the caller. */
+int hipdvr_synth_turbulence_device_noalloc(hipdvr_ctx* ctx, uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz,
+                                           float empty_fraction, uint16_t* d_out, double* empty_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@
 #include "errors.h"
 #include "kernels.h"
 #include "objects.h"
+#include "synth_scratch.h"
 #include "volume.h"
 
 using namespace hipdvr;
@@ -410,6 +411,8 @@ int hipdvr_create(hipdvr_ctx** out, int device) {
         hip_check(hipEventCreate(&ctx->ev0), "hipEventCreate");
         hip_check(hipEventCreate(&ctx->ev1), "hipEventCreate");
         hip_check(hipMalloc((void**)&ctx->d_counters, kNumCounters * sizeof(unsigned long long)), "hipMalloc(counters)");
+        // the no-allocation generator's scratch address on this device
+        synth_scratch_init(device);
     });
     if (rc != HIPDVR_OK) {
         std::fprintf(stderr, "hipdvr_create: %s\n", ctx->last_error.c_str());

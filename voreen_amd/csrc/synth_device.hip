@@ -12,12 +12,21 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
-#include <vector>
+#include <mutex>
 
 #include "../../include/hipdvr.h"
 #include "../../include/hipdvr_synth.h"
 #include "ctx.h"
 #include "errors.h"
+#include "kernels.h"
+#include "synth_scratch.h"
+
+// Histogram and zero counter of hipdvr_synth_turbulence_device_noalloc:
+// 4096 bins + 1 counter of uint64 (32,776 bytes), whatever the volume size.
+// One copy per GPU, held until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device
+// (g_scratch below), never through this name.
+static __device__ __align__(256) unsigned long long scratch[4096 + 1];
 
 namespace hipdvr {
 
@@ -123,7 +132,82 @@ __global__ __launch_bounds__(256) void threshold_kernel(uint16_t* __restrict__ v
     if ((threadIdx.x & 63) == 0 && z) atomicAdd(zeros, z);
 }
 
+static_assert(sizeof(scratch) == (kHistBins + 1) * sizeof(unsigned long long), "scratch size");
+
+// Device address of `scratch` per device ordinal, set once per device by
+// synth_scratch_init with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch[kMaxDevices] = {};
+
+// Turbulence, histogram threshold and count, with the histogram (and the
+// zero counter after it) in d_hist: kHistBins + 1 uint64.  clear_by_kernel:
+// clear it with a fill kernel instead of hipMemsetAsync.
+void turbulence_impl(hipdvr_ctx* ctx, uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, float empty_fraction,
+                     uint16_t* d_out, double* empty_out, unsigned long long* d_hist, bool clear_by_kernel) {
+    hipStream_t st = ctx_stream(ctx);
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    const uint32_t m = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+    const uint64_t want = (n + 255) / 256;
+    const uint32_t grid = (uint32_t)(want < (uint64_t)cus * 16 ? want : (uint64_t)cus * 16);
+    const uint32_t s32 = (uint32_t)(seed ^ (seed >> 32));
+    hipLaunchKernelGGL(turbulence_kernel, dim3(grid), dim3(256), 0, st, d_out, nx, ny, n, 1.0f / (float)m, s32);
+    hip_check(hipGetLastError(), "turbulence kernel");
+    if (clear_by_kernel) {
+        launch_zero_u64(d_hist, kHistBins + 1, st);
+        hip_check(hipGetLastError(), "histogram clear");
+    } else {
+        hip_check(hipMemsetAsync(d_hist, 0, (kHistBins + 1) * sizeof(unsigned long long), st), "memset");
+    }
+    hipLaunchKernelGGL(histogram_kernel, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_out, n, d_hist);
+    hip_check(hipGetLastError(), "histogram kernel");
+    unsigned long long hist[kHistBins];
+    hip_check(hipMemcpyAsync(hist, d_hist, kHistBins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+              "histogram download");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    // bin boundary b*16 whose count below is nearest the target
+    const double target = (double)empty_fraction * (double)n;
+    uint32_t best = 0;
+    double best_err = target, below = 0.0;
+    for (int b = 0; b < kHistBins; ++b) {
+        below += (double)hist[b];
+        const double err = std::fabs(below - target);
+        if (err < best_err) {
+            best_err = err;
+            best = (uint32_t)(b + 1);
+        }
+    }
+    const uint32_t thr = best >= kHistBins ? 65535u : best * 16u;
+    uint32_t top = kHistBins - 1;
+    while (top > 0 && hist[top] == 0) --top;
+    const uint32_t vmax = top * 16u + 15u;   // upper end of the highest occupied bin
+    hipLaunchKernelGGL(threshold_kernel, dim3(grid), dim3(256), 0, st, d_out, n, thr, vmax, d_hist + kHistBins);
+    hip_check(hipGetLastError(), "threshold kernel");
+    unsigned long long zeros = 0;
+    hip_check(hipMemcpyAsync(&zeros, d_hist + kHistBins, sizeof zeros, hipMemcpyDeviceToHost, st), "count download");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (empty_out) *empty_out = (double)zeros / (double)n;
+}
+
+void check_turbulence_args(uint32_t nx, uint32_t ny, uint32_t nz, float empty_fraction, const uint16_t* d_out) {
+    if (!d_out || nx == 0 || ny == 0 || nz == 0) fail(HIPDVR_ERR_INVALID, "bad turbulence arguments");
+    if (!(empty_fraction >= 0.0f && empty_fraction < 1.0f)) fail(HIPDVR_ERR_INVALID, "empty_fraction must be in [0, 1)");
+}
+
 }  // namespace
+
+void synth_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(HIPDVR_ERR_INVALID, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) fail(HIPDVR_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch[device] = (unsigned long long*)p;
+}
 
 }  // namespace hipdvr
 
@@ -134,60 +218,33 @@ extern "C" int hipdvr_synth_turbulence_device(hipdvr_ctx* ctx, uint64_t seed, ui
                                               double* empty_out) {
     if (!ctx) return HIPDVR_ERR_INVALID;
     return ctx_guarded(ctx, [&] {
-        if (!d_out || nx == 0 || ny == 0 || nz == 0) fail(HIPDVR_ERR_INVALID, "bad turbulence arguments");
-        if (!(empty_fraction >= 0.0f && empty_fraction < 1.0f))
-            fail(HIPDVR_ERR_INVALID, "empty_fraction must be in [0, 1)");
+        check_turbulence_args(nx, ny, nz, empty_fraction, d_out);
         ctx_set_device(ctx);
-        hipStream_t st = ctx_stream(ctx);
-        const uint64_t n = (uint64_t)nx * ny * nz;
-        const uint32_t m = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
-        int dev = 0, cus = 0;
-        hip_check(hipGetDevice(&dev), "hipGetDevice");
-        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
-        const uint64_t want = (n + 255) / 256;
-        const uint32_t grid = (uint32_t)(want < (uint64_t)cus * 16 ? want : (uint64_t)cus * 16);
         unsigned long long* d_hist = nullptr;
         hip_check(hipMalloc((void**)&d_hist, (kHistBins + 1) * sizeof(unsigned long long)), "hipMalloc(histogram)");
         try {
-            const uint32_t s32 = (uint32_t)(seed ^ (seed >> 32));
-            hipLaunchKernelGGL(turbulence_kernel, dim3(grid), dim3(256), 0, st, d_out, nx, ny, n, 1.0f / (float)m,
-                               s32);
-            hip_check(hipGetLastError(), "turbulence kernel");
-            hip_check(hipMemsetAsync(d_hist, 0, (kHistBins + 1) * sizeof(unsigned long long), st), "memset");
-            hipLaunchKernelGGL(histogram_kernel, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_out, n, d_hist);
-            hip_check(hipGetLastError(), "histogram kernel");
-            std::vector<unsigned long long> hist(kHistBins);
-            hip_check(hipMemcpyAsync(hist.data(), d_hist, kHistBins * sizeof(unsigned long long),
-                                     hipMemcpyDeviceToHost, st), "histogram download");
-            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            // bin boundary b*16 whose count below is nearest the target
-            const double target = (double)empty_fraction * (double)n;
-            uint32_t best = 0;
-            double best_err = target, below = 0.0;
-            for (int b = 0; b < kHistBins; ++b) {
-                below += (double)hist[(size_t)b];
-                const double err = std::fabs(below - target);
-                if (err < best_err) {
-                    best_err = err;
-                    best = (uint32_t)(b + 1);
-                }
-            }
-            const uint32_t thr = best >= kHistBins ? 65535u : best * 16u;
-            uint32_t top = kHistBins - 1;
-            while (top > 0 && hist[top] == 0) --top;
-            const uint32_t vmax = top * 16u + 15u;   // upper end of the highest occupied bin
-            hipLaunchKernelGGL(threshold_kernel, dim3(grid), dim3(256), 0, st, d_out, n, thr, vmax,
-                               d_hist + kHistBins);
-            hip_check(hipGetLastError(), "threshold kernel");
-            unsigned long long zeros = 0;
-            hip_check(hipMemcpyAsync(&zeros, d_hist + kHistBins, sizeof zeros, hipMemcpyDeviceToHost, st),
-                      "count download");
-            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            if (empty_out) *empty_out = (double)zeros / (double)n;
+            turbulence_impl(ctx, seed, nx, ny, nz, empty_fraction, d_out, empty_out, d_hist, false);
         } catch (...) {
             (void)hipFree(d_hist);
             throw;
         }
         (void)hipFree(d_hist);
+    });
+}
+
+extern "C" int hipdvr_synth_turbulence_device_noalloc(hipdvr_ctx* ctx, uint64_t seed, uint32_t nx, uint32_t ny,
+                                                      uint32_t nz, float empty_fraction, uint16_t* d_out,
+                                                      double* empty_out) {
+    if (!ctx) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(ctx, [&] {
+        check_turbulence_args(nx, ny, nz, empty_fraction, d_out);
+        ctx_set_device(ctx);
+        unsigned long long* d_hist = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            if (ctx->device >= 0 && ctx->device < kMaxDevices) d_hist = g_scratch[ctx->device];
+        }
+        if (!d_hist) fail(HIPDVR_ERR_HIP, "the device's scratch address is not set up (hipdvr_create sets it)");
+        turbulence_impl(ctx, seed, nx, ny, nz, empty_fraction, d_out, empty_out, d_hist, true);
     });
 }

@@ -38,6 +38,7 @@ EXPORTS = (
     "hipdvr_render", "hipdvr_render_device", "hipdvr_entry_exit_device",
     "hipdvr_buffer_alloc", "hipdvr_buffer_free", "hipdvr_memcpy_h2d", "hipdvr_memcpy_d2h",
     "hipdvr_synth_blobs", "hipdvr_synth_shells", "hipdvr_synth_turbulence_device",
+    "hipdvr_synth_turbulence_device_noalloc",
     "hipdvr_timing_begin", "hipdvr_timing_read", "hipdvr_copy_device",
     "hipdvr_mlp_create", "hipdvr_mlp_destroy", "hipdvr_mlp_get_params", "hipdvr_mlp_set_params",
     "hipdvr_mlp_forward", "hipdvr_mlp_gradient", "hipdvr_mlp_train_step",
@@ -177,6 +178,8 @@ def load():
         "hipdvr_synth_blobs": ([C.c_uint64, u32, u32, u32, i32, vp], C.c_int),
         "hipdvr_synth_shells": ([C.c_uint64, u32, u32, u32, vp], C.c_int),
         "hipdvr_synth_turbulence_device": ([vp, C.c_uint64, u32, u32, u32, C.c_float, vp, P(C.c_double)], C.c_int),
+        "hipdvr_synth_turbulence_device_noalloc": ([vp, C.c_uint64, u32, u32, u32, C.c_float, vp, P(C.c_double)],
+                                                   C.c_int),
         "hipdvr_timing_begin": ([vp, u32], C.c_int),
         "hipdvr_timing_read": ([vp, C.POINTER(C.c_float), u32, C.POINTER(u32)], C.c_int),
         "hipdvr_copy_device": ([vp, vp, vp, C.c_size_t], C.c_int),
@@ -373,16 +376,16 @@ class Context:
                                                C.byref(t.h)), "hipdvr_tf2d_create")
         return t
 
-    def synth_turbulence(self, seed, dims, empty_fraction=0.4, out=None):
-        """uint16 turbulence volume made on the device (hipdvr_synth_turbulence_device):
-        (device buffer, exact empty fraction); dims = (x, y, z); into `out`
-        when given, else a new buffer."""
+    def synth_turbulence(self, seed, dims, empty_fraction=0.4, out=None, noalloc=False):
+        """uint16 turbulence volume made on the device (hipdvr_synth_turbulence_device,
+        or its no-allocation variant): (device buffer, exact empty fraction);
+        dims = (x, y, z); into `out` when given, else a new buffer."""
         nx, ny, nz = dims
         buf = out if out is not None else self.alloc(nx * ny * nz * 2)
         if buf.nbytes < nx * ny * nz * 2:
             raise ValueError("output buffer too small")
         frac = C.c_double()
-        fn = "hipdvr_synth_turbulence_device"
+        fn = "hipdvr_synth_turbulence_device_noalloc" if noalloc else "hipdvr_synth_turbulence_device"
         try:
             self.check(getattr(self.lib, fn)(self.h, seed, nx, ny, nz, empty_fraction, buf.ptr, C.byref(frac)), fn)
         except Exception:
