@@ -312,9 +312,10 @@ def test_process_tf2d_and_gradient_modes(rc, oracle):
 def test_process_with_learned_stepping(rc, oracle, shading):
     """hipLearnedStepping on: the processor trains the stepping network on the
     device (hipdvr_learned_train, 48 random TFs plus the current one), renders
-    through the learned path and stays >= 45 dB against its own plain frame,
-    with fewer reconstructed samples; an option the learned path does not
-    serve (nearest filter) renders plainly."""
+    through the learned path and stays >= 45 dB against its own plain frame
+    without falling back; an option the learned path does not serve (nearest
+    filter) renders plainly.  (What the network gains is tested on the full
+    512^3 workload, tests/test_gpu_fullsize.py.)"""
     from voreen_amd import workloads as W
     dims = (128, 128, 128)
     vol = W.make_volume("c1", dims=dims)
@@ -329,7 +330,7 @@ def test_process_with_learned_stepping(rc, oracle, shading):
     assert rc.set("shading", shading) == 0
     plain, nerr = rc.process()
     assert nerr == 0, rc.last_error()
-    fetched = rc.stats()["fetched"]
+    plain_samples = rc.stats()["samples"]
     assert rc.set("hipLearnedTrainingTFs", 48) == 0
     assert rc.set("hipLearnedStepping", "true") == 0
     out, nerr = rc.process()
@@ -337,11 +338,11 @@ def test_process_with_learned_stepping(rc, oracle, shading):
     learned, ls = rc.learned_stats()
     mse = float(np.mean((out.astype(np.float64) - plain.astype(np.float64)) ** 2))
     psnr = 999.0 if mse == 0 else 10 * np.log10(1 / mse)
-    print(f"learned {learned}: psnr {psnr:.1f} dB, reconstructed {ls['samples']} of {fetched}, "
+    print(f"learned {learned}: psnr {psnr:.1f} dB, reconstructed {ls['samples']} (plain loop {plain_samples}), "
           f"sparse {ls['sparse_psnr']:.1f} dB, fell back {ls['fell_back']}", flush=True)
     assert learned and ls["fell_back"] == 0
-    assert psnr >= 45.0
-    assert ls["samples"] < fetched
+    assert psnr >= 45.0 and ls["sparse_psnr"] >= 45.0
+    assert 0 < ls["samples"] < plain_samples
     # a second frame reuses the network and the stepping table
     out2, nerr = rc.process()
     assert nerr == 0 and np.array_equal(out, out2)
