@@ -20,10 +20,15 @@ on a bounded band of rows of view 0).
 volumes; the 8 GiB stress volume with 8 4K views), each printing its own line.
 """
 import argparse
+import csv
 import ctypes as C
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -59,6 +64,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (hipdvr_render) leg")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate) of this run")
     return ap.parse_args()
 
 
@@ -74,6 +82,84 @@ def measured_traffic(config, shading):
         if len(f) == 4 and not line.startswith("#") and f[0] == config and f[1] == shading:
             return {"bytes": int(f[2]), "source": f[3]}
     return None
+
+
+MARCH_KERNELS = {"auto": "march_cell_kernel", "cell": "march_cell_kernel", "slab": "march_slab_kernel",
+                 "bricked": "march_fast_kernel", "reference": "march_ref_kernel"}
+
+
+def _read_counters(outdir):
+    """(kernel name, dispatch id, counter, value) rows of a rocprofv3 counter
+    collection CSV under outdir."""
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection*.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                try:
+                    rows.append((r["Kernel_Name"], int(r.get("Dispatch_Id") or 0), r["Counter_Name"],
+                                 float(r["Counter_Value"])))
+                except (KeyError, ValueError):
+                    continue
+    return rows
+
+
+def pmc_leg(args, views=4):
+    """This run's HBM traffic and L2 hit rate of the march kernel: rocprofv3
+    counter passes (FETCH_SIZE; WRITE_SIZE; TCC_HIT + TCC_MISS) over a child run
+    of this bench on `views` views of the same workload, each byte counter
+    scaled by what it reads for a plain 1 GiB copy in the same child.  Runs
+    before this process touches the GPU; None when rocprofv3 is missing or a
+    pass fails."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    kname = MARCH_KERNELS[args.kernel]
+    child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--kernel", args.kernel,
+             "--shading", args.shading, "--brick-log2", str(args.brick_log2), "--batch", str(args.batch),
+             "--views", str(views), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-learned", "--no-pmc", "--no-host",
+             "--copy-calibration", "1024"]
+    passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"], "tcc": ["TCC_HIT_sum", "TCC_MISS_sum"]}
+    got = {}
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    for name, counters in passes.items():
+        outdir = tempfile.mkdtemp(prefix=f"hipdvr_pmc_{name}_")
+        cmd = ["timeout", "-s", "KILL", "120", exe, "--pmc", *counters, "--output-format", "csv", "-d", outdir,
+               "-o", "run", "--", *child]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, timeout=150)
+        except (subprocess.TimeoutExpired, OSError):
+            shutil.rmtree(outdir, ignore_errors=True)
+            return None
+        rows = _read_counters(outdir)
+        shutil.rmtree(outdir, ignore_errors=True)
+        if not rows:
+            return None
+        got[name] = rows
+        print(f"[pmc] {' '.join(counters)}: {len(rows)} counter rows", file=sys.stderr, flush=True)
+
+    def per_dispatch(rows, counter, match):
+        d = {}
+        for k, disp, c, v in rows:
+            if c.startswith(counter) and match in k:
+                d[disp] = d.get(disp, 0.0) + v
+        return [d[i] for i in sorted(d)]
+
+    # the timed launches are the last `views` dispatches of the march kernel
+    fetch = per_dispatch(got["fetch"], "FETCH_SIZE", kname)[-views:]
+    write = per_dispatch(got["write"], "WRITE_SIZE", kname)[-views:]
+    hit = per_dispatch(got["tcc"], "TCC_HIT", kname)[-views:]
+    miss = per_dispatch(got["tcc"], "TCC_MISS", kname)[-views:]
+    cf = per_dispatch(got["fetch"], "FETCH_SIZE", "copy16_kernel")
+    cw = per_dispatch(got["write"], "WRITE_SIZE", "copy16_kernel")
+    if not (fetch and write and hit and miss and cf and cw):
+        return None
+    gib_kib = 1024.0 * 1024.0
+    f_fac, w_fac = gib_kib / cf[-1], gib_kib / cw[-1]
+    traffic = (np.mean(fetch) * f_fac + np.mean(write) * w_fac) * 1024.0
+    return {"traffic_bytes": int(traffic), "l2_hit_rate": round(float(np.sum(hit) / (np.sum(hit) + np.sum(miss))), 4),
+            "fetch_factor": round(f_fac, 3), "write_factor": round(w_fac, 3),
+            "source": f"rocprofv3 --pmc in this run: views 0..{views - 1} of the orbit, FETCH_SIZE/WRITE_SIZE "
+                      "(KiB per dispatch) scaled by a 1 GiB copy kernel, TCC_HIT/TCC_MISS summed over channels"}
 
 
 def cpu_model():
@@ -478,6 +564,9 @@ def main():
         return main_batch(args)
     if args.config == "c4":
         return main_c4(args)
+    # counter passes first: child processes under rocprofv3, before this
+    # process touches the GPU
+    pmc = None if args.no_pmc else pmc_leg(args)
     from voreen_amd import hipdvr as H, workloads as W
 
     cfg = dict(W.CONFIGS[args.config])
@@ -547,7 +636,9 @@ def main():
     kernel_s = sum(launch_ms) / 1e3
     achieved = bytes_orbit * args.steps / kernel_s / 1e9
     traffic = None
-    if args.kernel == "auto" and args.batch == 0 and args.brick_log2 == 4:
+    if pmc:
+        traffic = {"bytes": pmc["traffic_bytes"], "source": pmc["source"]}
+    elif args.kernel == "auto" and args.batch == 0 and args.brick_log2 == 4:
         traffic = measured_traffic(args.config, "phong" if cfg["shading"] else "none")
     out = {
         "metric": "frames/s + Gsamples/s at 1024^2, 512^3 u16; learned vs plain at >=45 dB PSNR",
@@ -589,6 +680,9 @@ def main():
         },
         "setup_s": round(setup_s, 2),
     }
+    if pmc:
+        out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
+        out["roofline"]["counter_calibration"] = {"fetch": pmc["fetch_factor"], "write": pmc["write_factor"]}
     if traffic:
         out["roofline"]["traffic_source"] = traffic["source"]
         out["roofline"]["algorithmic_bytes_per_launch"] = round(bytes_orbit / nviews)
@@ -602,7 +696,8 @@ def main():
         a.free()
         b.free()
         out["copy_calibration_bytes"] = nb
-    out["host_path"] = host_path_leg(ctx, v, t, bufs, params, wimg, himg, launch_ms[0])
+    if not args.no_host:
+        out["host_path"] = host_path_leg(ctx, v, t, bufs, params, wimg, himg, launch_ms[0])
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, fps, samples / nviews,
                                      args)
