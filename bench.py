@@ -67,6 +67,7 @@ def parse():
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (hipdvr_render) leg")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate) of this run")
+    ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch eagerly (no captured graph)")
     return ap.parse_args()
 
 
@@ -103,30 +104,36 @@ def _read_counters(outdir):
     return rows
 
 
-def pmc_leg(args, views=4):
+def pmc_leg(args, views=4, c4=False):
     """This run's HBM traffic and L2 hit rate of the march kernel: rocprofv3
     counter passes (FETCH_SIZE; WRITE_SIZE; TCC_HIT + TCC_MISS) over a child run
-    of this bench on `views` views of the same workload, each byte counter
-    scaled by what it reads for a plain 1 GiB copy in the same child.  Runs
-    before this process touches the GPU; None when rocprofv3 is missing or a
-    pass fails."""
+    of this bench on `views` views of the same workload (c4: one eager launch
+    of the batched kernel over all views), each byte counter scaled by what it
+    reads for a plain 1 GiB copy in the same child.  Runs before this process
+    touches the GPU; None when rocprofv3 is missing or a pass fails."""
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(exe):
         return None
-    kname = MARCH_KERNELS[args.kernel]
-    child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--kernel", args.kernel,
-             "--shading", args.shading, "--brick-log2", str(args.brick_log2), "--batch", str(args.batch),
-             "--views", str(views), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-learned", "--no-pmc", "--no-host",
-             "--copy-calibration", "1024"]
+    common = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-learned", "--no-pmc", "--copy-calibration", "1024"]
+    if c4:
+        kname, views, limit = "batch_kernel<", 1, 420
+        child = [sys.executable, os.path.abspath(__file__), "--config", "c4", "--brick-log2", str(args.brick_log2),
+                 "--no-graph", *common]
+    else:
+        kname, limit = MARCH_KERNELS[args.kernel], 120
+        child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--kernel", args.kernel,
+                 "--shading", args.shading, "--brick-log2", str(args.brick_log2), "--batch", str(args.batch),
+                 "--views", str(views), "--no-host", *common]
     passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"], "tcc": ["TCC_HIT_sum", "TCC_MISS_sum"]}
     got = {}
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     for name, counters in passes.items():
         outdir = tempfile.mkdtemp(prefix=f"hipdvr_pmc_{name}_")
-        cmd = ["timeout", "-s", "KILL", "120", exe, "--pmc", *counters, "--output-format", "csv", "-d", outdir,
+        cmd = ["timeout", "-s", "KILL", str(limit), exe, "--pmc", *counters, "--output-format", "csv", "-d", outdir,
                "-o", "run", "--", *child]
+        print(f"[pmc] pass {' '.join(counters)} ...", file=sys.stderr, flush=True)
         try:
-            subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, timeout=150)
+            subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, timeout=limit + 30)
         except (subprocess.TimeoutExpired, OSError):
             shutil.rmtree(outdir, ignore_errors=True)
             return None
@@ -156,9 +163,10 @@ def pmc_leg(args, views=4):
     gib_kib = 1024.0 * 1024.0
     f_fac, w_fac = gib_kib / cf[-1], gib_kib / cw[-1]
     traffic = (np.mean(fetch) * f_fac + np.mean(write) * w_fac) * 1024.0
+    what = "one eager batched launch of all views" if c4 else f"views 0..{views - 1} of the orbit"
     return {"traffic_bytes": int(traffic), "l2_hit_rate": round(float(np.sum(hit) / (np.sum(hit) + np.sum(miss))), 4),
             "fetch_factor": round(f_fac, 3), "write_factor": round(w_fac, 3),
-            "source": f"rocprofv3 --pmc in this run: views 0..{views - 1} of the orbit, FETCH_SIZE/WRITE_SIZE "
+            "source": f"rocprofv3 --pmc in this run: {what}, FETCH_SIZE/WRITE_SIZE "
                       "(KiB per dispatch) scaled by a 1 GiB copy kernel, TCC_HIT/TCC_MISS summed over channels"}
 
 
@@ -336,7 +344,8 @@ def main_batch(args):
     b = H.Batch(ctx, jobs)
     tuned, tuned_ms = (b.config(), None) if args.no_autotune else b.autotune(reps=2)
     st = b.launch(stats=True)
-    b.capture()
+    if not args.no_graph:
+        b.capture()
     setup_s = time.perf_counter() - t_setup
     for _ in range(args.warmup):
         b.launch()
@@ -389,11 +398,15 @@ def main_c4(args):
     """Large-volume stress: 2048x2048x1024 uint16 turbulence (8 GiB, 40% of
     the voxels empty) made on the device, 8 views of 3840x2160 with Phong.
     One step = the 8 views in one batched persistent launch (plain path).
-    Beside it: the same views as single frames, learned stepping per view
-    (network trained on plain traces of a 512^3 volume of the same generator:
+    Beside it: the same views as single frames, learned stepping per view and
+    all 8 learned views in one batched launch (hipdvr_batch_launch_learned;
+    network trained on plain traces of a 512^3 volume of the same generator:
     the trace kernel addresses the tiled copy with 32-bit offsets, below the
     large volume's 8 GiB) with each view's PSNR against the plain one, and
     every 32nd pixel of an unshaded view 0 against the CPU oracle."""
+    # counter passes first: child processes under rocprofv3, before this
+    # process touches the GPU
+    pmc = None if args.no_pmc else pmc_leg(args, c4=True)
     from voreen_amd import hipdvr as H, workloads as W, stepper
     cfg = W.CONFIGS["c4"]
     dims = cfg["dims"]
@@ -441,7 +454,8 @@ def main_c4(args):
     b = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, outs)])
     b.set_config(H.TILE_8x8, args.brick_log2, 2)
     st = b.launch(stats=True)
-    b.capture()
+    if not args.no_graph:
+        b.capture()
     setup_s = time.perf_counter() - t_setup
     for _ in range(args.warmup):
         b.launch()
@@ -475,12 +489,27 @@ def main_c4(args):
         "batched_equals_single_frames": bool(batched_equal),
         "single_frames_fps": round(nviews / single_s, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc["traffic_bytes"] if pmc else None},
         "setup_s": round(setup_s, 2),
     }
+    if pmc:
+        out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
+        out["roofline"]["counter_calibration"] = {"fetch": pmc["fetch_factor"], "write": pmc["write_factor"]}
+        out["roofline"]["traffic_source"] = pmc["source"]
+        out["roofline"]["algorithmic_bytes_per_launch"] = int(bytes_step)
+        out["roofline"]["reuse"] = round(bytes_step / pmc["traffic_bytes"], 3)
     assert st["samples"] == samples and st["fetched"] == fetched and st["shaded"] == shaded
     for o in outs:
         o.free()
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        ca, cb = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_device(cb, ca, nb)
+        ctx.synchronize()
+        ca.free()
+        cb.free()
+        out["copy_calibration_bytes"] = nb
 
     if not args.no_learned:
         tdims = (512, 512, 512)
@@ -497,6 +526,7 @@ def main_c4(args):
         progress(f"network trained on {tdims}: {train_s:.1f} s")
         lr = H.Learned(ctx, v)
         dl = [ctx.alloc(img_bytes) for _ in range(nviews)]
+        bl = None
         try:
             lr.prepare(t, params[0], mlp)
             lst = [lr.render_device(t, de, dx, wimg, himg, p, o, stats=True)
@@ -510,7 +540,29 @@ def main_c4(args):
                 progress("learned views done")
             lfps = nviews * args.steps / (time.perf_counter() - t0)
             psnrs = [psnr_db(o.download((himg, wimg, 4), np.float32), ref) for o, ref in zip(dl, plain_imgs)]
+            learned_imgs = [o.download((himg, wimg, 4), np.float32) for o in dl]
+            # the same views in one batched launch with learned stepping
+            bl = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, dl)])
+            bl.set_config(H.TILE_8x8, args.brick_log2, 2)
+            bl.set_learned([lr] * nviews)
+            bst = bl.launch_learned(stats=True)
+            for _ in range(args.warmup):
+                bl.launch_learned()
+            ctx.synchronize()
+            ctx.timing_begin(args.steps)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                bl.launch_learned()
+                ctx.synchronize()
+                progress("batched learned launch done")
+            bfps = nviews * args.steps / (time.perf_counter() - t0)
+            bms = ctx.timing_read()
+            bimgs = [o.download((himg, wimg, 4), np.float32) for o in dl]
+            bpsnrs = [psnr_db(x, ref) for x, ref in zip(bimgs, plain_imgs)]
+            b_equal = all(np.array_equal(x, y) for x, y in zip(bimgs, learned_imgs))
         finally:
+            if bl is not None:
+                bl.close()
             for o in dl:
                 o.free()
             lr.close()
@@ -519,6 +571,12 @@ def main_c4(args):
                           "psnr_min_db": round(min(psnrs), 2), "psnr_mean_db": round(float(np.mean(psnrs)), 2),
                           "views_fell_back": int(sum(s["fell_back"] for s in lst)),
                           "reconstructed_per_view": int(sum(s["samples"] for s in lst) / nviews),
+                          "batched": {"fps": round(bfps, 4), "speedup_vs_batched_plain": round(bfps / out["value"], 3),
+                                      "ms_per_launch": round(sum(bms) / len(bms), 3),
+                                      "psnr_min_db": round(min(bpsnrs), 2),
+                                      "views_fell_back": int(sum(s["fell_back"] for s in bst)),
+                                      "equals_single_view_learned": bool(b_equal),
+                                      "reconstructed_per_launch": int(bst[0]["samples"])},
                           "train": {"volume": "512^3 turbulence, same generator", "tfs": info["tfs"],
                                     "views": info["views"], "rows": info["rows"], "seconds": round(train_s, 1)}}
 

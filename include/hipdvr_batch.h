@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "hipdvr.h"
+#include "hipdvr_learned.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -63,6 +64,26 @@ int hipdvr_batch_capture(hipdvr_batch* b);
 /* Asynchronous on the context stream: the graph when captured, else a direct
  * launch; with stats != NULL it is a counted, synchronous launch. */
 int hipdvr_batch_launch(hipdvr_batch* b, hipdvr_stats* stats);
+
+/* Learned stepping in the batch: learned[i] (in the order the jobs were given
+ * to hipdvr_batch_create; NULL keeps job i plain) is a learned state of job
+ * i's volume whose last hipdvr_learned_prepare was for job i's TF.  Used by
+ * hipdvr_batch_launch_learned only (hipdvr_batch_launch and a captured graph
+ * stay plain).  Jobs sharing a learned state share its step; jobs shading
+ * with the exact Phong path cannot be learned.  A later prepare of a state
+ * (a new network, the same TF) takes effect at the next launch; the states
+ * must outlive their use by the batch.  Calling again replaces the set. */
+int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, uint32_t n,
+                             const hipdvr_learned_opts* opts);
+/* One batched launch with learned stepping: the learned and plain jobs share
+ * the persistent launch; beside it, on a second stream, the judge re-marches
+ * each learned view's sparse pixels plainly; after it each learned view is
+ * judged and, below the PSNR floor, rendered plainly (the steps of
+ * hipdvr_render_learned_device, for all views of the batch at once).
+ * Asynchronous unless stats != NULL: n_jobs entries in the caller's order,
+ * sparse_psnr and fell_back per learned job, samples and shaded the totals
+ * of the whole launch. */
+int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats);
 
 #ifdef __cplusplus
 }

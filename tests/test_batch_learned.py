@@ -1,0 +1,224 @@
+"""Learned stepping inside the batched persistent launch
+(hipdvr_batch_set_learned / hipdvr_batch_launch_learned): a learned job's
+image and judge equal its single-view learned render
+(hipdvr_render_learned_device) bit for bit, plain jobs of the same launch equal
+their plain frames, a bad network trips that job's judge so it falls back to
+the plain frame exactly, and invalid learned sets are refused."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def scene(ctx):
+    """A 128^3 u16 turbulence-shell volume (learned) and a 64^3 u8 blob volume
+    (plain); four views of the first, one of the second, ragged images."""
+    from voreen_amd import hipdvr as H, stepper, workloads as W
+    dims = (128, 128, 128)
+    vol = W.make_volume("c1", dims=dims)
+    llf, urb = W.world_box(dims)
+    sp = tuple(float((urb[i] - llf[i]) / dims[i]) for i in range(3))
+    off = tuple(float(llf[i] + 0.5 * sp[i]) for i in range(3))
+    v1 = ctx.volume(vol, spacing=sp, offset=off)
+    keys, _ = W.make_tf("c1")
+    t1 = ctx.transfunc(H.tf_from_keys(keys, 4096))
+    t1b = ctx.transfunc(H.tf_from_keys(keys, 4096))      # same table, another TF object
+    bdims = (64, 64, 64)
+    v2 = ctx.volume(H.synth_blobs(11, bdims, H.U8))
+    t2 = ctx.transfunc(H.tf_from_keys(W.random_keys(5, 5), 256))
+    step1 = W.step_for(dims, 2.0)
+    views = []
+    for vol_h, tf, d, view, (w, h) in ((v1, t1, dims, 5, (256, 256)), (v1, t1, dims, 9, (200, 150)),
+                                       (v2, t2, bdims, 3, (96, 96)), (v1, t1, dims, 13, (128, 128))):
+        cam = W.orbit_camera(view, 36)
+        lo, hi = W.world_box(d)
+        de, dx, do, dr = (ctx.alloc(w * h * 16) for _ in range(4))
+        ctx.entry_exit_device(cam, lo, hi, w, h, de, dx)
+        views.append(dict(v=vol_h, t=tf, de=de, dx=dx, do=do, dr=dr, w=w, h=h, eye=tuple(cam.eye),
+                          step=step1 if vol_h is v1 else W.step_for(bdims, 2.0)))
+    say("train stepping network for the batched learned tests (24 TFs x 8 views)")
+    mlp, info = stepper.train(ctx, v1, dims, n_tfs=24, n_views=8, trace_size=128, epochs=6, tau_step=4e-3,
+                              tau_skip=2e-3)
+    print(f"training: {info}", flush=True)
+    yield dict(v1=v1, v2=v2, t1=t1, t1b=t1b, t2=t2, views=views, mlp=mlp)
+    mlp.close()
+    for j in views:
+        for b in ("de", "dx", "do", "dr"):
+            j[b].free()
+    for o in (v1, v2, t1, t1b, t2):
+        o.close()
+
+
+def _params(j, shading, exact=0):
+    from voreen_amd import hipdvr as H
+    return H.default_params(j["step"], shading=shading, camera_pos=j["eye"], exact_shading=exact)
+
+
+def _batch(ctx, views, shading, exact=0):
+    from voreen_amd import hipdvr as H
+    return H.Batch(ctx, [(j["v"], j["t"], j["de"], j["dx"], j["do"], j["w"], j["h"], _params(j, shading, exact))
+                         for j in views])
+
+
+def _img(buf, j):
+    return buf.download((j["h"], j["w"], 4), np.float32)
+
+
+def _clear(views):
+    for j in views:
+        j["do"].upload(np.full((j["h"], j["w"], 4), -1.0, np.float32))
+
+
+def _plain(ctx, j, shading):
+    from voreen_amd import hipdvr as H
+    p = H.default_params(j["step"], shading=shading, camera_pos=j["eye"], kernel=H.KERNEL_CELL)
+    ctx.render_device(j["v"], j["t"], j["de"], j["dx"], j["w"], j["h"], p, j["dr"])
+    return _img(j["dr"], j)
+
+
+@pytest.mark.parametrize("shading", [0, 1])
+def test_batch_learned_equals_single_learned(ctx, scene, shading):
+    from voreen_amd import hipdvr as H
+    views = scene["views"]
+    l1 = H.Learned(ctx, scene["v1"])
+    b = _batch(ctx, views, shading)
+    try:
+        l1.prepare(scene["t1"], _params(views[0], shading), scene["mlp"])
+        opts = H.learned_opts()
+        learned = [l1, l1, None, None]
+        # single-view references: learned renders and plain frames
+        ref, ref_st = [], []
+        for j, l in zip(views, learned):
+            if l is None:
+                ref.append(_plain(ctx, j, shading))
+                ref_st.append(None)
+            else:
+                s = l.render_device(j["t"], j["de"], j["dx"], j["w"], j["h"], _params(j, shading), j["dr"],
+                                    opts=opts, stats=True)
+                ref.append(_img(j["dr"], j))
+                ref_st.append(s)
+        b.set_learned(learned, opts)
+        for cfg in ((0, 4, 4), (1, 3, 2), (2, 5, 4)):
+            say(f"batched learned shading {shading} config {cfg}")
+            b.set_config(*cfg)
+            _clear(views)
+            st = b.launch_learned(stats=True)
+            for i, (j, r) in enumerate(zip(views, ref)):
+                assert np.array_equal(_img(j["do"], j), r), f"job {i} config {cfg}"
+                if ref_st[i] is not None:
+                    assert st[i]["sparse_psnr"] == ref_st[i]["sparse_psnr"], f"job {i}"
+                    assert st[i]["fell_back"] == ref_st[i]["fell_back"], f"job {i}"
+                else:
+                    assert st[i]["fell_back"] == 0 and st[i]["sparse_psnr"] == 0.0
+            print(f"judge {[round(s['sparse_psnr'], 1) for s in st]} samples {st[0]['samples']}", flush=True)
+        # asynchronous launch, then the plain launch of the same batch
+        _clear(views)
+        b.launch_learned()
+        ctx.synchronize()
+        for i, (j, r) in enumerate(zip(views, ref)):
+            assert np.array_equal(_img(j["do"], j), r), f"async job {i}"
+        _clear(views)
+        b.launch()
+        ctx.synchronize()
+        for i, j in enumerate(views):
+            assert np.array_equal(_img(j["do"], j), _plain(ctx, j, shading)), f"plain launch job {i}"
+    finally:
+        b.close()
+        l1.close()
+
+
+def test_batch_learned_bad_network_falls_back_per_job(ctx, scene):
+    from voreen_amd import hipdvr as H
+    views = scene["views"]
+    good, bad = H.Learned(ctx, scene["v1"]), H.Learned(ctx, scene["v1"])
+    m = H.Mlp(ctx, seed=3)
+    b = _batch(ctx, views, 0)
+    try:
+        w = m.params()
+        w[-2] = 50.0      # b3[0]: skip logit for every brick
+        m.set_params(w)
+        bad.prepare(scene["t1"], _params(views[0], 0), m)
+        good.prepare(scene["t1"], _params(views[0], 0), scene["mlp"])
+        say("batched learned: one job with a skip-everything network")
+        b.set_learned([bad, good, None, good])
+        _clear(views)
+        st = b.launch_learned(stats=True)
+        print(f"judge {[round(s['sparse_psnr'], 1) for s in st]} fell_back {[s['fell_back'] for s in st]}",
+              flush=True)
+        assert st[0]["fell_back"] == 1 and st[0]["sparse_psnr"] < 45.0
+        assert np.array_equal(_img(views[0]["do"], views[0]), _plain(ctx, views[0], 0))
+        for i in (1, 3):
+            img, plain = _img(views[i]["do"], views[i]), _plain(ctx, views[i], 0)
+            if st[i]["fell_back"]:
+                assert np.array_equal(img, plain), f"job {i}"
+            else:
+                mse = float(np.mean((img.astype(np.float64) - plain) ** 2))
+                assert mse == 0 or 10 * np.log10(1 / mse) >= 45.0, f"job {i}"
+        assert np.array_equal(_img(views[2]["do"], views[2]), _plain(ctx, views[2], 0))
+        # a later prepare of a state (the no-network table in place of the bad
+        # one) takes effect at the next launch without a new set
+        bad.prepare(scene["t1"], _params(views[0], 0), None)
+        st = b.launch_learned(stats=True)
+        assert st[0]["fell_back"] == 0 and st[0]["sparse_psnr"] > 60.0
+    finally:
+        b.close()
+        good.close()
+        bad.close()
+        m.close()
+
+
+def test_batch_learned_rejects_invalid_sets(ctx, scene):
+    from voreen_amd import hipdvr as H
+    views = scene["views"]
+    l1, l2 = H.Learned(ctx, scene["v1"]), H.Learned(ctx, scene["v2"])
+    b = _batch(ctx, views, 0)
+    try:
+        say("batched learned: invalid sets")
+        with pytest.raises(H.HipDvrError) as e:
+            b.launch_learned()
+        assert e.value.code == H.ERR_INVALID
+        with pytest.raises(H.HipDvrError) as e:      # not prepared
+            b.set_learned([l1, None, None, None])
+        assert e.value.code == H.ERR_INVALID
+        l1.prepare(scene["t1"], _params(views[0], 0), None)
+        l2.prepare(scene["t2"], _params(views[2], 0), None)
+        with pytest.raises(H.HipDvrError) as e:      # one per job
+            b.set_learned([l1, None])
+        assert e.value.code == H.ERR_INVALID
+        with pytest.raises(H.HipDvrError) as e:      # another volume's state
+            b.set_learned([l2, None, None, None])
+        assert e.value.code == H.ERR_INVALID
+        l1.prepare(scene["t1b"], _params(views[0], 0), None)
+        with pytest.raises(H.HipDvrError) as e:      # prepared for another TF
+            b.set_learned([l1, None, None, None])
+        assert e.value.code == H.ERR_INVALID
+        l1.prepare(scene["t1"], _params(views[0], 0), None)
+        b.set_learned([l1, None, l2, None])          # accepted
+        b.launch_learned()
+        ctx.synchronize()
+        # jobs sharing a state with different steps
+        odd = [dict(j) for j in views]
+        odd[1]["step"] = odd[1]["step"] * 0.5
+        b2 = _batch(ctx, odd, 0)
+        try:
+            with pytest.raises(H.HipDvrError) as e:
+                b2.set_learned([l1, l1, None, None])
+            assert e.value.code == H.ERR_INVALID
+        finally:
+            b2.close()
+        b3 = _batch(ctx, views, 1, exact=1)          # exact Phong
+        try:
+            with pytest.raises(H.HipDvrError) as e:
+                b3.set_learned([l1, None, None, None])
+            assert e.value.code == H.ERR_UNSUPPORTED
+        finally:
+            b3.close()
+    finally:
+        b.close()
+        l1.close()
+        l2.close()

@@ -98,7 +98,8 @@ def build(verbose=True, incremental=False):
             list(ex.map(_compile, todo))
     objs = [os.path.join(BUILD_DIR, s + ".o") for s in srcs]
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-o", tmp, *objs, "-lpthread"]
+    cmd = [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs,
+           "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

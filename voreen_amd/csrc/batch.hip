@@ -11,14 +11,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "../../include/hipdvr_batch.h"
+#include "../../include/hipdvr_learned.h"
 #include "ctx.h"
 #include "errors.h"
 #include "kernels.h"
+#include "learned.h"
+#include "learned_ray.h"
 #include "march_cell.h"
 #include "objects.h"
 #include "volume.h"
@@ -40,7 +44,7 @@ __global__ void batch_reset_kernel(uint32_t* counter) {
 }
 
 template <bool WIDE, int SHADE, bool COUNT, int K, int TILE>
-__global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict__ jobs,
+__global__ __launch_bounds__(256) void batch_kernel(const LearnedArgs* __restrict__ jobs,
                                                     const uint32_t* __restrict__ tile_start, uint32_t njobs,
                                                     uint32_t total_tiles, uint32_t* __restrict__ counter,
                                                     unsigned long long* __restrict__ counters) {
@@ -65,10 +69,13 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
             else hi = mid;
         }
         const int job = (int)__builtin_amdgcn_readfirstlane(lo);
-        const MarchArgs& a = jobs[job];
+        const LearnedArgs& la = jobs[job];
+        const MarchArgs& a = la.m;
+        // a job with a stepping table marches with learned stepping
+        const bool learned = la.table != nullptr;
         if (job != cur_job) {
             for (int i = tid; i < a.tf_width; i += 256) s_tf[i] = a.tf[i];
-            s_alpha[tid] = a.alpha_corr[tid];
+            s_alpha[tid] = learned ? la.alpha_m[tid] : a.alpha_corr[tid];
             __syncthreads();
             cur_job = job;
         }
@@ -84,7 +91,20 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
             if (!is_background(e, l)) {
                 // WIDE: 64-bit record addressing for every job (a volume's
                 // records exceed 4 GiB); else 32-bit buffer offsets
-                if (a.dtype == 0) {
+                if (learned) {
+                    unsigned long long ns = 0, nsh = 0;
+                    if (a.dtype == 0) {
+                        const CellFetch<uint8_t, WIDE> f = cell_fetch_of<uint8_t, WIDE>(a);
+                        res = learned_ray<uint8_t, WIDE, SHADE != 0>(la, f, s_tf, s_alpha, px, py, e, l, ns, nsh);
+                    } else {
+                        const CellFetch<uint16_t, WIDE> f = cell_fetch_of<uint16_t, WIDE>(a);
+                        res = learned_ray<uint16_t, WIDE, SHADE != 0>(la, f, s_tf, s_alpha, px, py, e, l, ns, nsh);
+                    }
+                    ++n.rays;
+                    n.samples += ns;
+                    n.fetched += ns;
+                    n.shaded += nsh;
+                } else if (a.dtype == 0) {
                     const CellFetch<uint8_t, WIDE> f = cell_fetch_of<uint8_t, WIDE>(a);
                     res = cell_ray<uint8_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
                 } else {
@@ -105,7 +125,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const MarchArgs* __restrict_
     }
 }
 
-typedef void (*BatchKernel)(const MarchArgs*, const uint32_t*, uint32_t, uint32_t, uint32_t*,
+typedef void (*BatchKernel)(const LearnedArgs*, const uint32_t*, uint32_t, uint32_t, uint32_t*,
                             unsigned long long*);
 
 template <bool WIDE, int SHADE, bool COUNT>
@@ -140,12 +160,29 @@ using namespace hipdvr;
 struct hipdvr_batch {
     hipdvr_ctx* ctx = nullptr;
     std::vector<hipdvr_batch_job> jobs;    // in launch order (largest first)
+    std::vector<uint32_t> order;           // caller's index of each launch position
     int shade = 0;                         // 0 none, 1 Phong, 2 exact Phong
     bool wide = false;                     // some volume's cell records exceed 4 GiB
     int sms = 0;
-    // per brick size: device brick tables per job and the jobs' MarchArgs
+    // per brick size: device brick tables per job and the jobs' arguments
+    // (march arguments, plus the stepping table of learned jobs)
     std::vector<void*> allocations;
-    MarchArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
+    std::vector<LearnedArgs> h_args[kBrickSizes];
+    LearnedArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
+    // the same with the stepping tables of the learned jobs
+    // (hipdvr_batch_launch_learned; hipdvr_batch_launch stays plain)
+    LearnedArgs* d_largs[kBrickSizes] = {nullptr, nullptr, nullptr};
+    // learned jobs (hipdvr_batch_set_learned), per launch position: the judge's
+    // grid, sparse image, segments, sum and flag
+    std::vector<hipdvr_learned*> learned;
+    hipdvr_learned_opts lopts{};
+    std::vector<hipdvr::JudgeGrid> grids;
+    std::vector<float4*> d_sparse;
+    std::vector<float4*> d_seg;
+    std::vector<float*> d_sse;
+    std::vector<int*> d_flag;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     uint32_t* d_tile_start[kTileShapes] = {nullptr, nullptr, nullptr};
     uint32_t total_tiles[kTileShapes] = {0, 0, 0};
     uint32_t* d_counter = nullptr;
@@ -167,6 +204,9 @@ void free_batch(hipdvr_batch* b) {
     if (!b) return;
     drop_graph(b);
     for (void* p : b->allocations) (void)hipFree(p);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     delete b;
 }
 
@@ -184,7 +224,7 @@ void validate_config(const hipdvr_batch_config& c) {
     if (c.batch != 2 && c.batch != 4) fail(HIPDVR_ERR_INVALID, "batch must be 2 or 4");
 }
 
-void launch_batch(hipdvr_batch* b, bool count) {
+void launch_batch(hipdvr_batch* b, bool count, bool learned = false) {
     hipStream_t st = ctx_stream(b->ctx);
     const int t = b->cfg.tile, bs = b->cfg.brick_log2 - 3;
     BatchKernel k = kernel_for(b->wide, b->shade, count, t, b->cfg.batch);
@@ -198,7 +238,8 @@ void launch_batch(hipdvr_batch* b, bool count) {
     uint32_t grid = (uint32_t)(b->sms * per_cu);
     if (grid > b->total_tiles[t]) grid = b->total_tiles[t] ? b->total_tiles[t] : 1;
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->d_counter);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, (const MarchArgs*)b->d_args[bs],
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st,
+                       (const LearnedArgs*)(learned ? b->d_largs[bs] : b->d_args[bs]),
                        (const uint32_t*)b->d_tile_start[t], (uint32_t)b->jobs.size(), b->total_tiles[t],
                        b->d_counter, b->ctx->d_counters);
     hip_check(hipGetLastError(), "batch launch");
@@ -248,13 +289,15 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             };
             std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
             for (uint32_t i : order) b->jobs.push_back(jobs[i]);
+            b->order = order;
             // cell records, then per brick size the brick tables of every job
             for (const hipdvr_batch_job& j : b->jobs) {
                 ensure_cells(const_cast<hipdvr_volume*>(j.volume), st);
                 b->wide = b->wide || j.volume->cells_wide;
             }
-            std::vector<MarchArgs> args(n);
             for (int bs = 0; bs < kBrickSizes; ++bs) {
+                std::vector<LearnedArgs>& args = b->h_args[bs];
+                args.assign(n, LearnedArgs{});
                 const uint32_t bl = (uint32_t)bs + 3;
                 for (uint32_t i = 0; i < n; ++i) {
                     const hipdvr_batch_job& j = b->jobs[i];
@@ -283,7 +326,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
                     o.alpha_prefix = j.tf->d_alpha_prefix;
                     o.bits = bits;
                     launch_occupancy(o, st);
-                    MarchArgs& a = args[i];
+                    MarchArgs& a = args[i].m;
                     fill_march_args(a, v, j.tf, &j.params, j.d_entry, j.d_exit, j.d_out, j.w, j.h, nullptr);
                     a.cells = v->d_cells;
                     a.brick_log2 = bl;
@@ -292,10 +335,11 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
                     a.occ_bits = bits;
                     a.occ_words = words;
                 }
-                b->d_args[bs] = dev_alloc<MarchArgs>(b, n * sizeof(MarchArgs));
-                hip_check(hipMemcpyAsync(b->d_args[bs], args.data(), n * sizeof(MarchArgs), hipMemcpyHostToDevice, st),
+                b->d_args[bs] = dev_alloc<LearnedArgs>(b, n * sizeof(LearnedArgs));
+                hip_check(hipMemcpyAsync(b->d_args[bs], args.data(), n * sizeof(LearnedArgs), hipMemcpyHostToDevice,
+                                         st),
                           "upload");
-                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");   // args is reused
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
             }
             for (int t = 0; t < kTileShapes; ++t) {
                 std::vector<uint32_t> start(n + 1, 0);
@@ -312,6 +356,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
                 hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
             }
             b->d_counter = dev_alloc<uint32_t>(b, 64);
+            b->learned.assign(n, nullptr);
             hip_check(hipGetLastError(), "batch setup");
             hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         } catch (...) {
@@ -437,6 +482,167 @@ int hipdvr_batch_autotune(hipdvr_batch* b, uint32_t reps, hipdvr_batch_config* b
         b->cfg = win;
         if (best) *best = win;
         if (best_ms) *best_ms = win_ms;
+    });
+}
+
+int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, uint32_t n,
+                             const hipdvr_learned_opts* o) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        if (!learned || n != b->jobs.size()) fail(HIPDVR_ERR_INVALID, "one learned state (or NULL) per job");
+        ctx_set_device(b->ctx);
+        hipStream_t st = ctx_stream(b->ctx);
+        const hipdvr_learned_opts lopts = learned_options(o);
+        const size_t nj = b->jobs.size();
+        // built aside and taken over only when every job is accepted
+        std::vector<hipdvr_learned*> lv(nj, nullptr);
+        std::vector<JudgeGrid> grids(nj, JudgeGrid{});
+        std::vector<float4*> sparse(nj, nullptr), seg(nj, nullptr);
+        std::vector<float*> sse(nj, nullptr);
+        std::vector<int*> flag(nj, nullptr);
+        bool any = false;
+        std::vector<LearnedArgs> largs[kBrickSizes];
+        for (int bs = 0; bs < kBrickSizes; ++bs) largs[bs] = b->h_args[bs];
+        for (size_t i = 0; i < nj; ++i) {
+            hipdvr_learned* l = learned[b->order[i]];
+            const hipdvr_batch_job& j = b->jobs[i];
+            if (!l) continue;
+            const std::string job = "job " + std::to_string(b->order[i]);
+            if (learned_volume(l) != j.volume) fail(HIPDVR_ERR_INVALID, job + ": the learned state of another volume");
+            if (learned_prepared_tf(l) != j.tf)
+                fail(HIPDVR_ERR_INVALID, job + ": the learned state was not prepared for the job's TF");
+            if (b->shade == 2) fail(HIPDVR_ERR_UNSUPPORTED, "learned jobs shade with the approximate Phong path");
+            // a learned state holds the opacity tables of one step
+            for (size_t k = 0; k < i; ++k)
+                if (lv[k] == l && b->jobs[k].params.step != j.params.step)
+                    fail(HIPDVR_ERR_INVALID, "jobs sharing a learned state must share the step");
+            for (int bs = 0; bs < kBrickSizes; ++bs) learned_stepping_args(l, &j.params, largs[bs][i]);
+            lv[i] = l;
+            grids[i] = learned_judge_grid(j.w, j.h, lopts);
+            any = true;
+        }
+        for (size_t i = 0; i < nj; ++i) {
+            if (!lv[i]) continue;
+            if (i < b->d_sparse.size() && b->d_sparse[i] && b->grids[i].w == grids[i].w &&
+                b->grids[i].h == grids[i].h) {   // same grid as before: keep its buffers
+                sparse[i] = b->d_sparse[i];
+                seg[i] = b->d_seg[i];
+                sse[i] = b->d_sse[i];
+                flag[i] = b->d_flag[i];
+                continue;
+            }
+            const size_t px = (size_t)grids[i].w * grids[i].h;
+            sparse[i] = dev_alloc<float4>(b, px * sizeof(float4));
+            seg[i] = dev_alloc<float4>(b, sparse_segment_bytes((uint32_t)px));
+            sse[i] = dev_alloc<float>(b, 64);
+            flag[i] = dev_alloc<int>(b, 64);
+        }
+        if (any && !b->side) {
+            hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
+        }
+        for (int bs = 0; bs < kBrickSizes; ++bs) {
+            if (!b->d_largs[bs]) b->d_largs[bs] = dev_alloc<LearnedArgs>(b, nj * sizeof(LearnedArgs));
+            hip_check(hipMemcpyAsync(b->d_largs[bs], largs[bs].data(), nj * sizeof(LearnedArgs), hipMemcpyHostToDevice,
+                                     st),
+                      "upload");
+        }
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        b->lopts = lopts;
+        b->learned.swap(lv);
+        b->grids.swap(grids);
+        b->d_sparse.swap(sparse);
+        b->d_seg.swap(seg);
+        b->d_sse.swap(sse);
+        b->d_flag.swap(flag);
+    });
+}
+
+int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats) {
+    if (!b) return HIPDVR_ERR_INVALID;
+    return ctx_guarded(b->ctx, [&] {
+        ctx_set_device(b->ctx);
+        hipStream_t st = ctx_stream(b->ctx);
+        const size_t nj = b->jobs.size();
+        const int bs = b->cfg.brick_log2 - 3;
+        const int shade = b->shade;
+        bool any = false;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!b->learned[i]) continue;
+            any = true;
+            // the state's opacity tables follow the last step it rendered with
+            learned_ensure_step(b->learned[i], b->jobs[i].params.step);
+        }
+        const bool count = stats != nullptr;
+        if (count) launch_zero_u64(b->ctx->d_counters, kNumCounters, st);
+        const bool timed = ctx_timing_start(b->ctx);
+        // 1. the judges' plain re-marches on the side stream, beside 2.
+        if (any) {
+            hip_check(hipEventRecord(b->fork, st), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(b->side, b->fork, 0), "hipStreamWaitEvent");
+            for (size_t i = 0; i < nj; ++i) {
+                if (!b->learned[i]) continue;
+                const JudgeGrid& g = b->grids[i];
+                MarchArgs sp = b->h_args[bs][i].m;
+                sp.out = b->d_sparse[i];
+                sp.counters = nullptr;
+                sp.grid_w = g.w;
+                sp.grid_h = g.h;
+                sp.stride = (int32_t)g.stride;
+                sp.pix_block = (int32_t)g.block;
+                sp.pix_offset = (int32_t)g.offs;
+                launch_sparse_remarch(sp, sp.dtype, shade, b->jobs[i].volume->cells_wide, b->d_seg[i], b->side);
+            }
+            hip_check(hipEventRecord(b->join, b->side), "hipEventRecord");
+        }
+        // 2. every job, learned or plain, in one persistent launch
+        if (!b->d_largs[0]) fail(HIPDVR_ERR_INVALID, "hipdvr_batch_set_learned has not been called");
+        launch_batch(b, count, true);
+        if (any) {
+            hip_check(hipStreamWaitEvent(st, b->join, 0), "hipStreamWaitEvent");
+            // 3. per learned view: the judge, then the plain frame when flagged
+            const float mse_max = powf(10.f, -b->lopts.psnr_floor / 10.f);
+            for (size_t i = 0; i < nj; ++i) {
+                if (!b->learned[i]) continue;
+                const JudgeGrid& g = b->grids[i];
+                const hipdvr_batch_job& j = b->jobs[i];
+                launch_judge((const float4*)j.d_out, (int)j.w, (int)j.h, b->d_sparse[i], g.w, g.h, (int)g.stride,
+                             (int)g.block, (int)g.offs, b->d_sse[i], b->d_flag[i], mse_max, st);
+                MarchArgs fb = b->h_args[bs][i].m;
+                fb.counters = nullptr;
+                fb.cond_flag = b->d_flag[i];
+                launch_march_cell(fb, fb.dtype, shade, false, j.volume->cells_wide, default_batch(shade), st);
+            }
+        }
+        hip_check(hipGetLastError(), "batched learned launch");
+        if (timed) ctx_timing_stop(b->ctx);
+        if (count) {
+            unsigned long long c[kNumCounters];
+            std::vector<float> sse(nj, 0.f);
+            std::vector<int> flag(nj, 0);
+            hip_check(hipMemcpyAsync(c, b->ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st), "download");
+            for (size_t i = 0; i < nj; ++i) {
+                if (!b->learned[i]) continue;
+                hip_check(hipMemcpyAsync(&sse[i], b->d_sse[i], 4, hipMemcpyDeviceToHost, st), "download");
+                hip_check(hipMemcpyAsync(&flag[i], b->d_flag[i], 4, hipMemcpyDeviceToHost, st), "download");
+            }
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            for (size_t i = 0; i < nj; ++i) {
+                hipdvr_learned_stats& s = stats[b->order[i]];
+                s.samples = c[1];
+                s.shaded = c[3];
+                if (b->learned[i]) {
+                    const JudgeGrid& g = b->grids[i];
+                    const double mse = (double)sse[i] / ((double)g.w * g.h * 4);
+                    s.sparse_psnr = mse > 0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.f;
+                    s.fell_back = flag[i];
+                } else {
+                    s.sparse_psnr = 0.f;
+                    s.fell_back = 0;
+                }
+            }
+        }
     });
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/hipdvr_learned.h"
 #include "march_common.h"
 
 namespace hipdvr {
@@ -47,6 +48,22 @@ void launch_targets(const float* err, const unsigned int* visits, const float* f
                     float tau_skip, float tau_step, float* out_x, float* out_t, unsigned int* count,
                     uint32_t capacity, hipStream_t st);
 void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hipStream_t st);
+
+// For batched launches (batch.hip): the judge's sparse grid of a w x h image
+// (host arithmetic), the options with their defaults, the stepping fields of
+// a LearnedArgs for one job (table, occupancy, opacity tables for the job's
+// step; la.m is left to the caller), the opacity tables remade for a step
+// (nothing when they hold it), and the TF of the last prepare (NULL before).
+struct JudgeGrid {
+    uint32_t stride, block, offs;
+    int w, h;
+};
+JudgeGrid learned_judge_grid(uint32_t w, uint32_t h, const hipdvr_learned_opts& opts);
+hipdvr_learned_opts learned_options(const hipdvr_learned_opts* o);
+void learned_stepping_args(hipdvr_learned* l, const hipdvr_params* p, LearnedArgs& la);
+const hipdvr_volume* learned_volume(const hipdvr_learned* l);
+void learned_ensure_step(hipdvr_learned* l, float step);
+const hipdvr_tf* learned_prepared_tf(const hipdvr_learned* l);
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
                   int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st);
 

@@ -211,6 +211,7 @@ struct hipdvr_learned {
     float* d_alpha_m = nullptr;      // [8][256] corrected alpha for step * 1 .. 8
     std::vector<float> h_alpha_m;    // host copy (source of the last upload)
     float alpha_m_step = -1.f;       // step the tables were made for
+    const hipdvr_tf* prepared_tf = nullptr;   // TF of the last hipdvr_learned_prepare
     float4* d_sparse = nullptr;      // sparse plain image
     size_t sparse_cap = 0;
     float4* d_seg = nullptr;         // its ray segments
@@ -227,6 +228,7 @@ struct hipdvr_learned {
 namespace hipdvr {
 hipdvr_ctx* learned_ctx(const hipdvr_learned* l) { return l->ctx; }
 const hipdvr_volume* learned_volume(const hipdvr_learned* l) { return l->vol; }
+const hipdvr_tf* learned_prepared_tf(const hipdvr_learned* l) { return l->prepared_tf; }
 }  // namespace hipdvr
 
 namespace {
@@ -385,6 +387,28 @@ void fill_learned_args(LearnedArgs& la, hipdvr_learned* l, const hipdvr_tf* tf, 
 
 }  // namespace
 
+namespace hipdvr {
+
+JudgeGrid learned_judge_grid(uint32_t w, uint32_t h, const hipdvr_learned_opts& opts) {
+    const SparseGrid g = sparse_dims(w, h, (uint32_t)opts.remarch_stride);
+    return JudgeGrid{g.stride, g.block, g.offs, g.w, g.h};
+}
+
+hipdvr_learned_opts learned_options(const hipdvr_learned_opts* o) { return default_opts(o); }
+
+void learned_ensure_step(hipdvr_learned* l, float step) { ensure_alpha_tables(l, step, ctx_stream(l->ctx)); }
+
+void learned_stepping_args(hipdvr_learned* l, const hipdvr_params* p, LearnedArgs& la) {
+    ensure_alpha_tables(l, p->step, ctx_stream(l->ctx));
+    la.u16 = l->vol->desc.dtype == HIPDVR_U16 ? 1 : 0;
+    for (int i = 0; i < 3; ++i) la.nb16[i] = l->nb[i];
+    la.table = l->d_table;
+    la.occ16 = l->d_occ16;
+    la.alpha_m = l->d_alpha_m;
+}
+
+}  // namespace hipdvr
+
 extern "C" {
 
 int hipdvr_learned_create(hipdvr_ctx* ctx, const hipdvr_volume* vol, hipdvr_learned** out) {
@@ -478,6 +502,7 @@ int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_
         launch_merge_occ(l->d_table, l->d_occ16, l->rows, st);
         hip_check(hipGetLastError(), "learned prepare launch");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        l->prepared_tf = tf;
     });
 }
 

@@ -50,6 +50,7 @@ EXPORTS = (
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
     "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
     "hipdvr_batch_autotune", "hipdvr_batch_capture", "hipdvr_batch_launch",
+    "hipdvr_batch_set_learned", "hipdvr_batch_launch_learned",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -216,6 +217,8 @@ def load():
         "hipdvr_batch_autotune": ([vp, u32, P(BatchConfig), P(C.c_float)], C.c_int),
         "hipdvr_batch_capture": ([vp], C.c_int),
         "hipdvr_batch_launch": ([vp, P(Stats)], C.c_int),
+        "hipdvr_batch_set_learned": ([vp, P(vp), u32, P(LearnedOpts)], C.c_int),
+        "hipdvr_batch_launch_learned": ([vp, P(LearnedStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -746,6 +749,7 @@ class Batch:
             arr[i].volume, arr[i].tf = vol.h, tf.h
             arr[i].d_entry, arr[i].d_exit, arr[i].d_out = de.ptr, dx.ptr, do.ptr
             arr[i].w, arr[i].h, arr[i].params = w, h, prm
+        self.n = len(jobs)
         self.h = C.c_void_p()
         ctx.check(ctx.lib.hipdvr_batch_create(ctx.h, arr, len(jobs), C.byref(self.h)), "hipdvr_batch_create")
 
@@ -777,3 +781,19 @@ class Batch:
         self.ctx.check(self.ctx.lib.hipdvr_batch_launch(self.h, C.byref(st) if stats else None),
                        "hipdvr_batch_launch")
         return st.as_dict() if stats else None
+
+    def set_learned(self, learned, opts=None):
+        """learned: one Learned (prepared for that job's TF and step) or None
+        per job, in the order the jobs were given."""
+        arr = (C.c_void_p * len(learned))(*[l.h.value if l is not None else None for l in learned])
+        self.ctx.check(self.ctx.lib.hipdvr_batch_set_learned(self.h, arr, len(learned),
+                                                             C.byref(opts) if opts else None),
+                       "hipdvr_batch_set_learned")
+
+    def launch_learned(self, stats=False):
+        """One batched launch with learned stepping; with stats, one dict per
+        job (samples and shaded are the launch's totals)."""
+        st = (LearnedStats * self.n)()
+        self.ctx.check(self.ctx.lib.hipdvr_batch_launch_learned(self.h, st if stats else None),
+                       "hipdvr_batch_launch_learned")
+        return [s.as_dict() for s in st] if stats else None
