@@ -24,6 +24,7 @@
 
 #include <type_traits>
 
+#include "cell_layout.h"
 #include "march_common.h"
 #include "step_run.h"
 
@@ -82,19 +83,22 @@ struct CellRecord<uint8_t> {
     }
 };
 
-// Record addressing: 2x2x2 cells per group (one 128-byte line for uint16),
-// groups x-fastest; the record index is separable per axis.
+// Record addressing (cell_layout.h): 2x2x2 cells per group (one 128-byte
+// line for uint16), groups x-fastest inside bricks of kCellBrick^3 groups,
+// bricks x-fastest.
 template <typename T, bool WIDE>
 struct CellFetch {
     typedef typename CellRecord<T>::type rec_t;
     typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type off_t;
     const rec_t* p;
     __amdgpu_buffer_rsrc_t rsrc;
-    off_t sy, sz;      // records per group row (8*GX) and per group slab (8*GX*GY)
+    uint32_t nbx, nby;   // bricks along x and y
 
-    __device__ __forceinline__ off_t index(const uint32_t lo[3]) const {
-        return (off_t)(((lo[0] >> 1) << 3) | (lo[0] & 1)) + (off_t)(lo[1] >> 1) * sy + ((lo[1] & 1) << 1) +
-               (off_t)(lo[2] >> 1) * sz + ((lo[2] & 1) << 2);
+    __device__ __forceinline__ off_t index(const uint32_t lo[3]) const { return cell_index<off_t>(lo, nbx, nby); }
+    // brick stride of axis i, in records
+    __device__ __forceinline__ off_t brick_stride(int i) const {
+        const off_t one = (off_t)1 << kCellBrickRecLog2;
+        return i == 0 ? one : (i == 1 ? one * nbx : one * nbx * nby);
     }
     __device__ __forceinline__ rec_t load(off_t i) const {
         if constexpr (WIDE) {
@@ -175,8 +179,9 @@ __device__ __forceinline__ void cell_gradient(const CellFetch<T, WIDE>& f, const
 // in stored units and scaled once by grad_raw = grad_scale * rwm_scale /
 // range, the four-term sums use fused multiply-adds, and the neighbour
 // records are found from the sample's own record index `own` (its low three
-// bits are the parities of lo; a step across a 2x2x2 group moves by 8, sy or
-// sz).  Returns false (and leaves gr) outside the interior.
+// bits are the parities of lo; a step to the neighbouring group moves by the
+// group stride inside a brick, else to the neighbouring brick's facing group;
+// cell_layout.h).  Returns false (and leaves gr) outside the interior.
 template <typename T, bool WIDE>
 __device__ __forceinline__ bool cell_gradient_interior(const CellFetch<T, WIDE>& f, const MarchArgs& a,
                                                        const float s[3],
@@ -189,16 +194,9 @@ __device__ __forceinline__ bool cell_gradient_interior(const CellFetch<T, WIDE>&
         const float u = (a.fdims[i] * s[i]) + (-0.5f);
         if (!(u >= 1.0f) || !(u + 3.0f < a.fdims[i]) || !(p[i] > 0.0f)) return false;
     }
-    const uint32_t par = (uint32_t)own & 7u;
-    const off_t sy2 = f.sy - 2, sz4 = f.sz - 4;
-    const off_t nb[6] = {
-        (par & 1u) ? own - 1 : own - 7,            // x - 1
-        (par & 2u) ? own - 2 : own - sy2,          // y - 1
-        (par & 4u) ? own - 4 : own - sz4,          // z - 1
-        (par & 1u) ? own + 7 : own + 1,            // x + 1
-        (par & 2u) ? own + sy2 : own + 2,          // y + 1
-        (par & 4u) ? own + sz4 : own + 4,          // z + 1
-    };
+    off_t nb[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cell_neighbours<off_t>(own, i, f.brick_stride(i), nb[i], nb[3 + i]);
     typename CellFetch<T, WIDE>::rec_t nr[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) nr[k] = f.load(nb[k]);
@@ -424,11 +422,11 @@ __device__ __forceinline__ CellFetch<T, WIDE> cell_fetch_of(const MarchArgs& a) 
     typedef typename CellFetch<T, WIDE>::rec_t rec_t;
     CellFetch<T, WIDE> f;
     f.p = (const rec_t*)a.cells;
-    const uint32_t GX = (a.dims[0] + 1) >> 1, GY = (a.dims[1] + 1) >> 1, GZ = (a.dims[2] + 1) >> 1;
-    f.sy = (typename CellFetch<T, WIDE>::off_t)GX << 3;
-    f.sz = f.sy * GY;
+    f.nbx = cell_bricks(a.dims[0]);
+    f.nby = cell_bricks(a.dims[1]);
     if (!WIDE) {
-        const uint32_t bytes = f.sz * GZ * (uint32_t)sizeof(rec_t);
+        const uint32_t dims[3] = {a.dims[0], a.dims[1], a.dims[2]};
+        const uint32_t bytes = (uint32_t)(cell_record_count(dims) * sizeof(rec_t));
         f.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.cells, (short)0, (int)bytes, 0x00020000);
     }
     return f;

@@ -4,6 +4,7 @@
 
 #include <string>
 
+#include "cell_layout.h"
 #include "errors.h"
 #include "march_common.h"
 #include "volume.h"
@@ -91,18 +92,16 @@ __global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ src, 
 }
 
 // One lane per record: the 8 footprint voxels of cell (x, y, z), clamped.
+// Records in the order of cell_layout.h; cells of the padding hold zeros.
 template <typename T>
 __global__ __launch_bounds__(256) void cells_kernel(const T* __restrict__ src, void* __restrict__ dst, uint32_t X,
-                                                    uint32_t Y, uint32_t Z, uint32_t gx, uint32_t gy,
-                                                    uint64_t total) {
+                                                    uint32_t Y, uint32_t Z, uint64_t total) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t dims[3] = {X, Y, Z};
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint32_t in = (uint32_t)(i & 7);
-        const uint64_t grp = i >> 3;
-        const uint32_t x = (uint32_t)(grp % gx) * 2 + (in & 1);
-        const uint64_t r = grp / gx;
-        const uint32_t y = (uint32_t)(r % gy) * 2 + ((in >> 1) & 1);
-        const uint32_t z = (uint32_t)(r / gy) * 2 + (in >> 2);
+        uint32_t c[3];
+        cell_of_record(i, dims, c);
+        const uint32_t x = c[0], y = c[1], z = c[2];
         uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (x < X && y < Y && z < Z) {
             const uint32_t x1 = x + 1 < X ? x + 1 : X - 1;
@@ -297,9 +296,7 @@ void launch_brick_minmax(const hipdvr_volume* v, uint32_t bl, const uint32_t nb[
 }
 
 uint64_t cell_record_bytes(const hipdvr_volume* v) {
-    uint64_t n = 8;
-    for (int i = 0; i < 3; ++i) n *= (v->desc.dims[i] + 1) / 2;
-    return n * (v->desc.dtype == HIPDVR_U8 ? 8 : 16);
+    return cell_record_count(v->desc.dims) * (v->desc.dtype == HIPDVR_U8 ? 8 : 16);
 }
 
 bool cells_allowed(const hipdvr_volume* v) { return cell_record_bytes(v) <= kMaxCellBytes; }
@@ -309,16 +306,15 @@ void ensure_cells(hipdvr_volume* v, hipStream_t st) {
     const uint64_t bytes = cell_record_bytes(v);
     if (bytes > kMaxCellBytes) fail(HIPDVR_ERR_UNSUPPORTED, "cell records of this volume exceed the size limit");
     const uint64_t total = bytes / (v->desc.dtype == HIPDVR_U8 ? 8 : 16);
-    const uint32_t gx = (v->desc.dims[0] + 1) / 2, gy = (v->desc.dims[1] + 1) / 2;
     void* p = nullptr;
     hip_check(hipMalloc(&p, bytes), "hipMalloc(cell records)");
     const dim3 grid(stride_grid(total));
     if (v->desc.dtype == HIPDVR_U8)
         hipLaunchKernelGGL(cells_kernel<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)v->d_linear, p,
-                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], gx, gy, total);
+                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], total);
     else
         hipLaunchKernelGGL(cells_kernel<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)v->d_linear, p,
-                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], gx, gy, total);
+                           v->desc.dims[0], v->desc.dims[1], v->desc.dims[2], total);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(st);
     if (e1 != hipSuccess || e2 != hipSuccess) {
