@@ -300,6 +300,7 @@ __global__ __launch_bounds__(256) void trace_kernel(const LearnedArgs la, float*
     float dir[3];
     const float tend = ray_setup(first, last, a.step, dir);
     const uint32_t oct = (dir[0] < 0.f ? 1u : 0u) | (dir[1] < 0.f ? 2u : 0u) | (dir[2] < 0.f ? 4u : 0u);
+    const BoxExit bx = box_exit_setup(a, dir);
     float A = 0.f;
     float t = 0.f;
     int guard = 0;
@@ -313,7 +314,9 @@ __global__ __launch_bounds__(256) void trace_kernel(const LearnedArgs la, float*
         uint32_t bc[3];
         for (int i = 0; i < 3; ++i) bc[i] = (uint32_t)vp[i] >> kLBrickLog2;
         const uint32_t brick = (bc[2] * la.nb16[1] + bc[1]) * la.nb16[0] + bc[0];
-        const float texit = fminf(box_exit(a, first, dir, bc, kLBrick, la.nb16), tend);
+        const float texit = fminf(kLearnedFastExit ? box_exit_fast(bx, first, dir, bc, kLBrick, la.nb16)
+                                                   : box_exit(a, first, dir, bc, kLBrick, la.nb16),
+                                  tend);
         // plain segment and the coarse variants from the same start
         SegAcc seg[4];
         for (int k = 0; k < 4; ++k) seg[k] = SegAcc{0.f, 0.f, 0.f, 0.f};

@@ -26,6 +26,41 @@ __device__ __forceinline__ float box_exit(const MarchArgs& a, const float first[
     return tl;
 }
 
+// box_exit with the divisions taken once per ray: the reciprocals of the
+// direction and of the volume's dims.  The brick faces are then
+// ((k*B + 0.5) * (1/dim) - first) * (1/dir), a few ulps from box_exit.  A
+// wavefront enters its rays' next bricks at different samples, so it runs
+// this on most passes of the march loop; the learned march and the trace
+// that labels its training rows both use it.
+struct BoxExit {
+    float inv_dir[3], inv_dim[3];
+};
+
+__device__ __forceinline__ BoxExit box_exit_setup(const MarchArgs& a, const float dir[3]) {
+    BoxExit x;
+    for (int i = 0; i < 3; ++i) {
+        x.inv_dir[i] = 1.0f / dir[i];
+        x.inv_dim[i] = 1.0f / a.fdims[i];
+    }
+    return x;
+}
+
+__device__ __forceinline__ float box_exit_fast(const BoxExit& x, const float first[3], const float dir[3],
+                                               const uint32_t b[3], uint32_t B, const uint32_t nb[3]) {
+    float tl = INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        if (dir[i] > 0.0f && b[i] + 1 < nb[i]) {
+            const float sb = ((float)((b[i] + 1) * B) + 0.5f) * x.inv_dim[i];
+            tl = fminf(tl, (sb - first[i]) * x.inv_dir[i]);
+        } else if (dir[i] < 0.0f && b[i] > 0) {
+            const float sb = ((float)(b[i] * B) + 0.5f) * x.inv_dim[i];
+            tl = fminf(tl, (sb - first[i]) * x.inv_dir[i]);
+        }
+    }
+    return tl;
+}
+constexpr bool kLearnedFastExit = true;
+
 // ---------------------------------------------------------------------------
 // the learned marcher: the cell-record march (march_cell.h) with the stepping
 // table in place of exact empty-space skipping.  Per 16^3 brick and ray
@@ -62,6 +97,7 @@ __device__ __forceinline__ float4 learned_ray(const LearnedArgs& la, const CellF
     float dir[3];
     const float tend = ray_setup(first, last, a.step, dir);
     const uint32_t oct = (dir[0] < 0.f ? 1u : 0u) | (dir[1] < 0.f ? 2u : 0u) | (dir[2] < 0.f ? 4u : 0u);
+    const BoxExit bx = box_exit_setup(a, dir);
     float t = 0.f, r = 0.f, g = 0.f, bb = 0.f, A = 0.f;
     int guard = 0;
     bool finished = false;
@@ -81,7 +117,8 @@ __device__ __forceinline__ float4 learned_ray(const LearnedArgs& la, const CellF
         if (brick != cur) {
             cur = brick;
             const uint8_t ent = la.table[brick * 8 + oct];
-            t_exit = box_exit(a, first, dir, bc, kLBrick, la.nb16);
+            t_exit = kLearnedFastExit ? box_exit_fast(bx, first, dir, bc, kLBrick, la.nb16)
+                                      : box_exit(a, first, dir, bc, kLBrick, la.nb16);
             skip = (ent & 0x80) != 0;
             mmax = 1 << (ent & 3);
         }
