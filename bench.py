@@ -5,13 +5,19 @@ Workload (default `--config c1`): 512^3 uint16 nested-shell volume, 6-key TF
 (4096 entries), Phong with central-difference gradients, 1024^2 image, step
 1/1024 (samplingRate 2), 36 orbit views.  One bench step = one orbit (36
 frames).  Volume, TF and all entry/exit images are resident in HBM before the
-timed region; the timed region issues the 36 march launches per step on the
-library's stream and synchronizes.
+timed region.  The step's views are rendered twice: as 36 per-frame march
+launches on the library's stream (what HIPRaycaster::process issues; the
+`per_frame` figures) and as one persistent batched launch of all 36 views
+(hipdvr_batch, autotuned, replayed from a captured graph), whose images equal
+the per-frame ones bit for bit.  `value` is the batched rate when a step has
+more than one view (C0's single view: the per-frame launch).
 
-Prints one JSON line: frames/s (value), Gsamples/s, the roofline of the march
-kernel (algorithmic bytes per launch / HIP-event launch time), and the CPU
-baseline (the FP32 CPURaycaster restatement, no shading, on the host's cores,
-on a bounded band of rows of view 0).
+Prints one JSON line: frames/s (value), Gsamples/s, the roofline of the timed
+kernel (algorithmic bytes per launch / HIP-event launch time; HBM traffic and
+L2 hit rate from rocprofv3 counter passes of the same kernel), the per-frame
+launches, the host-buffer path, learned stepping, and the CPU baseline (the
+FP32 CPURaycaster restatement, no shading, on the host's cores, all rays of
+view 0).
 
     python bench.py [--steps K] [--warmup W] [--config c1|c0|c2|c4] [--kernel auto|reference|bricked|cell]
                     [--batch 1|2|4|8]
@@ -68,6 +74,8 @@ def parse():
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate) of this run")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch eagerly (no captured graph)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the leg that renders all views of a step in one batched launch")
     return ap.parse_args()
 
 
@@ -115,15 +123,23 @@ def pmc_leg(args, views=4, c4=False):
     if not os.path.exists(exe):
         return None
     common = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-learned", "--no-pmc", "--copy-calibration", "1024"]
+    from voreen_amd import workloads as W   # host-side tables only; no GPU
+    nviews = args.views or W.CONFIGS[args.config]["views"]
     if c4:
         kname, views, limit = "batch_kernel<", 1, 420
         child = [sys.executable, os.path.abspath(__file__), "--config", "c4", "--brick-log2", str(args.brick_log2),
                  "--no-graph", *common]
+    elif batched_mode(args, nviews):
+        # the timed launch: all views in one batched launch, launched eagerly
+        # after the autotuner's launches (the last batch_kernel dispatch)
+        kname, views, limit = "batch_kernel<", 1, 300
+        child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--shading", args.shading,
+                 "--brick-log2", str(args.brick_log2), "--views", str(nviews), "--no-host", "--no-graph", *common]
     else:
         kname, limit = MARCH_KERNELS[args.kernel], 120
         child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--kernel", args.kernel,
                  "--shading", args.shading, "--brick-log2", str(args.brick_log2), "--batch", str(args.batch),
-                 "--views", str(views), "--no-host", *common]
+                 "--views", str(views), "--no-host", "--no-batched", *common]
     passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"], "tcc": ["TCC_HIT_sum", "TCC_MISS_sum"]}
     got = {}
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
@@ -163,7 +179,8 @@ def pmc_leg(args, views=4, c4=False):
     gib_kib = 1024.0 * 1024.0
     f_fac, w_fac = gib_kib / cf[-1], gib_kib / cw[-1]
     traffic = (np.mean(fetch) * f_fac + np.mean(write) * w_fac) * 1024.0
-    what = "one eager batched launch of all views" if c4 else f"views 0..{views - 1} of the orbit"
+    what = ("one eager batched launch of all views" if kname.startswith("batch") else
+            f"views 0..{views - 1} of the orbit")
     return {"traffic_bytes": int(traffic), "l2_hit_rate": round(float(np.sum(hit) / (np.sum(hit) + np.sum(miss))), 4),
             "fetch_factor": round(f_fac, 3), "write_factor": round(w_fac, 3),
             "source": f"rocprofv3 --pmc in this run: {what}, FETCH_SIZE/WRITE_SIZE "
@@ -247,12 +264,15 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     }
 
 
-def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samples, args):
+def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samples, args, batched=None):
     """Learned stepping on the same frames: train the stepping network on
     plain traces of seeded random TFs (the library's own kernels), then time
     the learned render of every view (learned march + 1/64 sparse plain
     re-march + judge + conditional fallback) and compare each frame with the
-    plain one (PSNR over RGBA)."""
+    plain one (PSNR over RGBA).  With `batched` (the plain batched leg's
+    result): also every view in one batched launch with learned stepping
+    (hipdvr_batch_launch_learned, the same configuration), compared with the
+    plain batched launch; the speedup is then the batched one's."""
     from voreen_amd import hipdvr as H, stepper
     t0 = time.perf_counter()
     mlp, info = stepper.train(ctx, v, dims, n_tfs=args.learned_tfs, n_views=args.learned_views)
@@ -280,12 +300,44 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
             b = o.download((himg, wimg, 4), np.float32).astype(np.float64)
             mse = float(np.mean((a - b) ** 2))
             psnrs.append(999.0 if mse == 0 else 10 * np.log10(1 / mse))
+        bres = None
+        if batched:
+            single = [o.download((himg, wimg, 4), np.float32) for o in dl]
+            b = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, dl)])
+            try:
+                c = batched["config"]
+                b.set_config(c["tile"], c["brick_log2"], c["batch"])
+                b.set_learned([lr] * nviews)
+                bst = b.launch_learned(stats=True)
+                for _ in range(max(args.warmup, 1)):
+                    b.launch_learned()
+                ctx.synchronize()
+                ctx.timing_begin(args.steps)
+                ta = time.perf_counter()
+                for _ in range(args.steps):
+                    b.launch_learned()
+                ctx.synchronize()
+                bfps = nviews * args.steps / (time.perf_counter() - ta)
+                bms = ctx.timing_read()
+                bpsnr = []
+                equal = True
+                for (_, _, do), o, sref in zip(bufs, dl, single):
+                    x = o.download((himg, wimg, 4), np.float32)
+                    equal = equal and np.array_equal(x, sref)
+                    mse = float(np.mean((do.download((himg, wimg, 4), np.float32).astype(np.float64) - x) ** 2))
+                    bpsnr.append(999.0 if mse == 0 else 10 * np.log10(1 / mse))
+            finally:
+                b.close()
+            bres = {"fps": round(bfps, 3), "speedup_vs_batched_plain": round(bfps / batched["fps"], 3),
+                    "ms_per_launch": round(sum(bms) / len(bms), 4), "psnr_min_db": round(min(bpsnr), 2),
+                    "views_fell_back": int(sum(x["fell_back"] for x in bst)),
+                    "equals_single_view_learned": bool(equal)}
     finally:
         for o in dl:
             o.free()
         lr.close()
         mlp.close()
-    return {
+    res = {
         "fps": round(fps, 3),
         "speedup_vs_plain": round(fps / plain_fps, 3),
         "samples_equiv_gsamples_per_s": round(plain_samples * fps / 1e9, 3),
@@ -297,6 +349,16 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
                   "loss_first": round(info["loss_first"], 4), "loss_last": round(info["loss_last"], 4),
                   "seconds": round(train_s, 2)},
     }
+    if bres:
+        # the headline comparison: the same kind of launch on both sides
+        res["per_frame"] = {"fps": res["fps"], "speedup_vs_per_frame_plain": res["speedup_vs_plain"],
+                            "psnr_min_db": res["psnr_min_db"], "views_fell_back": res["views_fell_back"]}
+        res["fps"] = bres["fps"]
+        res["speedup_vs_plain"] = bres["speedup_vs_batched_plain"]
+        res["samples_equiv_gsamples_per_s"] = round(plain_samples * bres["fps"] / 1e9, 3)
+        res["psnr_min_db"] = min(res["psnr_min_db"], bres["psnr_min_db"])
+        res["batched"] = bres
+    return res
 
 
 def host_path_leg(ctx, v, t, bufs, params, wimg, himg, kernel_ms, reps=20):
@@ -383,6 +445,50 @@ def main_batch(args):
         (r.free if hasattr(r, "free") else r.close)()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def batched_mode(args, nviews):
+    """Whether the bench times a step's views in one batched launch: steps of
+    several views with the default kernel choice."""
+    return nviews > 1 and not args.no_batched and args.kernel == "auto" and args.batch == 0
+
+
+def batched_views_leg(ctx, v, t, bufs, params, wimg, himg, args, bytes_orbit):
+    """All views of the step in one persistent launch (hipdvr_batch_*,
+    autotuned configuration, captured graph), each view's image compared with
+    its single-frame image (the per-view launches just timed left them in
+    the output buffers).  A reported rate beside `value`, which stays the
+    per-frame launches."""
+    from voreen_amd import hipdvr as H
+    nviews = len(bufs)
+    ref = [do.download((himg, wimg, 4), np.float32) for _, _, do in bufs]
+    outs = [ctx.alloc(wimg * himg * 16) for _ in range(nviews)]
+    b = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, outs)])
+    try:
+        cfg, _ = b.autotune(reps=2)
+        if not args.no_graph:
+            b.capture()
+        for _ in range(max(args.warmup, 1)):
+            b.launch()
+        ctx.synchronize()
+        ctx.timing_begin(args.steps)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.launch()
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = ctx.timing_read()
+        equal = all(np.array_equal(o.download((himg, wimg, 4), np.float32), r) for o, r in zip(outs, ref))
+    finally:
+        b.close()
+        for o in outs:
+            o.free()
+    launch_s = sum(ms) / len(ms) / 1e3
+    achieved = bytes_orbit / launch_s / 1e9
+    return {"fps": round(nviews * args.steps / elapsed, 3), "ms_per_launch": round(launch_s * 1e3, 4),
+            "views_per_launch": nviews, "config": {"tile": cfg[0], "brick_log2": cfg[1], "batch": cfg[2]},
+            "equals_single_frames": bool(equal),
+            "roofline": {"achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4)}}
 
 
 def progress(msg):
@@ -693,19 +799,43 @@ def main():
     bytes_orbit = fetched * 8 * vsz + shaded * 24 * vsz + 48 * wimg * himg * nviews
     kernel_s = sum(launch_ms) / 1e3
     achieved = bytes_orbit * args.steps / kernel_s / 1e9
+    per_frame = {"fps": round(fps, 4), "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+                 "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 4),
+                 "roofline": {"achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_device(b, a, nb)
+        ctx.synchronize()
+        a.free()
+        b.free()
+    # the step's views in one persistent launch: the timed figure when a step
+    # has more than one view
+    bv = None
+    if batched_mode(args, nviews):
+        bv = batched_views_leg(ctx, v, t, bufs, params, wimg, himg, args, bytes_orbit)
+    if bv:
+        value, step_s = bv["fps"], nviews / bv["fps"]
+        kernel_ms, views_per_launch = bv["ms_per_launch"], nviews
+        achieved = bv["roofline"]["achieved"]
+    else:
+        value, step_s = fps, elapsed / args.steps
+        kernel_ms, views_per_launch = sum(launch_ms) / len(launch_ms), 1
     traffic = None
     if pmc:
         traffic = {"bytes": pmc["traffic_bytes"], "source": pmc["source"]}
-    elif args.kernel == "auto" and args.batch == 0 and args.brick_log2 == 4:
+    elif not bv and args.kernel == "auto" and args.batch == 0 and args.brick_log2 == 4:
         traffic = measured_traffic(args.config, "phong" if cfg["shading"] else "none")
+    launch = (f"all {nviews} views of a step in one persistent batched launch (hipdvr_batch, autotuned, "
+              "graph-replayed); per-frame launches under per_frame" if bv else "one launch per view")
     out = {
         "metric": "frames/s + Gsamples/s at 1024^2, 512^3 u16; learned vs plain at >=45 dB PSNR",
-        "value": round(fps, 4),
+        "value": round(value, 4),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(step_s * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -715,17 +845,19 @@ def main():
             "workload": (f"{args.config}: {dims[0]}x{dims[1]}x{dims[2]} "
                          f"{'u8' if vsz == 1 else 'u16'}, {wimg}x{himg}, step 1/{int(round(1 / step))}, "
                          f"{'phong' if cfg['shading'] else 'no shading'}, {nviews} orbit views per step"),
+            "launch": launch,
             "kernel": args.kernel,
             "batch": args.batch,
             "brick": 1 << args.brick_log2,
             "voxel": "u8" if vsz == 1 else "u16",
         },
-        "gsamples_per_s": round(samples * args.steps / elapsed / 1e9, 4),
+        "gsamples_per_s": round(samples / nviews * value / 1e9, 4),
         "samples_per_frame": samples // nviews,
         "fetched_per_frame": fetched // nviews,
         "shaded_per_frame": shaded // nviews,
         "rays_per_frame": rays // nviews,
-        "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 4),
+        "kernel_ms_avg": round(kernel_ms, 4),
+        "views_per_launch": views_per_launch,
         "simd_efficiency": round(samples / max(lane_slots, 1), 4),
         "shade_efficiency": round(shaded / max(shade_slots, 1), 4),
         "roofline": {
@@ -736,29 +868,27 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic["bytes"] if traffic else None,
         },
+        "per_frame": per_frame,
         "setup_s": round(setup_s, 2),
     }
+    if bv:
+        out["batched_views"] = {k: bv[k] for k in ("config", "equals_single_frames")}
+    per_launch_bytes = bytes_orbit if bv else bytes_orbit / nviews
     if pmc:
         out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
         out["roofline"]["counter_calibration"] = {"fetch": pmc["fetch_factor"], "write": pmc["write_factor"]}
     if traffic:
         out["roofline"]["traffic_source"] = traffic["source"]
-        out["roofline"]["algorithmic_bytes_per_launch"] = round(bytes_orbit / nviews)
+        out["roofline"]["algorithmic_bytes_per_launch"] = round(per_launch_bytes)
         # reuse: algorithmic bytes per HBM byte (records served from L2 / Infinity Cache)
-        out["roofline"]["reuse"] = round(bytes_orbit / nviews / traffic["bytes"], 3)
+        out["roofline"]["reuse"] = round(per_launch_bytes / traffic["bytes"], 3)
     if args.copy_calibration:
-        nb = args.copy_calibration << 20
-        a, b = ctx.alloc(nb), ctx.alloc(nb)
-        ctx.copy_device(b, a, nb)
-        ctx.synchronize()
-        a.free()
-        b.free()
-        out["copy_calibration_bytes"] = nb
+        out["copy_calibration_bytes"] = args.copy_calibration << 20
     if not args.no_host:
         out["host_path"] = host_path_leg(ctx, v, t, bufs, params, wimg, himg, launch_ms[0])
     if not args.no_learned:
         out["learned"] = learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, fps, samples / nviews,
-                                     args)
+                                     args, batched=bv)
     if not args.no_cpu:
         de, dx, _ = bufs[0]
         en = de.download((himg, wimg, 4), np.float32)

@@ -181,8 +181,17 @@ struct hipdvr_batch {
     std::vector<float4*> d_seg;
     std::vector<float*> d_sse;
     std::vector<int*> d_flag;
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // the judges' re-marches of the learned views in one launch pair per
+    // voxel type (index = dtype), and their judges in one launch
+    struct SparseGroup {
+        uint32_t njobs = 0, rays = 0;
+        bool wide = false;
+        hipdvr::MarchArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
+        uint32_t* d_ray_start = nullptr;
+        float4** d_segs = nullptr;
+    } sparse[2];
+    hipdvr::JudgeJob* d_judge = nullptr;
+    uint32_t n_judge = 0;
     uint32_t* d_tile_start[kTileShapes] = {nullptr, nullptr, nullptr};
     uint32_t total_tiles[kTileShapes] = {0, 0, 0};
     uint32_t* d_counter = nullptr;
@@ -204,9 +213,6 @@ void free_batch(hipdvr_batch* b) {
     if (!b) return;
     drop_graph(b);
     for (void* p : b->allocations) (void)hipFree(p);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->join) (void)hipEventDestroy(b->join);
-    if (b->side) (void)hipStreamDestroy(b->side);
     delete b;
 }
 
@@ -537,18 +543,94 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             sse[i] = dev_alloc<float>(b, 64);
             flag[i] = dev_alloc<int>(b, 64);
         }
-        if (any && !b->side) {
-            hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
-            hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
-            hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
+        // per voxel type: the learned views' sparse re-march arguments (one
+        // set per brick size), ray ranges padded to whole workgroups, and
+        // segment buffers; the judges of all learned views
+        std::vector<MarchArgs> sp_args[2][kBrickSizes];
+        std::vector<uint32_t> ray_start[2];
+        std::vector<float4*> segp[2];
+        bool gwide[2] = {false, false};
+        std::vector<JudgeJob> judges;
+        const uint32_t blk = sparse_ray_block();
+        const float mse_max = powf(10.f, -lopts.psnr_floor / 10.f);
+        for (size_t i = 0; i < nj; ++i) {
+            if (!lv[i]) continue;
+            const JudgeGrid& g = grids[i];
+            const hipdvr_batch_job& j = b->jobs[i];
+            const int d = b->h_args[0][i].m.dtype ? 1 : 0;
+            if (ray_start[d].empty()) ray_start[d].push_back(0);
+            const uint32_t rays = (uint32_t)g.w * (uint32_t)g.h;
+            ray_start[d].push_back(ray_start[d].back() + (rays + blk - 1) / blk * blk);
+            for (int bs = 0; bs < kBrickSizes; ++bs) {
+                MarchArgs sp = b->h_args[bs][i].m;
+                sp.out = sparse[i];
+                sp.counters = nullptr;
+                sp.cond_flag = nullptr;
+                sp.grid_w = g.w;
+                sp.grid_h = g.h;
+                sp.stride = (int32_t)g.stride;
+                sp.pix_block = (int32_t)g.block;
+                sp.pix_offset = (int32_t)g.offs;
+                sp_args[d][bs].push_back(sp);
+            }
+            segp[d].push_back(seg[i]);
+            gwide[d] = gwide[d] || j.volume->cells_wide;
+            JudgeJob jj;
+            jj.learned = (const float4*)j.d_out;
+            jj.sparse = sparse[i];
+            jj.sse = sse[i];
+            jj.flag = flag[i];
+            jj.w = (int32_t)j.w;
+            jj.h = (int32_t)j.h;
+            jj.sw = g.w;
+            jj.sh = g.h;
+            jj.stride = (int32_t)g.stride;
+            jj.block = (int32_t)g.block;
+            jj.offs = (int32_t)g.offs;
+            jj.denom = (float)(g.w * g.h * 4);
+            jj.mse_max = mse_max;
+            judges.push_back(jj);
         }
+        for (int d = 0; d < 2; ++d) {
+            hipdvr_batch::SparseGroup& G = b->sparse[d];
+            if (!G.d_ray_start) {   // capacity: every job of the batch
+                for (int bs = 0; bs < kBrickSizes; ++bs) G.d_args[bs] = dev_alloc<MarchArgs>(b, nj * sizeof(MarchArgs));
+                G.d_ray_start = dev_alloc<uint32_t>(b, (nj + 1) * 4);
+                G.d_segs = dev_alloc<float4*>(b, nj * sizeof(float4*));
+            }
+        }
+        if (!b->d_judge) b->d_judge = dev_alloc<JudgeJob>(b, nj * sizeof(JudgeJob));
         for (int bs = 0; bs < kBrickSizes; ++bs) {
             if (!b->d_largs[bs]) b->d_largs[bs] = dev_alloc<LearnedArgs>(b, nj * sizeof(LearnedArgs));
             hip_check(hipMemcpyAsync(b->d_largs[bs], largs[bs].data(), nj * sizeof(LearnedArgs), hipMemcpyHostToDevice,
                                      st),
                       "upload");
         }
+        for (int d = 0; d < 2; ++d) {
+            hipdvr_batch::SparseGroup& G = b->sparse[d];
+            const uint32_t n = (uint32_t)segp[d].size();
+            if (n) {
+                for (int bs = 0; bs < kBrickSizes; ++bs)
+                    hip_check(hipMemcpyAsync(G.d_args[bs], sp_args[d][bs].data(), n * sizeof(MarchArgs),
+                                             hipMemcpyHostToDevice, st),
+                              "upload");
+                hip_check(hipMemcpyAsync(G.d_ray_start, ray_start[d].data(), (n + 1) * 4, hipMemcpyHostToDevice, st),
+                          "upload");
+                hip_check(hipMemcpyAsync(G.d_segs, segp[d].data(), n * sizeof(float4*), hipMemcpyHostToDevice, st),
+                          "upload");
+            }
+        }
+        if (!judges.empty())
+            hip_check(hipMemcpyAsync(b->d_judge, judges.data(), judges.size() * sizeof(JudgeJob), hipMemcpyHostToDevice,
+                                     st),
+                      "upload");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int d = 0; d < 2; ++d) {
+            b->sparse[d].njobs = (uint32_t)segp[d].size();
+            b->sparse[d].rays = segp[d].empty() ? 0 : ray_start[d].back();
+            b->sparse[d].wide = gwide[d];
+        }
+        b->n_judge = (uint32_t)judges.size();
         b->lopts = lopts;
         b->learned.swap(lv);
         b->grids.swap(grids);
@@ -577,38 +659,25 @@ int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats) {
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(b->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(b->ctx);
-        // 1. the judges' plain re-marches on the side stream, beside 2.
-        if (any) {
-            hip_check(hipEventRecord(b->fork, st), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(b->side, b->fork, 0), "hipStreamWaitEvent");
-            for (size_t i = 0; i < nj; ++i) {
-                if (!b->learned[i]) continue;
-                const JudgeGrid& g = b->grids[i];
-                MarchArgs sp = b->h_args[bs][i].m;
-                sp.out = b->d_sparse[i];
-                sp.counters = nullptr;
-                sp.grid_w = g.w;
-                sp.grid_h = g.h;
-                sp.stride = (int32_t)g.stride;
-                sp.pix_block = (int32_t)g.block;
-                sp.pix_offset = (int32_t)g.offs;
-                launch_sparse_remarch(sp, sp.dtype, shade, b->jobs[i].volume->cells_wide, b->d_seg[i], b->side);
-            }
-            hip_check(hipEventRecord(b->join, b->side), "hipEventRecord");
+        // 1. the judges' plain re-marches of every learned view: one launch
+        //    pair per voxel type, the rays of all views in each
+        if (!b->d_largs[0]) fail(HIPDVR_ERR_INVALID, "hipdvr_batch_set_learned has not been called");
+        for (int d = 0; d < 2; ++d) {
+            const hipdvr_batch::SparseGroup& G = b->sparse[d];
+            if (G.njobs)
+                launch_sparse_remarch_multi(G.d_args[bs], G.d_ray_start, G.njobs, G.rays, d, shade, G.wide, G.d_segs,
+                                            st);
         }
         // 2. every job, learned or plain, in one persistent launch
-        if (!b->d_largs[0]) fail(HIPDVR_ERR_INVALID, "hipdvr_batch_set_learned has not been called");
         launch_batch(b, count, true);
         if (any) {
-            hip_check(hipStreamWaitEvent(st, b->join, 0), "hipStreamWaitEvent");
-            // 3. per learned view: the judge, then the plain frame when flagged
-            const float mse_max = powf(10.f, -b->lopts.psnr_floor / 10.f);
+            // 3. the judges of all learned views in one launch, then per
+            //    learned view the plain frame, launched always, which runs
+            //    only when its view was flagged
+            launch_judge_multi(b->d_judge, b->n_judge, st);
             for (size_t i = 0; i < nj; ++i) {
                 if (!b->learned[i]) continue;
-                const JudgeGrid& g = b->grids[i];
                 const hipdvr_batch_job& j = b->jobs[i];
-                launch_judge((const float4*)j.d_out, (int)j.w, (int)j.h, b->d_sparse[i], g.w, g.h, (int)g.stride,
-                             (int)g.block, (int)g.offs, b->d_sse[i], b->d_flag[i], mse_max, st);
                 MarchArgs fb = b->h_args[bs][i].m;
                 fb.counters = nullptr;
                 fb.cond_flag = b->d_flag[i];

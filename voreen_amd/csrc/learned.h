@@ -66,5 +66,16 @@ void learned_ensure_step(hipdvr_learned* l, float step);
 const hipdvr_tf* learned_prepared_tf(const hipdvr_learned* l);
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
                   int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st);
+// The judges of n views in one launch (jobs in device memory); denom is the
+// sparse grid's 4 * sw * sh, as launch_judge passes it.
+struct JudgeJob {
+    const float4* learned;
+    const float4* sparse;
+    float* sse;
+    int* flag;
+    int32_t w, h, sw, sh, stride, block, offs;
+    float denom, mse_max;
+};
+void launch_judge_multi(const JudgeJob* jobs, uint32_t n, hipStream_t st);
 
 }  // namespace hipdvr

@@ -450,11 +450,10 @@ void launch_learned(const LearnedArgs& la, bool phong, bool count, bool wide, hi
 // same sum on every run.
 constexpr int kJudgeThreads = 1024;
 
-__global__ __launch_bounds__(kJudgeThreads) void judge_kernel(const float4* __restrict__ learned, int w, int h,
-                                                              const float4* __restrict__ sparse, int sw, int sh,
-                                                              int stride, int block, int offs, float denom,
-                                                              float mse_max, float* __restrict__ sse,
-                                                              int* __restrict__ flag) {
+__device__ __forceinline__ void judge_view(const float4* __restrict__ learned, int w, int h,
+                                           const float4* __restrict__ sparse, int sw, int sh, int stride, int block,
+                                           int offs, float denom, float mse_max, float* __restrict__ sse,
+                                           int* __restrict__ flag) {
     __shared__ float part[kJudgeThreads / 64];
     float e = 0.f;
     for (int i = threadIdx.x; i < sw * sh; i += kJudgeThreads) {
@@ -472,6 +471,25 @@ __global__ __launch_bounds__(kJudgeThreads) void judge_kernel(const float4* __re
         sse[0] = t;
         flag[0] = (t / denom > mse_max) ? 1 : 0;
     }
+}
+
+__global__ __launch_bounds__(kJudgeThreads) void judge_kernel(const float4* __restrict__ learned, int w, int h,
+                                                              const float4* __restrict__ sparse, int sw, int sh,
+                                                              int stride, int block, int offs, float denom,
+                                                              float mse_max, float* __restrict__ sse,
+                                                              int* __restrict__ flag) {
+    judge_view(learned, w, h, sparse, sw, sh, stride, block, offs, denom, mse_max, sse, flag);
+}
+
+// the judges of many views, one workgroup each (the same sums as judge_kernel)
+__global__ __launch_bounds__(kJudgeThreads) void judge_multi_kernel(const JudgeJob* __restrict__ jobs) {
+    const JudgeJob& j = jobs[blockIdx.x];
+    judge_view(j.learned, j.w, j.h, j.sparse, j.sw, j.sh, j.stride, j.block, j.offs, j.denom, j.mse_max, j.sse,
+               j.flag);
+}
+
+void launch_judge_multi(const JudgeJob* jobs, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(judge_multi_kernel, dim3(n), dim3(kJudgeThreads), 0, st, jobs);
 }
 
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,

@@ -16,6 +16,7 @@
 // is never joined.  The sample positions are those of the reference loop:
 // the start of a run is t after k steps of t += step, taken in closed form
 // inside binades (closed_form_steps).
+#include "errors.h"
 #include "kernels.h"
 #include "march_cell.h"
 #include "march_common.h"
@@ -79,16 +80,11 @@ __device__ __forceinline__ void load_tables(const MarchArgs& a, uint32_t* s_tf, 
 // spread them over all CUs instead of a quarter of them.
 constexpr uint32_t kJoinBlock = 64;
 
+// Segment j of sparse ray `ray` of the re-march a: the composited run of its
+// samples [j*L, (j+1)*L), from A = 0.
 template <typename T, bool WIDE, int SHADE>
-__global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a, float4* __restrict__ seg) {
-    __shared__ uint32_t s_tf[kMaxTfWidth];
-    __shared__ float s_alpha[256];
-    load_tables(a, s_tf, s_alpha);
-    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
-    constexpr uint32_t kSeg = segments<SHADE>();
-    const uint32_t ray = gid / kSeg, j = gid % kSeg;
-    if (ray >= rays) return;
+__device__ __forceinline__ float4 sparse_segment(const MarchArgs& a, const uint32_t* s_tf, const float* s_alpha,
+                                                 uint32_t ray, uint32_t j) {
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
     const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
     const int py = judge_pixel(gy, a.stride, a.pix_block, a.pix_offset);
@@ -114,17 +110,14 @@ __global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a,
             res = make_float4(r, g, bb, A);
         }
     }
-    seg[gid] = res;
+    return res;
 }
 
+// Sparse ray `ray` of the re-march a from its segments seg[0 .. segments)
+// (written to a.out).
 template <typename T, bool WIDE, int SHADE>
-__global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs a, const float4* __restrict__ seg) {
-    __shared__ uint32_t s_tf[kMaxTfWidth];
-    __shared__ float s_alpha[256];
-    load_tables(a, s_tf, s_alpha);
-    const uint32_t ray = blockIdx.x * kJoinBlock + threadIdx.x;
-    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
-    if (ray >= rays) return;
+__device__ __forceinline__ void sparse_join(const MarchArgs& a, const uint32_t* s_tf, const float* s_alpha,
+                                            uint32_t ray, const float4* __restrict__ seg) {
     const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
     const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
     const int py = judge_pixel(gy, a.stride, a.pix_block, a.pix_offset);
@@ -144,7 +137,7 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
             int jc = -1;
             for (; j < segments<SHADE>(); ++j) {
                 if (j * L >= N) break;
-                const float4 s4 = seg[(size_t)ray * segments<SHADE>() + j];
+                const float4 s4 = seg[j];
                 const float An = A + (1.0f - A) * s4.w;
                 if (An >= kErtOpacity - 1.0e-5f) {
                     jc = j;
@@ -177,6 +170,77 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
 }
 
 template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(256) void sparse_segments_kernel(const MarchArgs a, float4* __restrict__ seg) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
+    constexpr uint32_t kSeg = segments<SHADE>();
+    const uint32_t ray = gid / kSeg, j = gid % kSeg;
+    if (ray >= rays) return;
+    seg[gid] = sparse_segment<T, WIDE, SHADE>(a, s_tf, s_alpha, ray, j);
+}
+
+template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs a, const float4* __restrict__ seg) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t ray = blockIdx.x * kJoinBlock + threadIdx.x;
+    const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
+    if (ray >= rays) return;
+    sparse_join<T, WIDE, SHADE>(a, s_tf, s_alpha, ray, seg + (size_t)ray * segments<SHADE>());
+}
+
+// The re-marches of many views in one launch each: view k's rays are
+// [ray_start[k], ray_start[k] + its grid) of the launch's ray range, every
+// ray_start a multiple of kJoinBlock, so that each workgroup (256/segments
+// rays of the segment kernel, kJoinBlock rays of the join) serves one view
+// and stages that view's tables.
+__device__ __forceinline__ uint32_t job_of_ray(const uint32_t* __restrict__ ray_start, uint32_t njobs,
+                                               uint32_t ray) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ray_start[mid] <= ray) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(256) void sparse_segments_multi_kernel(const MarchArgs* __restrict__ jobs,
+                                                                    const uint32_t* __restrict__ ray_start,
+                                                                    uint32_t njobs, float4* const* __restrict__ segs) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    constexpr uint32_t kSeg = segments<SHADE>();
+    const uint32_t job = job_of_ray(ray_start, njobs, blockIdx.x * (256u / kSeg));
+    const MarchArgs& a = jobs[job];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t ray = gid / kSeg - ray_start[job], j = gid % kSeg;
+    if (ray >= (uint32_t)a.grid_w * (uint32_t)a.grid_h) return;
+    segs[job][(size_t)ray * kSeg + j] = sparse_segment<T, WIDE, SHADE>(a, s_tf, s_alpha, ray, j);
+}
+
+template <typename T, bool WIDE, int SHADE>
+__global__ __launch_bounds__(kJoinBlock) void sparse_join_multi_kernel(const MarchArgs* __restrict__ jobs,
+                                                                       const uint32_t* __restrict__ ray_start,
+                                                                       uint32_t njobs,
+                                                                       float4* const* __restrict__ segs) {
+    __shared__ uint32_t s_tf[kMaxTfWidth];
+    __shared__ float s_alpha[256];
+    const uint32_t job = job_of_ray(ray_start, njobs, blockIdx.x * kJoinBlock);
+    const MarchArgs& a = jobs[job];
+    load_tables(a, s_tf, s_alpha);
+    const uint32_t ray = blockIdx.x * kJoinBlock + threadIdx.x - ray_start[job];
+    if (ray >= (uint32_t)a.grid_w * (uint32_t)a.grid_h) return;
+    sparse_join<T, WIDE, SHADE>(a, s_tf, s_alpha, ray, segs[job] + (size_t)ray * segments<SHADE>());
+}
+
+template <typename T, bool WIDE, int SHADE>
 void launch_sparse_t(const MarchArgs& a, float4* seg, hipStream_t st) {
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
     hipLaunchKernelGGL((sparse_segments_kernel<T, WIDE, SHADE>), dim3((rays * segments<SHADE>() + 255) / 256),
@@ -194,9 +258,42 @@ void launch_sparse_w(const MarchArgs& a, int shade, float4* seg, hipStream_t st)
     else launch_sparse_t<T, WIDE, 0>(a, seg, st);
 }
 
+template <typename T, bool WIDE, int SHADE>
+void launch_multi_t(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays,
+                    float4* const* segs, hipStream_t st) {
+    hipLaunchKernelGGL((sparse_segments_multi_kernel<T, WIDE, SHADE>), dim3(rays * segments<SHADE>() / 256),
+                       dim3(256), 0, st, jobs, ray_start, njobs, segs);
+    hipLaunchKernelGGL((sparse_join_multi_kernel<T, WIDE, SHADE>), dim3(rays / kJoinBlock), dim3(kJoinBlock), 0,
+                       st, jobs, ray_start, njobs, segs);
+}
+
+template <typename T, bool WIDE>
+void launch_multi_w(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays, int shade,
+                    float4* const* segs, hipStream_t st) {
+    if (shade == 2) launch_multi_t<T, WIDE, 2>(jobs, ray_start, njobs, rays, segs, st);
+    else if (shade == 1) launch_multi_t<T, WIDE, 1>(jobs, ray_start, njobs, rays, segs, st);
+    else launch_multi_t<T, WIDE, 0>(jobs, ray_start, njobs, rays, segs, st);
+}
+
 }  // namespace
 
 size_t sparse_segment_bytes(uint32_t rays) { return (size_t)rays * kMaxSegments * sizeof(float4); }
+
+void launch_sparse_remarch_multi(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays,
+                                 int dtype, int shade, bool wide, float4* const* segs, hipStream_t st) {
+    static_assert(kJoinBlock % (256u / kMaxSegments) == 0 && kJoinBlock % (256u / 16u) == 0,
+                  "a join block holds whole segment-kernel workgroups");
+    if (rays % kJoinBlock != 0) fail(HIPDVR_ERR_INVALID, "sparse re-march ray ranges must be whole join blocks");
+    if (dtype == 0) {
+        if (wide) launch_multi_w<uint8_t, true>(jobs, ray_start, njobs, rays, shade, segs, st);
+        else launch_multi_w<uint8_t, false>(jobs, ray_start, njobs, rays, shade, segs, st);
+    } else {
+        if (wide) launch_multi_w<uint16_t, true>(jobs, ray_start, njobs, rays, shade, segs, st);
+        else launch_multi_w<uint16_t, false>(jobs, ray_start, njobs, rays, shade, segs, st);
+    }
+}
+
+uint32_t sparse_ray_block() { return kJoinBlock; }
 
 void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st) {
     if (dtype == 0) {
