@@ -1,9 +1,12 @@
 """Learned stepping inside the batched persistent launch
 (hipdvr_batch_set_learned / hipdvr_batch_launch_learned): a learned job's
-image and judge equal its single-view learned render
-(hipdvr_render_learned_device) bit for bit, plain jobs of the same launch equal
-their plain frames, a bad network trips that job's judge so it falls back to
-the plain frame exactly, and invalid learned sets are refused."""
+image equals its single-view learned render (hipdvr_render_learned_device) bit
+for bit; its judge marches the sparse pixels whole, so the judged PSNR is the
+learned image's PSNR against the plain frame at exactly those pixels (the
+single-view judge joins ray segments: the same up to float rounding); plain
+jobs of the same launch equal their plain frames; a bad network trips that
+job's judge so it falls back to the plain frame exactly; invalid learned sets
+are refused."""
 import numpy as np
 import pytest
 
@@ -74,6 +77,23 @@ def _clear(views):
         j["do"].upload(np.full((j["h"], j["w"], 4), -1.0, np.float32))
 
 
+def _judge_psnr(img, plain, remarch_stride=8, block=8):
+    """PSNR over RGBA of img against plain at the judge's sparse pixels:
+    8x8-pixel blocks, one in every 8*remarch_stride pixels per axis, offset
+    (stride-1)*8/2 (clamped into the image), as hipdvr_learned.h places
+    them."""
+    from voreen_amd import hipdvr as H
+    h, w = img.shape[:2]
+    sw, sh = H.learned_sparse_dims(w, h, remarch_stride)
+    offs = min((remarch_stride - 1) * block // 2, w - 1, h - 1)
+    pos = lambda g: (g // block) * (remarch_stride * block) + offs + g % block
+    xs = np.array([pos(g) for g in range(sw)])
+    ys = np.array([pos(g) for g in range(sh)])
+    d = img[np.ix_(ys, xs)].astype(np.float64) - plain[np.ix_(ys, xs)]
+    mse = float(np.mean(d * d))
+    return 999.0 if mse == 0 else 10 * np.log10(1 / mse)
+
+
 def _plain(ctx, j, shading):
     from voreen_amd import hipdvr as H
     p = H.default_params(j["step"], shading=shading, camera_pos=j["eye"], kernel=H.KERNEL_CELL)
@@ -111,7 +131,14 @@ def test_batch_learned_equals_single_learned(ctx, scene, shading):
             for i, (j, r) in enumerate(zip(views, ref)):
                 assert np.array_equal(_img(j["do"], j), r), f"job {i} config {cfg}"
                 if ref_st[i] is not None:
-                    assert st[i]["sparse_psnr"] == ref_st[i]["sparse_psnr"], f"job {i}"
+                    plain = _plain(ctx, j, shading)
+                    host = _judge_psnr(r, plain)
+                    # float32 sums on the device, float64 here
+                    assert abs(st[i]["sparse_psnr"] - host) < 1e-2, (i, st[i]["sparse_psnr"], host)
+                    # the single-view judge's segment joins round (~1e-7): above
+                    # 100 dB both say the sparse pixels match
+                    q_b, q_s = st[i]["sparse_psnr"], ref_st[i]["sparse_psnr"]
+                    assert abs(q_b - q_s) < 1e-2 or min(q_b, q_s) > 100.0, (i, q_b, q_s)
                     assert st[i]["fell_back"] == ref_st[i]["fell_back"], f"job {i}"
                 else:
                     assert st[i]["fell_back"] == 0 and st[i]["sparse_psnr"] == 0.0

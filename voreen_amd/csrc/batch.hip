@@ -173,12 +173,11 @@ struct hipdvr_batch {
     // (hipdvr_batch_launch_learned; hipdvr_batch_launch stays plain)
     LearnedArgs* d_largs[kBrickSizes] = {nullptr, nullptr, nullptr};
     // learned jobs (hipdvr_batch_set_learned), per launch position: the judge's
-    // grid, sparse image, segments, sum and flag
+    // grid, sparse image, sum and flag
     std::vector<hipdvr_learned*> learned;
     hipdvr_learned_opts lopts{};
     std::vector<hipdvr::JudgeGrid> grids;
     std::vector<float4*> d_sparse;
-    std::vector<float4*> d_seg;
     std::vector<float*> d_sse;
     std::vector<int*> d_flag;
     // the judges' re-marches of the learned views in one launch pair per
@@ -188,7 +187,6 @@ struct hipdvr_batch {
         bool wide = false;
         hipdvr::MarchArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
         uint32_t* d_ray_start = nullptr;
-        float4** d_segs = nullptr;
     } sparse[2];
     hipdvr::JudgeJob* d_judge = nullptr;
     uint32_t n_judge = 0;
@@ -503,7 +501,7 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
         // built aside and taken over only when every job is accepted
         std::vector<hipdvr_learned*> lv(nj, nullptr);
         std::vector<JudgeGrid> grids(nj, JudgeGrid{});
-        std::vector<float4*> sparse(nj, nullptr), seg(nj, nullptr);
+        std::vector<float4*> sparse(nj, nullptr);
         std::vector<float*> sse(nj, nullptr);
         std::vector<int*> flag(nj, nullptr);
         bool any = false;
@@ -532,23 +530,20 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             if (i < b->d_sparse.size() && b->d_sparse[i] && b->grids[i].w == grids[i].w &&
                 b->grids[i].h == grids[i].h) {   // same grid as before: keep its buffers
                 sparse[i] = b->d_sparse[i];
-                seg[i] = b->d_seg[i];
                 sse[i] = b->d_sse[i];
                 flag[i] = b->d_flag[i];
                 continue;
             }
             const size_t px = (size_t)grids[i].w * grids[i].h;
             sparse[i] = dev_alloc<float4>(b, px * sizeof(float4));
-            seg[i] = dev_alloc<float4>(b, sparse_segment_bytes((uint32_t)px));
             sse[i] = dev_alloc<float>(b, 64);
             flag[i] = dev_alloc<int>(b, 64);
         }
         // per voxel type: the learned views' sparse re-march arguments (one
-        // set per brick size), ray ranges padded to whole workgroups, and
-        // segment buffers; the judges of all learned views
+        // set per brick size) and ray ranges padded to whole workgroups; the
+        // judges of all learned views
         std::vector<MarchArgs> sp_args[2][kBrickSizes];
         std::vector<uint32_t> ray_start[2];
-        std::vector<float4*> segp[2];
         bool gwide[2] = {false, false};
         std::vector<JudgeJob> judges;
         const uint32_t blk = sparse_ray_block();
@@ -573,7 +568,6 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
                 sp.pix_offset = (int32_t)g.offs;
                 sp_args[d][bs].push_back(sp);
             }
-            segp[d].push_back(seg[i]);
             gwide[d] = gwide[d] || j.volume->cells_wide;
             JudgeJob jj;
             jj.learned = (const float4*)j.d_out;
@@ -596,7 +590,6 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             if (!G.d_ray_start) {   // capacity: every job of the batch
                 for (int bs = 0; bs < kBrickSizes; ++bs) G.d_args[bs] = dev_alloc<MarchArgs>(b, nj * sizeof(MarchArgs));
                 G.d_ray_start = dev_alloc<uint32_t>(b, (nj + 1) * 4);
-                G.d_segs = dev_alloc<float4*>(b, nj * sizeof(float4*));
             }
         }
         if (!b->d_judge) b->d_judge = dev_alloc<JudgeJob>(b, nj * sizeof(JudgeJob));
@@ -608,15 +601,13 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
         }
         for (int d = 0; d < 2; ++d) {
             hipdvr_batch::SparseGroup& G = b->sparse[d];
-            const uint32_t n = (uint32_t)segp[d].size();
+            const uint32_t n = ray_start[d].empty() ? 0u : (uint32_t)ray_start[d].size() - 1;
             if (n) {
                 for (int bs = 0; bs < kBrickSizes; ++bs)
                     hip_check(hipMemcpyAsync(G.d_args[bs], sp_args[d][bs].data(), n * sizeof(MarchArgs),
                                              hipMemcpyHostToDevice, st),
                               "upload");
                 hip_check(hipMemcpyAsync(G.d_ray_start, ray_start[d].data(), (n + 1) * 4, hipMemcpyHostToDevice, st),
-                          "upload");
-                hip_check(hipMemcpyAsync(G.d_segs, segp[d].data(), n * sizeof(float4*), hipMemcpyHostToDevice, st),
                           "upload");
             }
         }
@@ -626,8 +617,8 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
                       "upload");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         for (int d = 0; d < 2; ++d) {
-            b->sparse[d].njobs = (uint32_t)segp[d].size();
-            b->sparse[d].rays = segp[d].empty() ? 0 : ray_start[d].back();
+            b->sparse[d].njobs = ray_start[d].empty() ? 0u : (uint32_t)ray_start[d].size() - 1;
+            b->sparse[d].rays = ray_start[d].empty() ? 0u : ray_start[d].back();
             b->sparse[d].wide = gwide[d];
         }
         b->n_judge = (uint32_t)judges.size();
@@ -635,7 +626,6 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
         b->learned.swap(lv);
         b->grids.swap(grids);
         b->d_sparse.swap(sparse);
-        b->d_seg.swap(seg);
         b->d_sse.swap(sse);
         b->d_flag.swap(flag);
     });
@@ -660,13 +650,13 @@ int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats) {
         if (count) launch_zero_u64(b->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(b->ctx);
         // 1. the judges' plain re-marches of every learned view: one launch
-        //    pair per voxel type, the rays of all views in each
+        //    per voxel type, the rays of all views in each, each ray marched
+        //    whole (the plain march exactly; enough rays without segments)
         if (!b->d_largs[0]) fail(HIPDVR_ERR_INVALID, "hipdvr_batch_set_learned has not been called");
         for (int d = 0; d < 2; ++d) {
             const hipdvr_batch::SparseGroup& G = b->sparse[d];
             if (G.njobs)
-                launch_sparse_remarch_multi(G.d_args[bs], G.d_ray_start, G.njobs, G.rays, d, shade, G.wide, G.d_segs,
-                                            st);
+                launch_sparse_remarch_multi(G.d_args[bs], G.d_ray_start, G.njobs, G.rays, d, shade, G.wide, st);
         }
         // 2. every job, learned or plain, in one persistent launch
         launch_batch(b, count, true);
