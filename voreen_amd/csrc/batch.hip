@@ -39,8 +39,49 @@ constexpr int kBrickSizes = 3;   // brick_log2 3, 4, 5
 __host__ __device__ constexpr int wave_w(int tile) { return tile == 0 ? 8 : (tile == 1 ? 16 : 32); }
 __host__ __device__ constexpr int wave_h(int tile) { return 64 / wave_w(tile); }
 
+// Tile queues.  The hardware hands the persistent grid's workgroups to the
+// 8 XCDs in turn (workgroup b to XCD b % 8), and each XCD has its own L2.
+// With kXcdQueues the tiles come in blocks of kBlockW x kBlockH tiles of one
+// job (blocks row by row, tiles row by row inside a block; edge blocks
+// smaller); consecutive runs of kChunk tiles of the launch's tile range are
+// dealt to the XCDs in turn, and each XCD takes its runs from a counter of
+// its own, so the tiles an XCD marches at a time form a compact column of
+// blocks whose rays share that XCD's L2.  An XCD whose runs are taken takes
+// from the others' counters.  Without it: one counter, tiles row by row.
+constexpr bool kXcdQueues = true;
+constexpr int kBlockW = 8, kBlockH = 4;
+constexpr uint32_t kChunk = kBlockW * kBlockH;
+constexpr uint32_t kCounterStride = 32;   // counters 128 bytes apart
+
 __global__ void batch_reset_kernel(uint32_t* counter) {
-    if (threadIdx.x == 0) counter[0] = 0;
+    if (threadIdx.x < kXcds) counter[threadIdx.x * kCounterStride] = 0;
+}
+
+// Tiles dealt to XCD x out of total.
+__device__ __forceinline__ uint32_t xcd_tiles(uint32_t x, uint32_t total) {
+    const uint32_t chunks = (total + kChunk - 1) / kChunk;
+    if (x >= chunks) return 0;
+    const uint32_t owned = (chunks - 1 - x) / kXcds + 1;
+    uint32_t n = owned * kChunk;
+    if ((chunks - 1) % kXcds == x) n -= chunks * kChunk - total;   // its last run is the short one
+    return n;
+}
+
+// Tile (tx, ty) at position lt of a job of tiles_x x tiles_y tiles in block
+// order (a bijection of [0, tiles_x * tiles_y)).
+__device__ __forceinline__ void block_tile(uint32_t lt, uint32_t tiles_x, uint32_t tiles_y, uint32_t& tx,
+                                           uint32_t& ty) {
+    const uint32_t nby = (tiles_y + kBlockH - 1) / kBlockH, nbx = (tiles_x + kBlockW - 1) / kBlockW;
+    uint32_t by = lt / (kBlockH * tiles_x);
+    if (by > nby - 1) by = nby - 1;
+    const uint32_t r = lt - by * kBlockH * tiles_x;
+    const uint32_t bh = min((uint32_t)kBlockH, tiles_y - by * kBlockH);
+    uint32_t bx = r / (kBlockW * bh);
+    if (bx > nbx - 1) bx = nbx - 1;
+    const uint32_t r2 = r - bx * kBlockW * bh;
+    const uint32_t bw = min((uint32_t)kBlockW, tiles_x - bx * kBlockW);
+    ty = by * kBlockH + r2 / bw;
+    tx = bx * kBlockW + r2 % bw;
 }
 
 template <bool WIDE, int SHADE, bool COUNT, int K, int TILE>
@@ -55,8 +96,24 @@ __global__ __launch_bounds__(256) void batch_kernel(const LearnedArgs* __restric
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int cur_job = -1;
     RayCounters n;
+    uint32_t drained = 0;   // (lane 0) XCD queues seen empty
+    const uint32_t my_xcd = blockIdx.x % kXcds;
     while (true) {
-        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        if (tid == 0) {
+            if (kXcdQueues) {
+                uint32_t t = total_tiles;
+                for (uint32_t k = 0; k < kXcds && t == total_tiles; ++k) {
+                    const uint32_t x = (my_xcd + k) % kXcds;
+                    if (drained & (1u << x)) continue;
+                    const uint32_t i = atomicAdd(counter + x * kCounterStride, 1u);
+                    if (i < xcd_tiles(x, total_tiles)) t = (x + kXcds * (i / kChunk)) * kChunk + i % kChunk;
+                    else drained |= 1u << x;
+                }
+                s_tile = t;
+            } else {
+                s_tile = atomicAdd(counter, 1u);
+            }
+        }
         __syncthreads();   // also: every wave has finished the previous tile
         const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();   // s_tile is read before the next iteration writes it
@@ -81,8 +138,10 @@ __global__ __launch_bounds__(256) void batch_kernel(const LearnedArgs* __restric
         }
         const uint32_t lt = tile - tile_start[job];
         const uint32_t tiles_x = (uint32_t)(a.w + 2 * WW - 1) / (2 * WW);
-        const int px = (int)(lt % tiles_x) * (2 * WW) + (wave & 1) * WW + (lane % WW);
-        const int py = (int)(lt / tiles_x) * (2 * WH) + (wave >> 1) * WH + (lane / WW);
+        uint32_t tx = lt % tiles_x, ty = lt / tiles_x;
+        if (kXcdQueues) block_tile(lt, tiles_x, (uint32_t)(a.h + 2 * WH - 1) / (2 * WH), tx, ty);
+        const int px = (int)tx * (2 * WW) + (wave & 1) * WW + (lane % WW);
+        const int py = (int)ty * (2 * WH) + (wave >> 1) * WH + (lane / WW);
         if (px < a.w && py < a.h) {
             const size_t pix = (size_t)py * a.w + px;
             const float4 e = a.entry[pix];
@@ -359,7 +418,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
                           "upload");
                 hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
             }
-            b->d_counter = dev_alloc<uint32_t>(b, 64);
+            b->d_counter = dev_alloc<uint32_t>(b, kXcds * kCounterStride * 4);
             b->learned.assign(n, nullptr);
             hip_check(hipGetLastError(), "batch setup");
             hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
