@@ -225,6 +225,10 @@ int hipdvr_create(hipdvr_ctx** out, int device);
 int hipdvr_destroy(hipdvr_ctx* ctx);
 const char* hipdvr_last_error(const hipdvr_ctx* ctx);
 int hipdvr_abi_version(void);
+/* SHA-256 (hex) of the sources and public headers the library was built from
+ * (no counterpart in the reference): lets a caller that loads a prebuilt
+ * library check that it matches the sources shipped with it. */
+const char* hipdvr_source_digest(void);
 /* use a caller-owned HIP stream (hipStream_t as void*; NULL = context stream) */
 int hipdvr_set_stream(hipdvr_ctx* ctx, void* stream);
 int hipdvr_synchronize(hipdvr_ctx* ctx);

@@ -36,6 +36,14 @@ def test_library_exports_every_declared_symbol():
     assert lib.hipdvr_abi_version() == 1
 
 
+def test_library_was_built_from_these_sources():
+    """The library's embedded digest equals the digest of the sources and
+    headers in this tree: a prebuilt libhipdvr.so that travels with the
+    sources (the GPU runs load it without building) is the one they make."""
+    from voreen_amd import build, hipdvr as H
+    assert H.source_digest() == build.source_digest(), "libhipdvr.so is stale: run python -m voreen_amd.build"
+
+
 def test_create_on_a_missing_device_returns_an_error():
     from voreen_amd import hipdvr as H
     lib = H.load()

@@ -94,6 +94,13 @@ def c0_inputs(oracle, size=256, view=0, nviews=1):
     return vol, tf, en, ex, W.step_for((128, 128, 128), 2.0)
 
 
+def test_loaded_library_was_built_from_these_sources(ctx):
+    """On the GPU box the prebuilt libhipdvr.so is loaded as it travelled:
+    its embedded digest must equal the digest of the sources beside it."""
+    from voreen_amd import build, hipdvr as H
+    assert H.source_digest() == build.source_digest(), "libhipdvr.so is stale: run python -m voreen_amd.build"
+
+
 def test_entry_exit_device_matches_oracle(ctx, oracle):
     from voreen_amd import hipdvr as H, workloads as W
     cases = [(W.orbit_camera(3, 36), 256, 256, False), (W.orbit_camera(7, 36), 97, 61, True)]

@@ -50,7 +50,7 @@ EXPORTS = (
     "hipdvr_volume_minmax", "hipdvr_tf_preint_device",
     "hipdvr_batch_create", "hipdvr_batch_destroy", "hipdvr_batch_set_config", "hipdvr_batch_get_config",
     "hipdvr_batch_autotune", "hipdvr_batch_capture", "hipdvr_batch_launch",
-    "hipdvr_batch_set_learned", "hipdvr_batch_launch_learned",
+    "hipdvr_batch_set_learned", "hipdvr_batch_launch_learned", "hipdvr_source_digest",
 )
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_TARGETS = 16, 2, 3
@@ -155,6 +155,7 @@ def load():
         "hipdvr_destroy": ([vp], C.c_int),
         "hipdvr_last_error": ([vp], C.c_char_p),
         "hipdvr_abi_version": ([], C.c_int),
+        "hipdvr_source_digest": ([], C.c_char_p),
         "hipdvr_set_stream": ([vp, vp], C.c_int),
         "hipdvr_synchronize": ([vp], C.c_int),
         "hipdvr_volume_create": ([vp, P(VolumeDesc), vp, P(vp)], C.c_int),
@@ -725,6 +726,12 @@ class Learned:
             self.h, tf.h, d_entry.ptr, d_exit.ptr, w, h, C.byref(params), C.byref(opts) if opts else None,
             d_sparse.ptr, d_sparse.nbytes // 16, C.byref(sw), C.byref(sh)), "hipdvr_learned_sparse_remarch")
         return sw.value, sh.value
+
+
+def source_digest():
+    """Digest of the sources libhipdvr.so was built from
+    (hipdvr_source_digest); voreen_amd.build.source_digest() gives the tree's."""
+    return load().hipdvr_source_digest().decode()
 
 
 def learned_sparse_dims(w, h, remarch_stride=0):
