@@ -75,15 +75,15 @@ int hipdvr_batch_launch(hipdvr_batch* b, hipdvr_stats* stats);
  * must outlive their use by the batch.  Calling again replaces the set. */
 int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, uint32_t n,
                              const hipdvr_learned_opts* opts);
-/* One batched launch with learned stepping: first the judges' plain
- * re-march of every learned view's sparse pixels (one launch per voxel type,
- * each ray marched whole, so the sparse images equal the plain frames at
- * those pixels exactly), then the learned and plain jobs in one persistent
- * launch, then the judges of all learned views in one launch, and per
- * learned view the plain frame, which runs only when its view fell below the
- * PSNR floor (the steps of hipdvr_render_learned_device, for all views of the
- * batch at once; that entry point re-marches one view's judge rays in
- * segments, equal up to float rounding).  Asynchronous unless stats != NULL:
+/* One batched launch with learned stepping: one persistent launch marches
+ * the judges' plain re-march of every learned view's sparse pixels (each ray
+ * whole, so the sparse images equal the plain frames at those pixels
+ * exactly) and the learned and plain jobs; then the judges of all learned
+ * views run in one launch, and per learned view the plain frame, which runs
+ * only when its view fell below the PSNR floor (the steps of
+ * hipdvr_render_learned_device, for all views of the batch at once; that
+ * entry point re-marches one view's judge rays in segments, equal up to
+ * float rounding).  Asynchronous unless stats != NULL:
  * n_jobs entries in the caller's order, sparse_psnr and fell_back per learned
  * job, samples and shaded the totals of the whole launch. */
 int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats);

@@ -136,13 +136,22 @@ __global__ __launch_bounds__(256) void batch_kernel(const LearnedArgs* __restric
             __syncthreads();
             cur_job = job;
         }
+        // a job marches the pixels of its grid: the whole image, or (a
+        // judge's re-march: pix_block > 1) the sparse pixels judge_pixel
+        // places, written to out[gy * grid_w + gx]; their samples are not
+        // counted (the learned frame's statistics leave the judge out)
+        const bool sparse = a.pix_block > 1;
+        RayCounters uncounted;
+        RayCounters& nc = sparse ? uncounted : n;
         const uint32_t lt = tile - tile_start[job];
-        const uint32_t tiles_x = (uint32_t)(a.w + 2 * WW - 1) / (2 * WW);
+        const uint32_t tiles_x = (uint32_t)(a.grid_w + 2 * WW - 1) / (2 * WW);
         uint32_t tx = lt % tiles_x, ty = lt / tiles_x;
-        if (kXcdQueues) block_tile(lt, tiles_x, (uint32_t)(a.h + 2 * WH - 1) / (2 * WH), tx, ty);
-        const int px = (int)tx * (2 * WW) + (wave & 1) * WW + (lane % WW);
-        const int py = (int)ty * (2 * WH) + (wave >> 1) * WH + (lane / WW);
-        if (px < a.w && py < a.h) {
+        if (kXcdQueues) block_tile(lt, tiles_x, (uint32_t)(a.grid_h + 2 * WH - 1) / (2 * WH), tx, ty);
+        const int gx = (int)tx * (2 * WW) + (wave & 1) * WW + (lane % WW);
+        const int gy = (int)ty * (2 * WH) + (wave >> 1) * WH + (lane / WW);
+        if (gx < a.grid_w && gy < a.grid_h) {
+            const int px = sparse ? judge_pixel(gx, a.stride, a.pix_block, a.pix_offset) : gx;
+            const int py = sparse ? judge_pixel(gy, a.stride, a.pix_block, a.pix_offset) : gy;
             const size_t pix = (size_t)py * a.w + px;
             const float4 e = a.entry[pix];
             const float4 l = a.exit[pix];
@@ -165,13 +174,13 @@ __global__ __launch_bounds__(256) void batch_kernel(const LearnedArgs* __restric
                     n.shaded += nsh;
                 } else if (a.dtype == 0) {
                     const CellFetch<uint8_t, WIDE> f = cell_fetch_of<uint8_t, WIDE>(a);
-                    res = cell_ray<uint8_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
+                    res = cell_ray<uint8_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, nc);
                 } else {
                     const CellFetch<uint16_t, WIDE> f = cell_fetch_of<uint16_t, WIDE>(a);
-                    res = cell_ray<uint16_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
+                    res = cell_ray<uint16_t, WIDE, SHADE, COUNT, K>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, nc);
                 }
             }
-            a.out[pix] = res;
+            a.out[(size_t)gy * a.grid_w + gx] = res;
         }
     }
     if (COUNT) {
@@ -239,14 +248,12 @@ struct hipdvr_batch {
     std::vector<float4*> d_sparse;
     std::vector<float*> d_sse;
     std::vector<int*> d_flag;
-    // the judges' re-marches of the learned views in one launch pair per
-    // voxel type (index = dtype), and their judges in one launch
-    struct SparseGroup {
-        uint32_t njobs = 0, rays = 0;
-        bool wide = false;
-        hipdvr::MarchArgs* d_args[kBrickSizes] = {nullptr, nullptr, nullptr};
-        uint32_t* d_ray_start = nullptr;
-    } sparse[2];
+    // the learned launch's job list: first each learned view's judge
+    // re-march (its sparse grid, marched plainly into its sparse image),
+    // then every job; its tile ranges per tile shape
+    uint32_t n_ljobs = 0;
+    uint32_t* d_ltile_start[kTileShapes] = {nullptr, nullptr, nullptr};
+    uint32_t ltotal_tiles[kTileShapes] = {0, 0, 0};
     hipdvr::JudgeJob* d_judge = nullptr;
     uint32_t n_judge = 0;
     uint32_t* d_tile_start[kTileShapes] = {nullptr, nullptr, nullptr};
@@ -298,13 +305,14 @@ void launch_batch(hipdvr_batch* b, bool count, bool learned = false) {
     // other's cell records (64 uneven volumes, unshaded: 59,030 frames/s at
     // the occupancy limit, 63,436 at five, 62,716 at four)
     if (per_cu > kMaxResidentPerCu) per_cu = kMaxResidentPerCu;
+    const uint32_t total = learned ? b->ltotal_tiles[t] : b->total_tiles[t];
     uint32_t grid = (uint32_t)(b->sms * per_cu);
-    if (grid > b->total_tiles[t]) grid = b->total_tiles[t] ? b->total_tiles[t] : 1;
+    if (grid > total) grid = total ? total : 1;
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->d_counter);
     hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st,
                        (const LearnedArgs*)(learned ? b->d_largs[bs] : b->d_args[bs]),
-                       (const uint32_t*)b->d_tile_start[t], (uint32_t)b->jobs.size(), b->total_tiles[t],
-                       b->d_counter, b->ctx->d_counters);
+                       (const uint32_t*)(learned ? b->d_ltile_start[t] : b->d_tile_start[t]),
+                       learned ? b->n_ljobs : (uint32_t)b->jobs.size(), total, b->d_counter, b->ctx->d_counters);
     hip_check(hipGetLastError(), "batch launch");
 }
 
@@ -598,36 +606,29 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             sse[i] = dev_alloc<float>(b, 64);
             flag[i] = dev_alloc<int>(b, 64);
         }
-        // per voxel type: the learned views' sparse re-march arguments (one
-        // set per brick size) and ray ranges padded to whole workgroups; the
-        // judges of all learned views
-        std::vector<MarchArgs> sp_args[2][kBrickSizes];
-        std::vector<uint32_t> ray_start[2];
-        bool gwide[2] = {false, false};
+        // the learned launch's jobs: each learned view's judge re-march
+        // (plain, into its sparse image), then every job; and the judges
+        std::vector<LearnedArgs> ljobs[kBrickSizes];
         std::vector<JudgeJob> judges;
-        const uint32_t blk = sparse_ray_block();
         const float mse_max = powf(10.f, -lopts.psnr_floor / 10.f);
         for (size_t i = 0; i < nj; ++i) {
             if (!lv[i]) continue;
             const JudgeGrid& g = grids[i];
             const hipdvr_batch_job& j = b->jobs[i];
-            const int d = b->h_args[0][i].m.dtype ? 1 : 0;
-            if (ray_start[d].empty()) ray_start[d].push_back(0);
-            const uint32_t rays = (uint32_t)g.w * (uint32_t)g.h;
-            ray_start[d].push_back(ray_start[d].back() + (rays + blk - 1) / blk * blk);
             for (int bs = 0; bs < kBrickSizes; ++bs) {
-                MarchArgs sp = b->h_args[bs][i].m;
-                sp.out = sparse[i];
-                sp.counters = nullptr;
-                sp.cond_flag = nullptr;
-                sp.grid_w = g.w;
-                sp.grid_h = g.h;
-                sp.stride = (int32_t)g.stride;
-                sp.pix_block = (int32_t)g.block;
-                sp.pix_offset = (int32_t)g.offs;
-                sp_args[d][bs].push_back(sp);
+                LearnedArgs sj;
+                std::memset(&sj, 0, sizeof(sj));
+                sj.m = b->h_args[bs][i].m;
+                sj.m.out = sparse[i];
+                sj.m.counters = nullptr;
+                sj.m.cond_flag = nullptr;
+                sj.m.grid_w = g.w;
+                sj.m.grid_h = g.h;
+                sj.m.stride = (int32_t)g.stride;
+                sj.m.pix_block = (int32_t)g.block;
+                sj.m.pix_offset = (int32_t)g.offs;
+                ljobs[bs].push_back(sj);
             }
-            gwide[d] = gwide[d] || j.volume->cells_wide;
             JudgeJob jj;
             jj.learned = (const float4*)j.d_out;
             jj.sparse = sparse[i];
@@ -644,42 +645,39 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             jj.mse_max = mse_max;
             judges.push_back(jj);
         }
-        for (int d = 0; d < 2; ++d) {
-            hipdvr_batch::SparseGroup& G = b->sparse[d];
-            if (!G.d_ray_start) {   // capacity: every job of the batch
-                for (int bs = 0; bs < kBrickSizes; ++bs) G.d_args[bs] = dev_alloc<MarchArgs>(b, nj * sizeof(MarchArgs));
-                G.d_ray_start = dev_alloc<uint32_t>(b, (nj + 1) * 4);
+        for (int bs = 0; bs < kBrickSizes; ++bs) ljobs[bs].insert(ljobs[bs].end(), largs[bs].begin(), largs[bs].end());
+        const uint32_t nl = (uint32_t)ljobs[0].size();
+        std::vector<uint32_t> lstart[kTileShapes];
+        for (int t = 0; t < kTileShapes; ++t) {
+            const uint32_t tw = 2 * wave_w(t), th = 2 * wave_h(t);
+            lstart[t].assign(nl + 1, 0);
+            for (uint32_t k = 0; k < nl; ++k) {
+                const MarchArgs& m = ljobs[0][k].m;
+                const uint64_t tiles = (uint64_t)((m.grid_w + tw - 1) / tw) * ((m.grid_h + th - 1) / th);
+                if (lstart[t][k] + tiles > 0xfffffff0ull) fail(HIPDVR_ERR_UNSUPPORTED, "batch too large");
+                lstart[t][k + 1] = lstart[t][k] + (uint32_t)tiles;
             }
         }
+        // capacity: every job of the batch twice
+        for (int bs = 0; bs < kBrickSizes; ++bs)
+            if (!b->d_largs[bs]) b->d_largs[bs] = dev_alloc<LearnedArgs>(b, 2 * nj * sizeof(LearnedArgs));
+        for (int t = 0; t < kTileShapes; ++t)
+            if (!b->d_ltile_start[t]) b->d_ltile_start[t] = dev_alloc<uint32_t>(b, (2 * nj + 1) * 4);
         if (!b->d_judge) b->d_judge = dev_alloc<JudgeJob>(b, nj * sizeof(JudgeJob));
-        for (int bs = 0; bs < kBrickSizes; ++bs) {
-            if (!b->d_largs[bs]) b->d_largs[bs] = dev_alloc<LearnedArgs>(b, nj * sizeof(LearnedArgs));
-            hip_check(hipMemcpyAsync(b->d_largs[bs], largs[bs].data(), nj * sizeof(LearnedArgs), hipMemcpyHostToDevice,
+        for (int bs = 0; bs < kBrickSizes; ++bs)
+            hip_check(hipMemcpyAsync(b->d_largs[bs], ljobs[bs].data(), nl * sizeof(LearnedArgs), hipMemcpyHostToDevice,
                                      st),
                       "upload");
-        }
-        for (int d = 0; d < 2; ++d) {
-            hipdvr_batch::SparseGroup& G = b->sparse[d];
-            const uint32_t n = ray_start[d].empty() ? 0u : (uint32_t)ray_start[d].size() - 1;
-            if (n) {
-                for (int bs = 0; bs < kBrickSizes; ++bs)
-                    hip_check(hipMemcpyAsync(G.d_args[bs], sp_args[d][bs].data(), n * sizeof(MarchArgs),
-                                             hipMemcpyHostToDevice, st),
-                              "upload");
-                hip_check(hipMemcpyAsync(G.d_ray_start, ray_start[d].data(), (n + 1) * 4, hipMemcpyHostToDevice, st),
-                          "upload");
-            }
-        }
+        for (int t = 0; t < kTileShapes; ++t)
+            hip_check(hipMemcpyAsync(b->d_ltile_start[t], lstart[t].data(), (nl + 1) * 4, hipMemcpyHostToDevice, st),
+                      "upload");
         if (!judges.empty())
             hip_check(hipMemcpyAsync(b->d_judge, judges.data(), judges.size() * sizeof(JudgeJob), hipMemcpyHostToDevice,
                                      st),
                       "upload");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        for (int d = 0; d < 2; ++d) {
-            b->sparse[d].njobs = ray_start[d].empty() ? 0u : (uint32_t)ray_start[d].size() - 1;
-            b->sparse[d].rays = ray_start[d].empty() ? 0u : ray_start[d].back();
-            b->sparse[d].wide = gwide[d];
-        }
+        b->n_ljobs = nl;
+        for (int t = 0; t < kTileShapes; ++t) b->ltotal_tiles[t] = lstart[t][nl];
         b->n_judge = (uint32_t)judges.size();
         b->lopts = lopts;
         b->learned.swap(lv);
@@ -708,19 +706,12 @@ int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats) {
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(b->ctx->d_counters, kNumCounters, st);
         const bool timed = ctx_timing_start(b->ctx);
-        // 1. the judges' plain re-marches of every learned view: one launch
-        //    per voxel type, the rays of all views in each, each ray marched
-        //    whole (the plain march exactly; enough rays without segments)
         if (!b->d_largs[0]) fail(HIPDVR_ERR_INVALID, "hipdvr_batch_set_learned has not been called");
-        for (int d = 0; d < 2; ++d) {
-            const hipdvr_batch::SparseGroup& G = b->sparse[d];
-            if (G.njobs)
-                launch_sparse_remarch_multi(G.d_args[bs], G.d_ray_start, G.njobs, G.rays, d, shade, G.wide, st);
-        }
-        // 2. every job, learned or plain, in one persistent launch
+        // 1. the judges' plain re-marches of the learned views' sparse pixels
+        //    and every job, learned or plain, in one persistent launch
         launch_batch(b, count, true);
         if (any) {
-            // 3. the judges of all learned views in one launch, then per
+            // 2. the judges of all learned views in one launch, then per
             //    learned view the plain frame, launched always, which runs
             //    only when its view was flagged
             launch_judge_multi(b->d_judge, b->n_judge, st);

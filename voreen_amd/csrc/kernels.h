@@ -58,16 +58,6 @@ void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // a.pix_offset) in ray segments (sparse.hip); seg: sparse_segment_bytes().
 size_t sparse_segment_bytes(uint32_t rays);
 void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st);
-// The re-marches of many views (the learned views of a batch, all of voxel
-// type dtype) in one launch, each ray marched whole by the plain march (the
-// sparse image equals the plain frame at its pixels exactly; many views give
-// the GPU enough rays without cutting them into segments): view k's sparse
-// rays start at ray_start[k], a multiple of sparse_ray_block(); rays is the
-// padded total; jobs[k] (its out = view k's sparse image) and ray_start live
-// in device memory.
-void launch_sparse_remarch_multi(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays,
-                                 int dtype, int shade, bool wide, hipStream_t st);
-uint32_t sparse_ray_block();
 // Streaming copy of n 16-byte vectors (bandwidth calibration).
 void launch_copy16(void* dst, const void* src, size_t n, hipStream_t st);
 void launch_entry_exit(const RayGen& g, float4* entry, float4* exit, hipStream_t st);

@@ -193,52 +193,6 @@ __global__ __launch_bounds__(kJoinBlock) void sparse_join_kernel(const MarchArgs
     sparse_join<T, WIDE, SHADE>(a, s_tf, s_alpha, ray, seg + (size_t)ray * segments<SHADE>());
 }
 
-// The re-march of many views in one launch: view k's rays are
-// [ray_start[k], ray_start[k] + its grid) of the launch's ray range, every
-// ray_start a multiple of the workgroup size, so that each workgroup serves
-// one view and stages that view's tables.
-__device__ __forceinline__ uint32_t job_of_ray(const uint32_t* __restrict__ ray_start, uint32_t njobs,
-                                               uint32_t ray) {
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ray_start[mid] <= ray) lo = mid;
-        else hi = mid;
-    }
-    return __builtin_amdgcn_readfirstlane(lo);
-}
-
-// The re-march of many views without segments: one lane per sparse ray,
-// each ray marched whole by the plain march (cell_ray), so the sparse image
-// equals the plain frame at those pixels exactly.  Many views give the GPU
-// enough rays without cutting them (36 views of 1024^2: 590K rays).
-constexpr uint32_t kWholeBlock = 256;
-
-template <typename T, bool WIDE, int SHADE>
-__global__ __launch_bounds__(kWholeBlock) void sparse_whole_multi_kernel(const MarchArgs* __restrict__ jobs,
-                                                                         const uint32_t* __restrict__ ray_start,
-                                                                         uint32_t njobs) {
-    __shared__ uint32_t s_tf[kMaxTfWidth];
-    __shared__ float s_alpha[256];
-    const uint32_t job = job_of_ray(ray_start, njobs, blockIdx.x * kWholeBlock);
-    const MarchArgs& a = jobs[job];
-    load_tables(a, s_tf, s_alpha);
-    const uint32_t ray = blockIdx.x * kWholeBlock + threadIdx.x - ray_start[job];
-    if (ray >= (uint32_t)a.grid_w * (uint32_t)a.grid_h) return;
-    const int gx = (int)(ray % (uint32_t)a.grid_w), gy = (int)(ray / (uint32_t)a.grid_w);
-    const int px = judge_pixel(gx, a.stride, a.pix_block, a.pix_offset);
-    const int py = judge_pixel(gy, a.stride, a.pix_block, a.pix_offset);
-    const size_t pix = (size_t)py * a.w + px;
-    const float4 e = a.entry[pix], l = a.exit[pix];
-    float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!is_background(e, l)) {
-        const CellFetch<T, WIDE> f = cell_fetch_of<T, WIDE>(a);
-        RayCounters n;
-        res = cell_ray<T, WIDE, SHADE, false, sparse_k<SHADE>()>(a, f, s_tf, s_alpha, nullptr, px, py, e, l, n);
-    }
-    a.out[(size_t)gy * a.grid_w + gx] = res;
-}
-
 template <typename T, bool WIDE, int SHADE>
 void launch_sparse_t(const MarchArgs& a, float4* seg, hipStream_t st) {
     const uint32_t rays = (uint32_t)a.grid_w * (uint32_t)a.grid_h;
@@ -257,32 +211,9 @@ void launch_sparse_w(const MarchArgs& a, int shade, float4* seg, hipStream_t st)
     else launch_sparse_t<T, WIDE, 0>(a, seg, st);
 }
 
-template <typename T, bool WIDE>
-void launch_multi_w(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays, int shade,
-                    hipStream_t st) {
-    const dim3 grid(rays / kWholeBlock), block(kWholeBlock);
-    if (shade == 2) hipLaunchKernelGGL((sparse_whole_multi_kernel<T, WIDE, 2>), grid, block, 0, st, jobs, ray_start, njobs);
-    else if (shade == 1) hipLaunchKernelGGL((sparse_whole_multi_kernel<T, WIDE, 1>), grid, block, 0, st, jobs, ray_start, njobs);
-    else hipLaunchKernelGGL((sparse_whole_multi_kernel<T, WIDE, 0>), grid, block, 0, st, jobs, ray_start, njobs);
-}
-
 }  // namespace
 
 size_t sparse_segment_bytes(uint32_t rays) { return (size_t)rays * kMaxSegments * sizeof(float4); }
-
-void launch_sparse_remarch_multi(const MarchArgs* jobs, const uint32_t* ray_start, uint32_t njobs, uint32_t rays,
-                                 int dtype, int shade, bool wide, hipStream_t st) {
-    if (rays % kWholeBlock != 0) fail(HIPDVR_ERR_INVALID, "sparse re-march ray ranges must be whole ray blocks");
-    if (dtype == 0) {
-        if (wide) launch_multi_w<uint8_t, true>(jobs, ray_start, njobs, rays, shade, st);
-        else launch_multi_w<uint8_t, false>(jobs, ray_start, njobs, rays, shade, st);
-    } else {
-        if (wide) launch_multi_w<uint16_t, true>(jobs, ray_start, njobs, rays, shade, st);
-        else launch_multi_w<uint16_t, false>(jobs, ray_start, njobs, rays, shade, st);
-    }
-}
-
-uint32_t sparse_ray_block() { return kWholeBlock; }
 
 void launch_sparse_remarch(const MarchArgs& a, int dtype, int shade, bool wide, float4* seg, hipStream_t st) {
     if (dtype == 0) {
