@@ -70,7 +70,9 @@ int hipdvr_batch_launch(hipdvr_batch* b, hipdvr_stats* stats);
  * i's volume whose last hipdvr_learned_prepare was for job i's TF.  Used by
  * hipdvr_batch_launch_learned only (hipdvr_batch_launch and a captured graph
  * stay plain).  Jobs sharing a learned state share its step; jobs shading
- * with the exact Phong path cannot be learned.  A later prepare of a state
+ * with the exact Phong path cannot be learned.  A job whose state's table
+ * cannot gain (every brick that may hold alpha > 0 at the base step) is
+ * marched plainly (stats: sparse_psnr 999).  A later prepare of a state
  * (a new network, the same TF) takes effect at the next launch; the states
  * must outlive their use by the batch.  Calling again replaces the set. */
 int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, uint32_t n,

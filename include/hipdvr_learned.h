@@ -146,8 +146,11 @@ int hipdvr_learned_train(hipdvr_learned* l, hipdvr_mlp* mlp, const hipdvr_learne
                          hipdvr_learned_train_info* info);
 
 /* Render with learned stepping; the sparse plain re-march judges it and the
- * frame falls back to the plain marcher below the PSNR floor.  Asynchronous
- * unless stats != NULL. */
+ * frame falls back to the plain marcher below the PSNR floor.  When the last
+ * prepared table keeps every brick that may hold alpha > 0 at the base step
+ * (no gain possible), the plain frame is rendered directly, without the
+ * judge (stats: sparse_psnr 999, fell_back 0).  Asynchronous unless
+ * stats != NULL. */
 int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const float* d_entry,
                                  const float* d_exit, uint32_t w, uint32_t h, const hipdvr_params* prm,
                                  const hipdvr_learned_opts* opts, float* d_out,

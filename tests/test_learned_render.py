@@ -1,8 +1,9 @@
-"""Learned stepping end to end on the GPU: the no-network baseline matches
-the plain marcher, a network trained on plain traces (the library's own
-kernels) keeps >= 45 dB while reconstructing fewer samples than the baseline,
-and bad weights trip the sparse judge so the frame falls back to the plain
-image exactly."""
+"""Learned stepping end to end on the GPU: the no-network table (every brick
+at the base step) renders the plain frame itself, a network trained on plain
+traces (the library's own kernels) keeps >= 45 dB while reconstructing fewer
+samples than the plain march, a constant double-step network marches a wide
+frame completely, and bad weights trip the sparse judge so the frame falls
+back to the plain image exactly."""
 import numpy as np
 import pytest
 
@@ -59,38 +60,50 @@ def test_learned_baseline_matches_plain(ctx, scene):
     q = psnr(img, plain)
     print(f"baseline: psnr {q:.1f} dB sparse {ls['sparse_psnr']:.1f} samples {ls['samples']} "
           f"(plain fetched {st['fetched']}) fell_back {ls['fell_back']}", flush=True)
-    assert q > 60.0 and not ls["fell_back"]
+    # a table that keeps every occupied brick at the base step cannot gain:
+    # the plain frame itself, without the judge
+    assert np.array_equal(img, plain) and not ls["fell_back"] and ls["sparse_psnr"] == 999.0
+    assert ls["samples"] == st["fetched"]
 
 
-def test_learned_baseline_wide_frame(ctx, scene):
+def test_learned_march_wide_frame(ctx, scene):
     """A 560x136 frame spans more than one chip region of tiles, so the learned
-    march takes its tiles in per-XCD regions (partial ones at the edges): every
-    pixel is written and the baseline stays within 60 dB of the plain frame."""
+    march takes its tiles in per-XCD regions (partial ones at the edges).  A
+    network with constant outputs (no skip, step multiple 2 in every brick)
+    makes the learned march run; with the floor at 20 dB its own image is
+    kept: every pixel written, close to the plain frame, fewer samples."""
     from voreen_amd import hipdvr as H, workloads as W
-    print("[case] learned baseline 560x136", flush=True)
+    print("[case] learned march 560x136, constant double-step network", flush=True)
     w, h = 560, 136
     llf, urb = W.world_box(scene["dims"])
     bufs = [ctx.alloc(w * h * 16) for _ in range(4)]
     de, dx, do, dl = bufs
     l = H.Learned(ctx, scene["v"])
+    m = H.Mlp(ctx, seed=5)
     try:
+        prm = m.params()
+        prm[-130:-2] = 0.0          # last layer's weights: outputs = its biases
+        prm[-2], prm[-1] = -50.0, 1.0   # no skip; log2 step multiple 1
+        m.set_params(prm)
         ctx.entry_exit_device(W.orbit_camera(8, 36, radius=1.6), llf, urb, w, h, de, dx)
         p = H.default_params(scene["step"], camera_pos=tuple(scene["cam"].eye))
-        ctx.render_device(scene["v"], scene["t"], de, dx, w, h, p, do)
+        st = ctx.render_device(scene["v"], scene["t"], de, dx, w, h, p, do, stats=True)
         plain = do.download((h, w, 4), np.float32)
-        l.prepare(scene["t"], p, None)
+        l.prepare(scene["t"], p, m)
         dl.upload(np.full((h, w, 4), np.nan, np.float32))
-        ls = l.render_device(scene["t"], de, dx, w, h, p, dl, stats=True)
+        ls = l.render_device(scene["t"], de, dx, w, h, p, dl, opts=H.learned_opts(psnr_floor=20.0), stats=True)
         img = dl.download((h, w, 4), np.float32)
     finally:
         l.close()
+        m.close()
         for b in bufs:
             b.free()
     assert not np.isnan(img).any()
     assert np.mean(plain[..., 3] > 0) > 0.2
     q = psnr(img, plain)
-    print(f"wide baseline: psnr {q:.1f} dB fell_back {ls['fell_back']}", flush=True)
-    assert q > 60.0 and not ls["fell_back"]
+    print(f"double step: psnr {q:.1f} dB, samples {ls['samples']} (plain {st['fetched']}), "
+          f"fell_back {ls['fell_back']}", flush=True)
+    assert not ls["fell_back"] and q > 30.0 and 0 < ls["samples"] < st["fetched"]
 
 
 def test_trained_network_gains_over_baseline(ctx, scene):

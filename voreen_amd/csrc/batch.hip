@@ -243,6 +243,7 @@ struct hipdvr_batch {
     // learned jobs (hipdvr_batch_set_learned), per launch position: the judge's
     // grid, sparse image, sum and flag
     std::vector<hipdvr_learned*> learned;
+    std::vector<char> plain_learned;   // given a learned state whose table cannot gain: marched plainly
     hipdvr_learned_opts lopts{};
     std::vector<hipdvr::JudgeGrid> grids;
     std::vector<float4*> d_sparse;
@@ -428,6 +429,7 @@ int hipdvr_batch_create(hipdvr_ctx* ctx, const hipdvr_batch_job* jobs, uint32_t 
             }
             b->d_counter = dev_alloc<uint32_t>(b, kXcds * kCounterStride * 4);
             b->learned.assign(n, nullptr);
+            b->plain_learned.assign(n, 0);
             hip_check(hipGetLastError(), "batch setup");
             hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         } catch (...) {
@@ -567,6 +569,8 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
         const size_t nj = b->jobs.size();
         // built aside and taken over only when every job is accepted
         std::vector<hipdvr_learned*> lv(nj, nullptr);
+        std::vector<char> plain_lv(nj, 0);
+        std::vector<const hipdvr_learned*> given(nj, nullptr);   // the state given for each job
         std::vector<JudgeGrid> grids(nj, JudgeGrid{});
         std::vector<float4*> sparse(nj, nullptr);
         std::vector<float*> sse(nj, nullptr);
@@ -585,8 +589,15 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
             if (b->shade == 2) fail(HIPDVR_ERR_UNSUPPORTED, "learned jobs shade with the approximate Phong path");
             // a learned state holds the opacity tables of one step
             for (size_t k = 0; k < i; ++k)
-                if (lv[k] == l && b->jobs[k].params.step != j.params.step)
+                if (given[k] == l && b->jobs[k].params.step != j.params.step)
                     fail(HIPDVR_ERR_INVALID, "jobs sharing a learned state must share the step");
+            given[i] = l;
+            if (!learned_gains(l)) {
+                // every occupied brick at the base step: the plain march
+                // gives the exact frame at a lower cost
+                plain_lv[i] = 1;
+                continue;
+            }
             for (int bs = 0; bs < kBrickSizes; ++bs) learned_stepping_args(l, &j.params, largs[bs][i]);
             lv[i] = l;
             grids[i] = learned_judge_grid(j.w, j.h, lopts);
@@ -681,6 +692,7 @@ int hipdvr_batch_set_learned(hipdvr_batch* b, hipdvr_learned* const* learned, ui
         b->n_judge = (uint32_t)judges.size();
         b->lopts = lopts;
         b->learned.swap(lv);
+        b->plain_learned.swap(plain_lv);
         b->grids.swap(grids);
         b->d_sparse.swap(sparse);
         b->d_sse.swap(sse);
@@ -747,7 +759,7 @@ int hipdvr_batch_launch_learned(hipdvr_batch* b, hipdvr_learned_stats* stats) {
                     s.sparse_psnr = mse > 0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.f;
                     s.fell_back = flag[i];
                 } else {
-                    s.sparse_psnr = 0.f;
+                    s.sparse_psnr = b->plain_learned[i] ? 999.f : 0.f;
                     s.fell_back = 0;
                 }
             }

@@ -43,6 +43,9 @@ void launch_table(const float* y, uint32_t rows, uint8_t* table, float skip_logi
                   hipStream_t st);
 void launch_fill_table(uint8_t* table, uint32_t rows, uint8_t v, hipStream_t st);
 void launch_merge_occ(uint8_t* table, const uint8_t* occ16, uint32_t rows, hipStream_t st);
+// count[0] = rows where the table departs from the plain march
+void launch_table_gain(const uint8_t* table, const uint8_t* occ16, uint32_t rows, unsigned int* count,
+                       hipStream_t st);
 void launch_trace(const LearnedArgs& la, float* err, unsigned int* visits, hipStream_t st);
 void launch_targets(const float* err, const unsigned int* visits, const float* feat, uint32_t rows,
                     float tau_skip, float tau_step, float* out_x, float* out_t, unsigned int* count,
@@ -63,6 +66,9 @@ hipdvr_learned_opts learned_options(const hipdvr_learned_opts* o);
 void learned_stepping_args(hipdvr_learned* l, const hipdvr_params* p, LearnedArgs& la);
 const hipdvr_volume* learned_volume(const hipdvr_learned* l);
 void learned_ensure_step(hipdvr_learned* l, float step);
+// whether the last prepared table departs from the plain march anywhere
+// (false: the learned frame would be the plain frame at a higher cost)
+bool learned_gains(const hipdvr_learned* l);
 const hipdvr_tf* learned_prepared_tf(const hipdvr_learned* l);
 void launch_judge(const float4* learned, int w, int h, const float4* sparse, int sw, int sh, int stride,
                   int block, int offs, float* sse, int* flag, float mse_max, hipStream_t st);

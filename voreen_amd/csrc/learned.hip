@@ -212,6 +212,7 @@ struct hipdvr_learned {
     std::vector<float> h_alpha_m;    // host copy (source of the last upload)
     float alpha_m_step = -1.f;       // step the tables were made for
     const hipdvr_tf* prepared_tf = nullptr;   // TF of the last hipdvr_learned_prepare
+    bool gains = true;               // the prepared table departs from the plain march
     float4* d_sparse = nullptr;      // sparse plain image
     size_t sparse_cap = 0;
     float4* d_seg = nullptr;         // its ray segments
@@ -229,6 +230,7 @@ namespace hipdvr {
 hipdvr_ctx* learned_ctx(const hipdvr_learned* l) { return l->ctx; }
 const hipdvr_volume* learned_volume(const hipdvr_learned* l) { return l->vol; }
 const hipdvr_tf* learned_prepared_tf(const hipdvr_learned* l) { return l->prepared_tf; }
+bool learned_gains(const hipdvr_learned* l) { return l->gains; }
 }  // namespace hipdvr
 
 namespace {
@@ -500,9 +502,13 @@ int hipdvr_learned_prepare(hipdvr_learned* l, const hipdvr_tf* tf, const hipdvr_
             launch_fill_table(l->d_table, l->rows, 0, st);
         }
         launch_merge_occ(l->d_table, l->d_occ16, l->rows, st);
+        launch_table_gain(l->d_table, l->d_occ16, l->rows, l->d_count, st);
+        unsigned int gain_rows = 0;
+        hip_check(hipMemcpyAsync(&gain_rows, l->d_count, 4, hipMemcpyDeviceToHost, st), "download");
         hip_check(hipGetLastError(), "learned prepare launch");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         l->prepared_tf = tf;
+        l->gains = gain_rows != 0;
     });
 }
 
@@ -583,6 +589,28 @@ int hipdvr_render_learned_device(hipdvr_learned* l, const hipdvr_tf* tf, const f
         const int shade = phong ? (prm->exact_shading ? 2 : 1) : 0;
         const bool count = stats != nullptr;
         if (count) launch_zero_u64(l->ctx->d_counters, kNumCounters, st);
+        if (!l->gains) {
+            // the table keeps every occupied brick at the base step: the
+            // learned frame could only approximate the plain frame, at a
+            // higher cost, so the plain frame is rendered (no judge needed)
+            const bool timed0 = ctx_timing_start(l->ctx);
+            ensure_cells(vol, st);
+            MarchArgs pa;
+            fill_march_args(pa, vol, tf, prm, d_entry, d_exit, d_out, w, h, count ? l->ctx->d_counters : nullptr);
+            launch_march_cell(pa, vol->desc.dtype, shade, count, vol->cells_wide, default_batch(shade), st);
+            hip_check(hipGetLastError(), "learned render launch");
+            if (timed0) ctx_timing_stop(l->ctx);
+            if (count) {
+                unsigned long long c[kNumCounters];
+                hip_check(hipMemcpyAsync(c, l->ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st), "download");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                stats->samples = c[2];   // reconstructed samples, as the learned march counts them
+                stats->shaded = c[3];
+                stats->sparse_psnr = 999.f;
+                stats->fell_back = 0;
+            }
+            return;
+        }
         const bool timed = ctx_timing_start(l->ctx);
         // 1. sparse plain re-march (the plain march of the judge's pixels,
         //    sparse_grid) on the side stream, beside 2. the learned frame

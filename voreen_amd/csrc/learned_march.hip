@@ -168,6 +168,28 @@ void launch_merge_occ(uint8_t* table, const uint8_t* occ16, uint32_t rows, hipSt
     hipLaunchKernelGGL(merge_occ_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, table, occ16, rows);
 }
 
+// rows where the table departs from the plain march: a step multiple above 1,
+// or a skip of a brick that may hold alpha > 0 (count[0], cleared first)
+__global__ void table_gain_kernel(const uint8_t* __restrict__ table, const uint8_t* __restrict__ occ16,
+                                  uint32_t rows, unsigned int* __restrict__ count) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    bool gain = false;
+    if (row < rows) {
+        const uint8_t e = table[row];
+        gain = (e & 0x80) ? occ16[row >> 3] != 0 : (e & 3) != 0;
+    }
+    const unsigned long long b = __ballot(gain);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (unsigned int)__popcll(b));
+}
+
+__global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
+
+void launch_table_gain(const uint8_t* table, const uint8_t* occ16, uint32_t rows, unsigned int* count,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, st, count);
+    hipLaunchKernelGGL(table_gain_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, table, occ16, rows, count);
+}
+
 // ---------------------------------------------------------------------------
 // shared march helpers (linear filter, tiled volume, TF classification)
 // ---------------------------------------------------------------------------
