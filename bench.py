@@ -306,7 +306,7 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
             b = H.Batch(ctx, [(v, t, de, dx, o, wimg, himg, p) for (de, dx, _), p, o in zip(bufs, params, dl)])
             try:
                 c = batched["config"]
-                b.set_config(c["tile"], c["brick_log2"], c["batch"])
+                b.set_config(c["tile"], c["brick_log2"], c["batch"], c.get("resident", 0))
                 b.set_learned([lr] * nviews)
                 bst = b.launch_learned(stats=True)
                 for _ in range(max(args.warmup, 1)):
@@ -432,7 +432,7 @@ def main_batch(args):
         "config": {"workload": f"c2: {len(jobs)} uneven volumes (64..384 per axis, u8/u16), 512x512 each, "
                                "one persistent launch from a captured hipGraph per step",
                    "autotuned": {"tile": ["8x8", "16x4", "32x2"][tuned[0]], "brick": 1 << tuned[1],
-                                 "batch": tuned[2], "ms": tuned_ms}},
+                                 "batch": tuned[2], "resident_per_cu": tuned[3], "ms": tuned_ms}},
         "gsamples_per_s": round(st["samples"] * args.steps / elapsed / 1e9, 4),
         "samples_per_batch": st["samples"], "fetched_per_batch": st["fetched"],
         "kernel_ms_avg": round(sum(launch_ms) / len(launch_ms), 4),
@@ -486,7 +486,8 @@ def batched_views_leg(ctx, v, t, bufs, params, wimg, himg, args, bytes_orbit):
     launch_s = sum(ms) / len(ms) / 1e3
     achieved = bytes_orbit / launch_s / 1e9
     return {"fps": round(nviews * args.steps / elapsed, 3), "ms_per_launch": round(launch_s * 1e3, 4),
-            "views_per_launch": nviews, "config": {"tile": cfg[0], "brick_log2": cfg[1], "batch": cfg[2]},
+            "views_per_launch": nviews,
+            "config": {"tile": cfg[0], "brick_log2": cfg[1], "batch": cfg[2], "resident": cfg[3]},
             "equals_single_frames": bool(equal),
             "roofline": {"achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4)}}
 

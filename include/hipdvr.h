@@ -73,7 +73,7 @@
 extern "C" {
 #endif
 
-#define HIPDVR_ABI_VERSION 1
+#define HIPDVR_ABI_VERSION 2
 
 /* status codes */
 enum {

@@ -45,6 +45,9 @@ typedef struct hipdvr_batch_config {
     int32_t tile;           /* HIPDVR_TILE_* */
     int32_t brick_log2;     /* empty-space bricks of 8^3, 16^3, 32^3: 3, 4, 5 */
     int32_t batch;          /* samples fetched together per ray: 2 or 4 */
+    int32_t resident;       /* persistent workgroups per CU, 1..8 (at most what fits); 0: three
+                               when a job's cell records need 64-bit addressing (their caches
+                               thrash sooner), else as many as fit, at most five */
 } hipdvr_batch_config;
 
 /* Jobs are copied; the volumes, TFs and buffers must outlive the batch.
@@ -55,8 +58,10 @@ int hipdvr_batch_destroy(hipdvr_batch* b);
 /* select a configuration (drops a captured graph) */
 int hipdvr_batch_set_config(hipdvr_batch* b, const hipdvr_batch_config* cfg);
 int hipdvr_batch_get_config(const hipdvr_batch* b, hipdvr_batch_config* cfg);
-/* Time every configuration over reps launches each (HIP events), keep the
- * fastest; best_ms (may be NULL) receives its mean launch time. */
+/* Time every tile shape, brick size and batch over reps launches each (HIP
+ * events) at the default residency, then 3, 4 and 5 resident workgroups per
+ * CU with the fastest; keep the fastest; best_ms (may be NULL) receives its
+ * mean launch time. */
 int hipdvr_batch_autotune(hipdvr_batch* b, uint32_t reps, hipdvr_batch_config* best, float* best_ms);
 /* Capture one batched launch (queue reset + persistent march) as a hipGraph
  * on the context stream; hipdvr_batch_launch then replays it. */

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared if not hasattr(lib, n)]
     assert not missing, missing
     assert set(declared) == set(H.EXPORTS)
-    assert lib.hipdvr_abi_version() == 1
+    assert lib.hipdvr_abi_version() == 2
 
 
 def test_library_was_built_from_these_sources():

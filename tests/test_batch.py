@@ -71,6 +71,19 @@ def test_batch_equals_single_frames_for_every_config(ctx, jobs, oracle):
                     for i, (o, r) in enumerate(zip(outputs(jobs), ref)):
                         assert np.array_equal(o, r), f"job {i} tile {tile} bl {bl} k {k}"
                     assert st["samples"] == total
+        # resident workgroups per CU: the same images whatever the residency
+        for r in (1, 3, 8):
+            say(f"batch resident {r}")
+            b.set_config(0, 4, 2, r)
+            clear(ctx, jobs)
+            st = b.launch(stats=True)
+            for i, (o, ref_i) in enumerate(zip(outputs(jobs), ref)):
+                assert np.array_equal(o, ref_i), f"job {i} resident {r}"
+            assert st["samples"] == total
+        for r in (-1, 9):
+            with pytest.raises(H.HipDvrError) as e:
+                b.set_config(0, 4, 2, r)
+            assert e.value.code == H.ERR_INVALID
     finally:
         b.close()
     # one job against the oracle as well

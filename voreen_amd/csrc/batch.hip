@@ -260,7 +260,7 @@ struct hipdvr_batch {
     uint32_t* d_tile_start[kTileShapes] = {nullptr, nullptr, nullptr};
     uint32_t total_tiles[kTileShapes] = {0, 0, 0};
     uint32_t* d_counter = nullptr;
-    hipdvr_batch_config cfg{0, 4, 4};
+    hipdvr_batch_config cfg{0, 4, 4, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -293,6 +293,7 @@ void validate_config(const hipdvr_batch_config& c) {
     if (c.tile < HIPDVR_TILE_8x8 || c.tile > HIPDVR_TILE_32x2) fail(HIPDVR_ERR_INVALID, "unknown tile shape");
     if (c.brick_log2 < 3 || c.brick_log2 > 5) fail(HIPDVR_ERR_INVALID, "brick_log2 must be 3, 4 or 5");
     if (c.batch != 2 && c.batch != 4) fail(HIPDVR_ERR_INVALID, "batch must be 2 or 4");
+    if (c.resident < 0 || c.resident > 8) fail(HIPDVR_ERR_INVALID, "resident must be 0 (default) or 1..8");
 }
 
 void launch_batch(hipdvr_batch* b, bool count, bool learned = false) {
@@ -302,10 +303,13 @@ void launch_batch(hipdvr_batch* b, bool count, bool learned = false) {
     int per_cu = 0;
     hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 256, 0), "occupancy query");
     if (per_cu < 1) per_cu = 1;
-    // at most five resident workgroups per CU: more rays in flight evict each
-    // other's cell records (64 uneven volumes, unshaded: 59,030 frames/s at
-    // the occupancy limit, 63,436 at five, 62,716 at four)
-    if (per_cu > kMaxResidentPerCu) per_cu = kMaxResidentPerCu;
+    // resident workgroups per CU: more rays in flight evict each other's cell
+    // records.  Default: at most five (64 uneven volumes, unshaded: 59,030
+    // frames/s at the occupancy limit, 63,436 at five, 62,716 at four), three
+    // when records need 64-bit addressing (8 GiB volume, 8 views of 4K with
+    // Phong: 234 ms at three against 277 ms at four)
+    const int cap = b->cfg.resident > 0 ? b->cfg.resident : (b->wide ? 3 : kMaxResidentPerCu);
+    if (per_cu > cap) per_cu = cap;
     const uint32_t total = learned ? b->ltotal_tiles[t] : b->total_tiles[t];
     uint32_t grid = (uint32_t)(b->sms * per_cu);
     if (grid > total) grid = total ? total : 1;
@@ -527,24 +531,36 @@ int hipdvr_batch_autotune(hipdvr_batch* b, uint32_t reps, hipdvr_batch_config* b
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         hipdvr_batch_config win = b->cfg;
         float win_ms = 1e30f;
+        // mean launch time of configuration c over reps launches after a warm-up
+        auto time_config = [&](const hipdvr_batch_config& c) {
+            b->cfg = c;
+            launch_batch(b, false);
+            hip_check(hipEventRecord(e0, st), "hipEventRecord");
+            for (uint32_t r = 0; r < reps; ++r) launch_batch(b, false);
+            hip_check(hipEventRecord(e1, st), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            return ms / (float)reps;
+        };
         try {
             for (int t = 0; t < kTileShapes; ++t)
                 for (int bl = 3; bl <= 5; ++bl)
                     for (int k : {2, 4}) {
-                        b->cfg = hipdvr_batch_config{t, bl, k};
-                        launch_batch(b, false);   // warm-up
-                        hip_check(hipEventRecord(e0, st), "hipEventRecord");
-                        for (uint32_t r = 0; r < reps; ++r) launch_batch(b, false);
-                        hip_check(hipEventRecord(e1, st), "hipEventRecord");
-                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                        float ms = 0.f;
-                        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                        ms /= (float)reps;
+                        const float ms = time_config(hipdvr_batch_config{t, bl, k, 0});
                         if (ms < win_ms) {
                             win_ms = ms;
                             win = b->cfg;
                         }
                     }
+            const hipdvr_batch_config shape = win;
+            for (int r : {3, 4, 5}) {
+                const float ms = time_config(hipdvr_batch_config{shape.tile, shape.brick_log2, shape.batch, r});
+                if (ms < win_ms) {
+                    win_ms = ms;
+                    win = b->cfg;
+                }
+            }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);

@@ -122,7 +122,7 @@ class BatchJob(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("tile", C.c_int32), ("brick_log2", C.c_int32), ("batch", C.c_int32)]
+    _fields_ = [("tile", C.c_int32), ("brick_log2", C.c_int32), ("batch", C.c_int32), ("resident", C.c_int32)]
 
 
 TILE_8x8, TILE_16x4, TILE_32x2 = 0, 1, 2
@@ -765,20 +765,20 @@ class Batch:
             self.ctx.lib.hipdvr_batch_destroy(self.h)
             self.h = C.c_void_p()
 
-    def set_config(self, tile, brick_log2, batch):
-        c = BatchConfig(tile, brick_log2, batch)
+    def set_config(self, tile, brick_log2, batch, resident=0):
+        c = BatchConfig(tile, brick_log2, batch, resident)
         self.ctx.check(self.ctx.lib.hipdvr_batch_set_config(self.h, C.byref(c)), "hipdvr_batch_set_config")
 
     def config(self):
         c = BatchConfig()
         self.ctx.check(self.ctx.lib.hipdvr_batch_get_config(self.h, C.byref(c)), "hipdvr_batch_get_config")
-        return (c.tile, c.brick_log2, c.batch)
+        return (c.tile, c.brick_log2, c.batch, c.resident)
 
     def autotune(self, reps=3):
         c, ms = BatchConfig(), C.c_float()
         self.ctx.check(self.ctx.lib.hipdvr_batch_autotune(self.h, reps, C.byref(c), C.byref(ms)),
                        "hipdvr_batch_autotune")
-        return (c.tile, c.brick_log2, c.batch), ms.value
+        return (c.tile, c.brick_log2, c.batch, c.resident), ms.value
 
     def capture(self):
         self.ctx.check(self.ctx.lib.hipdvr_batch_capture(self.h), "hipdvr_batch_capture")
