@@ -43,6 +43,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, a wave64
+# instruction every 4 clocks on a 16-lane SIMD
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -113,8 +116,9 @@ def _read_counters(outdir):
 
 
 def pmc_leg(args, views=4, c4=False):
-    """This run's HBM traffic and L2 hit rate of the march kernel: rocprofv3
-    counter passes (FETCH_SIZE; WRITE_SIZE; TCC_HIT + TCC_MISS) over a child run
+    """This run's HBM traffic, L2 hit rate and VALU instruction count of the
+    march kernel: rocprofv3 counter passes (FETCH_SIZE; WRITE_SIZE; TCC_HIT +
+    TCC_MISS; SQ_INSTS_VALU) over a child run
     of this bench on `views` views of the same workload (c4: one eager launch
     of the batched kernel over all views), each byte counter scaled by what it
     reads for a plain 1 GiB copy in the same child.  Runs before this process
@@ -140,7 +144,8 @@ def pmc_leg(args, views=4, c4=False):
         child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--kernel", args.kernel,
                  "--shading", args.shading, "--brick-log2", str(args.brick_log2), "--batch", str(args.batch),
                  "--views", str(views), "--no-host", "--no-batched", *common]
-    passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"], "tcc": ["TCC_HIT_sum", "TCC_MISS_sum"]}
+    passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"], "tcc": ["TCC_HIT_sum", "TCC_MISS_sum"],
+              "valu": ["SQ_INSTS_VALU"]}
     got = {}
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     for name, counters in passes.items():
@@ -172,6 +177,7 @@ def pmc_leg(args, views=4, c4=False):
     write = per_dispatch(got["write"], "WRITE_SIZE", kname)[-views:]
     hit = per_dispatch(got["tcc"], "TCC_HIT", kname)[-views:]
     miss = per_dispatch(got["tcc"], "TCC_MISS", kname)[-views:]
+    valu = per_dispatch(got["valu"], "SQ_INSTS_VALU", kname)[-views:]
     cf = per_dispatch(got["fetch"], "FETCH_SIZE", "copy16_kernel")
     cw = per_dispatch(got["write"], "WRITE_SIZE", "copy16_kernel")
     if not (fetch and write and hit and miss and cf and cw):
@@ -182,6 +188,7 @@ def pmc_leg(args, views=4, c4=False):
     what = ("one eager batched launch of all views" if kname.startswith("batch") else
             f"views 0..{views - 1} of the orbit")
     return {"traffic_bytes": int(traffic), "l2_hit_rate": round(float(np.sum(hit) / (np.sum(hit) + np.sum(miss))), 4),
+            "valu_insts": float(np.mean(valu)) if valu else None,
             "fetch_factor": round(f_fac, 3), "write_factor": round(w_fac, 3),
             "source": f"rocprofv3 --pmc in this run: {what}, FETCH_SIZE/WRITE_SIZE "
                       "(KiB per dispatch) scaled by a 1 GiB copy kernel, TCC_HIT/TCC_MISS summed over channels"}
@@ -606,6 +613,10 @@ def main_c4(args):
         out["roofline"]["traffic_source"] = pmc["source"]
         out["roofline"]["algorithmic_bytes_per_launch"] = int(bytes_step)
         out["roofline"]["reuse"] = round(bytes_step / pmc["traffic_bytes"], 3)
+        if pmc.get("valu_insts"):
+            out["roofline"]["valu_insts_per_launch"] = int(pmc["valu_insts"])
+            out["roofline"]["valu_issue_frac"] = round(
+                pmc["valu_insts"] / (sum(launch_ms) / len(launch_ms) / 1e3 * VALU_ISSUE_PEAK), 4)
     assert st["samples"] == samples and st["fetched"] == fetched and st["shaded"] == shaded
     for o in outs:
         o.free()
@@ -878,6 +889,11 @@ def main():
     if pmc:
         out["roofline"]["l2_hit_rate"] = pmc["l2_hit_rate"]
         out["roofline"]["counter_calibration"] = {"fetch": pmc["fetch_factor"], "write": pmc["write_factor"]}
+        if pmc.get("valu_insts"):
+            # the kernel's VALU wave instructions per launch over what the
+            # chip can issue in the launch's time: the bound of this kernel
+            out["roofline"]["valu_insts_per_launch"] = int(pmc["valu_insts"])
+            out["roofline"]["valu_issue_frac"] = round(pmc["valu_insts"] / (kernel_ms / 1e3 * VALU_ISSUE_PEAK), 4)
     if traffic:
         out["roofline"]["traffic_source"] = traffic["source"]
         out["roofline"]["algorithmic_bytes_per_launch"] = round(per_launch_bytes)
