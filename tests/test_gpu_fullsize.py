@@ -187,6 +187,8 @@ def test_c3_learned_36_views_at_45_db_and_gain(ctx, c1):
     print(f"c3 trained: {info}", flush=True)
     lr = H.Learned(ctx, c1["v"])
     dl = ctx.alloc(c1["w"] * c1["h"] * 16)
+    BATCHED_VIEWS = (0, 6, 12, 18, 24, 30)
+    single = {}
     try:
         psnrs, learned_samples, base_samples, plain_samples = [], 0, 0, 0
         t_learned = t_base = t_plain = 0.0
@@ -209,7 +211,35 @@ def test_c3_learned_36_views_at_45_db_and_gain(ctx, c1):
             assert ls["fell_back"] == 0, f"view {view} fell back (sparse {ls['sparse_psnr']:.1f} dB)"
             learned_samples += ls["samples"]
             plain_samples += stp["fetched"]
+            if view in BATCHED_VIEWS:
+                single[view] = learned
+        # six of the views in one batched launch with learned stepping: the
+        # same images as their single-view learned frames
+        say("c3 batched learned launch of six views")
+        bufs = {v_: [ctx.alloc(c1["w"] * c1["h"] * 16) for _ in range(3)] for v_ in BATCHED_VIEWS}
+        b = None
+        try:
+            jobs = []
+            for v_ in BATCHED_VIEWS:
+                cam = W.orbit_camera(v_, c1["nviews"])
+                de, dx, do = bufs[v_]
+                ctx.entry_exit_device(cam, c1["llf"], c1["urb"], c1["w"], c1["h"], de, dx)
+                jobs.append((c1["v"], c1["t"], de, dx, do, c1["w"], c1["h"],
+                             H.default_params(c1["step"], shading=H.SHADE_PHONG, camera_pos=tuple(cam.eye))))
+            b = H.Batch(ctx, jobs)
+            b.set_learned([lr] * len(jobs))
+            bst = b.launch_learned(stats=True)
+            for v_, st_ in zip(BATCHED_VIEWS, bst):
+                img = bufs[v_][2].download((c1["h"], c1["w"], 4), np.float32)
+                assert st_["fell_back"] == 0 and np.array_equal(img, single[v_]), f"batched view {v_}"
+        finally:
+            if b is not None:
+                b.close()
+            for bb in bufs.values():
+                for x in bb:
+                    x.free()
         # the method without the network: occupancy skipping only, base step
+        # (a table that cannot gain: the plain frame itself)
         lr.prepare(c1["t"], p, None)
         for view in range(c1["nviews"]):
             cam = W.orbit_camera(view, c1["nviews"])
