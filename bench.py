@@ -76,7 +76,9 @@ def parse():
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (hipdvr_render) leg")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate) of this run")
-    ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch eagerly (no captured graph)")
+    ap.add_argument("--no-graph", action="store_true", help="c2, c4: launch the batch eagerly (no captured graph)")
+    ap.add_argument("--c2-config", default="",
+                    help="c2: tile,brick_log2,batch,resident instead of autotuning (the counter passes' child)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the leg that renders all views of a step in one batched launch")
     return ap.parse_args()
@@ -115,21 +117,29 @@ def _read_counters(outdir):
     return rows
 
 
-def pmc_leg(args, views=4, c4=False):
+def pmc_leg(args, views=4, c4=False, c2=None):
     """This run's HBM traffic, L2 hit rate and VALU instruction count of the
     march kernel: rocprofv3 counter passes (FETCH_SIZE; WRITE_SIZE; TCC_HIT +
     TCC_MISS; SQ_INSTS_VALU) over a child run
     of this bench on `views` views of the same workload (c4: one eager launch
-    of the batched kernel over all views), each byte counter scaled by what it
-    reads for a plain 1 GiB copy in the same child.  Runs before this process
-    touches the GPU; None when rocprofv3 is missing or a pass fails."""
+    of the batched kernel over all views; c2: one eager launch of the batch
+    of volumes in the configuration `c2` the parent tuned), each byte counter
+    scaled by what it reads for a plain 1 GiB copy in the same child.  Runs
+    before this process touches the GPU (c2: after it has released it); None
+    when rocprofv3 is missing or a pass fails."""
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(exe):
         return None
     common = ["--steps", "1", "--warmup", "0", "--no-cpu", "--no-learned", "--no-pmc", "--copy-calibration", "1024"]
     from voreen_amd import workloads as W   # host-side tables only; no GPU
-    nviews = args.views or W.CONFIGS[args.config]["views"]
-    if c4:
+    nviews = args.views or W.CONFIGS.get(args.config, {}).get("views", 1)
+    if c2 is not None:
+        # the batch of 64 volumes in the configuration the parent tuned,
+        # launched eagerly once (the child's last batch_kernel dispatch)
+        kname, views, limit = "batch_kernel<", 1, 300
+        child = [sys.executable, os.path.abspath(__file__), "--config", "c2", "--jobs", str(args.jobs),
+                 "--no-graph", "--c2-config", ",".join(str(x) for x in c2), *common]
+    elif c4:
         kname, views, limit = "batch_kernel<", 1, 420
         child = [sys.executable, os.path.abspath(__file__), "--config", "c4", "--brick-log2", str(args.brick_log2),
                  "--no-graph", *common]
@@ -185,7 +195,8 @@ def pmc_leg(args, views=4, c4=False):
     gib_kib = 1024.0 * 1024.0
     f_fac, w_fac = gib_kib / cf[-1], gib_kib / cw[-1]
     traffic = (np.mean(fetch) * f_fac + np.mean(write) * w_fac) * 1024.0
-    what = ("one eager batched launch of all views" if kname.startswith("batch") else
+    what = (f"one eager batched launch of {args.jobs} volumes" if c2 is not None else
+            "one eager batched launch of all views" if kname.startswith("batch") else
             f"views 0..{views - 1} of the orbit")
     return {"traffic_bytes": int(traffic), "l2_hit_rate": round(float(np.sum(hit) / (np.sum(hit) + np.sum(miss))), 4),
             "valu_insts": float(np.mean(valu)) if valu else None,
@@ -411,7 +422,11 @@ def main_batch(args):
         res += [v, t, de, dx, do]
         jobs.append((v, t, de, dx, do, w, h, p))
     b = H.Batch(ctx, jobs)
-    tuned, tuned_ms = (b.config(), None) if args.no_autotune else b.autotune(reps=2)
+    if args.c2_config:
+        b.set_config(*(int(x) for x in args.c2_config.split(",")))
+        tuned, tuned_ms = b.config(), None
+    else:
+        tuned, tuned_ms = (b.config(), None) if args.no_autotune else b.autotune(reps=2)
     st = b.launch(stats=True)
     if not args.no_graph:
         b.capture()
@@ -447,10 +462,30 @@ def main_batch(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "setup_s": round(setup_s, 2),
     }
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        ca, cb = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_device(cb, ca, nb)
+        ctx.synchronize()
+        ca.free()
+        cb.free()
     b.close()
     for r in res:
         (r.free if hasattr(r, "free") else r.close)()
     ctx.close()
+    # counter passes of the tuned configuration: a child per pass, after this
+    # process has released the GPU
+    pmc = None if (args.no_pmc or args.c2_config) else pmc_leg(args, c2=tuned)
+    if pmc:
+        kms = sum(launch_ms) / len(launch_ms)
+        out["roofline"].update({
+            "traffic": pmc["traffic_bytes"], "l2_hit_rate": pmc["l2_hit_rate"],
+            "counter_calibration": {"fetch": pmc["fetch_factor"], "write": pmc["write_factor"]},
+            "traffic_source": pmc["source"], "algorithmic_bytes_per_launch": int(bytes_batch),
+            "reuse": round(bytes_batch / pmc["traffic_bytes"], 3)})
+        if pmc.get("valu_insts"):
+            out["roofline"]["valu_insts_per_launch"] = int(pmc["valu_insts"])
+            out["roofline"]["valu_issue_frac"] = round(pmc["valu_insts"] / (kms / 1e3 * VALU_ISSUE_PEAK), 4)
     print(json.dumps(out), flush=True)
 
 
