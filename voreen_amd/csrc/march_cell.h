@@ -273,6 +273,15 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
     uint32_t cur_brick = 0xffffffffu;
     bool cur_occ = false;
     float t_lim = 0.0f;
+    // reciprocals for the empty-brick bound (a bound with a 0.05-voxel
+    // margin needs no correctly rounded quotient); an axis with |dir| below
+    // 1e-30 cannot move 0.05 voxel within a ray and bounds nothing
+    float rdir[3], rfd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        rdir[i] = fabsf(dir[i]) < 1e-30f ? 0.0f : __builtin_amdgcn_rcpf(dir[i]);
+        rfd[i] = __builtin_amdgcn_rcpf(a.fdims[i]);
+    }
     bool finished = false;
     while (!finished && loop < cap) {
         const float s[3] = {first[0] + t * dir[0], first[1] + t * dir[1], first[2] + t * dir[2]};
@@ -285,7 +294,7 @@ __device__ __forceinline__ bool cell_march(const MarchArgs& a, const CellFetch<T
             cur_brick = brick;
             // occupancy bits staged in LDS when they fit, else the byte table
             cur_occ = s_occ ? ((s_occ[brick >> 5] >> (brick & 31)) & 1u) != 0 : a.brick_occ[brick] != 0;
-            if (!cur_occ) t_lim = empty_brick_limit(a, first, dir, vp, bl);
+            if (!cur_occ) t_lim = empty_brick_limit_rcp(a, first, rdir, rfd, vp, bl);
         }
         if (!cur_occ) {
             // every sample at t' < t_lim lies in this empty brick: only

@@ -484,6 +484,28 @@ __device__ __forceinline__ float empty_brick_limit(const MarchArgs& a, const flo
     return tl;
 }
 
+// empty_brick_limit from the reciprocals rdir = 1/dir (0: the axis bounds
+// nothing) and rfd = 1/fdims, each within a few ulp: the limit moves by far
+// less than the 0.05-voxel margin.
+__device__ __forceinline__ float empty_brick_limit_rcp(const MarchArgs& a, const float first[3],
+                                                       const float rdir[3], const float rfd[3],
+                                                       const float vp[3], uint32_t bl) {
+    float tl = INFINITY;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t b = (uint32_t)vp[i] >> bl;
+        const float B = (float)(1u << bl);
+        if (rdir[i] > 0.0f && b + 1 < a.nbricks[i]) {
+            const float vb = (float)(b + 1) * B - 0.05f;
+            tl = fminf(tl, ((vb + 0.5f) * rfd[i] - first[i]) * rdir[i]);
+        } else if (rdir[i] < 0.0f && b > 0) {
+            const float vb = (float)b * B + 0.05f;
+            tl = fminf(tl, ((vb + 0.5f) * rfd[i] - first[i]) * rdir[i]);
+        }
+    }
+    return tl;
+}
+
 // 64-lane maximum of a per-lane counter, times 64, added once per wavefront.
 __device__ __forceinline__ void wave_add_max64(unsigned long long* dst, unsigned long long v) {
 #pragma unroll

@@ -14,7 +14,9 @@ namespace hipdvr {
 // to even, would alternate the increment; it is excluded by requiring the
 // second increment to equal the first.  Returns a number of steps j >= 0 that
 // may be taken as t + j*d: every t_k, k <= j, stays below bound and within
-// cap_left - 1 steps, with margins of a few steps for the float estimates.
+// cap_left - 1 steps, with margins of a few steps for the float estimates
+// (the two step counts are formed with one reciprocal of d: within 1 ulp of
+// the quotients, well inside the margins for counts up to 10^6).
 // The product and the sum are exact because t + j*d < 2^(e+1) lies on the
 // binade's ulp grid.
 __host__ __device__ inline int closed_form_steps(float t, float step, float bound, int cap_left, float& d) {
@@ -23,8 +25,9 @@ __host__ __device__ inline int closed_form_steps(float t, float step, float boun
     d = (t + step) - t;
     const float t1 = t + d;
     if (((t1 + step) - t1) != d) return 0;
-    const float room = (B - step - t) / d;      // steps k with t_k + step < B: k < room
-    const float est = (bound - t) / d;          // steps k with t_k < bound: k < est
+    const float rd = 1.0f / d;
+    const float room = (B - step - t) * rd;     // steps k with t_k + step < B: k < room
+    const float est = (bound - t) * rd;         // steps k with t_k < bound: k < est
     if (!(room > 6.0f) || !(est > 6.0f)) return 0;
     const int kr = room > 1.0e6f ? 1000000 : (int)room - 3;
     const int ke = est > 1.0e6f ? 1000000 : (int)est - 3;
