@@ -43,8 +43,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, a wave64
-# instruction every 4 clocks on a 16-lane SIMD
+# wave64 VALU instructions per second in a model of one wave64 instruction
+# every 4 clocks per 16-lane SIMD (256 CUs x 4 SIMDs x 2.4 GHz).  A reference
+# rate, not a ceiling: the chip sustains more on some instruction mixes (the
+# unshaded 512^3 march measures 1.13 of it), so roofline.valu_issue_frac is
+# the launch's VALU instruction rate relative to this model.
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
