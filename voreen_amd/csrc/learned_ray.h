@@ -123,10 +123,10 @@ __device__ __forceinline__ float4 learned_ray(const LearnedArgs& la, const CellF
             mmax = 1 << (ent & 3);
         }
         if (skip) {
-            // skip to the last step-grid point inside the brick; the
-            // next pass finds the following brick one step on
+            // skip to the first step-grid point past the brick (a point that
+            // rounding leaves inside it is stepped on by the next pass)
             const float k = floorf((t_exit - t) / a.step);
-            t = t + fmaxf(k, 1.f) * a.step;
+            t = t + (fmaxf(k, 0.f) + 1.f) * a.step;
             ++guard;
             continue;
         }
