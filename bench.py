@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-stepping leg")
     ap.add_argument("--learned-tfs", type=int, default=200, help="random TFs traced for training")
     ap.add_argument("--learned-views", type=int, default=16, help="orbit views traced per TF")
+    ap.add_argument("--learned-tau", default="",
+                    help="SKIP,STEP error thresholds of the training labels (default: the library's 3e-3,8e-3)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
@@ -285,6 +287,14 @@ def cpu_baseline(vol, keys, tf, en, ex, step, samples_per_frame, target_s):
     }
 
 
+def tau_args(args):
+    """stepper.train keyword arguments for --learned-tau (none: its defaults)."""
+    if not args.learned_tau:
+        return {}
+    ts, tt = (float(x) for x in args.learned_tau.split(","))
+    return {"tau_skip": ts, "tau_step": tt}
+
+
 def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samples, args, batched=None):
     """Learned stepping on the same frames: train the stepping network on
     plain traces of seeded random TFs (the library's own kernels), then time
@@ -296,7 +306,7 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
     plain batched launch; the speedup is then the batched one's."""
     from voreen_amd import hipdvr as H, stepper
     t0 = time.perf_counter()
-    mlp, info = stepper.train(ctx, v, dims, n_tfs=args.learned_tfs, n_views=args.learned_views)
+    mlp, info = stepper.train(ctx, v, dims, n_tfs=args.learned_tfs, n_views=args.learned_views, **tau_args(args))
     train_s = time.perf_counter() - t0
     lr = H.Learned(ctx, v)
     nviews = len(bufs)
@@ -368,7 +378,7 @@ def learned_leg(ctx, v, t, dims, bufs, params, wimg, himg, plain_fps, plain_samp
         "reconstructed_per_frame": int(sum(s["samples"] for s in per_view) / nviews),
         "train": {"tfs": info["tfs"], "views": info["views"], "rows": info["rows"], "steps": info["steps"],
                   "loss_first": round(info["loss_first"], 4), "loss_last": round(info["loss_last"], 4),
-                  "seconds": round(train_s, 2)},
+                  "seconds": round(train_s, 2), "tau_skip_step": args.learned_tau or "3e-3,8e-3"},
     }
     if bres:
         # the headline comparison: the same kind of launch on both sides
@@ -676,7 +686,8 @@ def main_c4(args):
         tv = ctx.volume_device(tb, tdims, H.U16, spacing=tsp, offset=toff)
         tb.free()
         t0 = time.perf_counter()
-        mlp, info = stepper.train(ctx, tv, tdims, n_tfs=args.learned_tfs, n_views=args.learned_views)
+        mlp, info = stepper.train(ctx, tv, tdims, n_tfs=args.learned_tfs, n_views=args.learned_views,
+                                  **tau_args(args))
         train_s = time.perf_counter() - t0
         tv.close()
         progress(f"network trained on {tdims}: {train_s:.1f} s")
