@@ -745,7 +745,8 @@ def main_c4(args):
                                       "equals_single_view_learned": bool(b_equal),
                                       "reconstructed_per_launch": int(bst[0]["samples"])},
                           "train": {"volume": "512^3 turbulence, same generator", "tfs": info["tfs"],
-                                    "views": info["views"], "rows": info["rows"], "seconds": round(train_s, 1)}}
+                                    "views": info["views"], "rows": info["rows"], "seconds": round(train_s, 1),
+                                    "tau_skip_step": args.learned_tau or "3e-3,8e-3"}}
 
     if not args.no_cpu:
         # every 32nd pixel of view 0, unshaded, against the oracle on the downloaded volume
