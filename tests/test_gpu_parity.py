@@ -418,6 +418,48 @@ def test_edge_cases(ctx, oracle, kernel):
         assert report(f"tf2 {dom}", r, oracle).max() <= TOL
 
 
+def test_axis_aligned_rays_through_empty_bricks(ctx, oracle, kernel):
+    """Rays along +-x, +-y, +-z across runs of empty bricks: exactly on an
+    axis, with direction components of 1e-35 (below the 1e-30 under which an
+    axis bounds no empty run), 1e-6 off the axis, and slightly oblique.  The
+    empty-space bounds may skip only samples the oracle also finds
+    transparent: every kernel equals the oracle bit for bit."""
+    rng = np.random.default_rng(17)
+    vol = np.zeros((72, 80, 96), np.uint16)             # z, y, x
+    for z0, y0, x0 in ((0, 10, 20), (30, 50, 60), (46, 5, 70)):
+        vol[z0:z0 + 24, y0:y0 + 28, x0:x0 + 22] = rng.integers(40000, 65536, (24, 28, 22), dtype=np.uint16)
+    tf = np.zeros((4096, 4), np.uint8)
+    tf[1500:] = (220, 140, 60, 90)                      # intensity 0: transparent, empty bricks
+    us = np.linspace(0.03, 0.97, 16, dtype=np.float32)
+    rows_en, rows_ex = [], []
+    for axis in range(3):
+        for sign in (1, -1):
+            lo, hi = (0.0, 1.0) if sign > 0 else (1.0, 0.0)
+            others = [i for i in range(3) if i != axis]
+            for kind in ("exact", "denormal", "near", "oblique"):
+                en = np.zeros((16, 4), np.float32)
+                ex = np.zeros((16, 4), np.float32)
+                en[:, 3] = ex[:, 3] = 1.0
+                en[:, axis], ex[:, axis] = lo, hi
+                for j, o in enumerate(others):
+                    if kind == "denormal":
+                        en[:, o], ex[:, o] = 0.0, 1e-35     # on the volume's face, |dir| ~ 1e-35
+                    else:
+                        d = {"exact": 0.0, "near": 1e-6, "oblique": 3e-3 * (1 + j)}[kind]
+                        en[:, o] = us if j == 0 else us[::-1]
+                        ex[:, o] = en[:, o] + np.float32(d)
+                rows_en.append(en)
+                rows_ex.append(ex)
+    en = np.ascontiguousarray(np.stack(rows_en))
+    ex = np.ascontiguousarray(np.stack(rows_ex))
+    say(f"axis-aligned rays through empty bricks, kernel {kernel}")
+    r = render_both(ctx, oracle, vol, tf, (0.0, 1.0), en, ex, 1 / 192, kernel=kernel)
+    report("axis-aligned", r, oracle)
+    assert np.array_equal(r["gpu"], r["cpu"])
+    assert r["stats"]["samples"] == r["n"]
+    assert np.count_nonzero(r["cpu"][..., 3]) > 0        # some rays meet the blobs
+
+
 MODES = [("mip", 1, 0.0), ("mop", 2, 0.0), ("mida", 3, 0.0), ("mida", 3, -0.5), ("mida", 3, 0.6),
          ("iso", 4, 0.0), ("fhp", 5, 0.0), ("fhn", 6, 0.0)]
 
